@@ -1,0 +1,205 @@
+/*
+ * lekf.h -- C ABI of the MI355X LETKF local-analysis library (liblekf_hip.so).
+ *
+ * Drop-in boundary for the reference's LETKF hot path:
+ *   letkf_core  common/common_letkf.f90:50-243   -> lekf_core_f64 (per point)
+ *                                                  lekf_core_batch (many points, device memory)
+ *   obs_local   speedy/letkf/letkf_tools.f90:272-603, lorenz96/letkf/letkf.f90:216-254,
+ *               scale/letkf/letkf_tools.f90:853-1178 -> the selection inside lekf_das_run and
+ *                                                  lekf_obs_local_* (device memory)
+ *   set_letkf_obs speedy/letkf/letkf_obs.f90:48-335 -> lekf_set_obs_global / lekf_set_obs_l96
+ *   das_letkf   speedy/letkf/letkf_tools.f90:45-267  -> lekf_*_create + lekf_set_ensemble +
+ *                                                  lekf_das_run + lekf_get_analysis
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's Fortran
+ * column-major layouts.  Every entry point returns a status code; the
+ * Fortran shims map nonzero codes to the reference's STOP codes.
+ * lekf_core_f64 is re-entrant and thread-safe (per-thread stream and
+ * buffers), as OpenMP callers of letkf_core require
+ * (scale/letkf/letkf_tools.f90:175).  A handle is used by one caller at a time.
+ */
+#ifndef LEKF_H
+#define LEKF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define LEKF_OK 0
+#define LEKF_ERR_EIGEN 2      /* eigensolver failure: common/common_mtx.f90:61-64,75-78 STOP 2 */
+#define LEKF_WARN_RANKDEF 4   /* lambda_min < lambda_max*sqrt(eps): the reference truncates the
+                                 rank (common_mtx.f90:66-74) and produces NaN; we return finite
+                                 values and flag the point */
+#define LEKF_ERR_SIZE 99      /* inconsistent sizes: speedy/letkf/letkf_tools.f90:557-561 STOP 99 */
+#define LEKF_ERR_HIP 100      /* HIP runtime error */
+#define LEKF_ERR_ARG 101      /* invalid argument */
+#define LEKF_ERR_NODEV 102    /* no GPU device */
+
+/* arithmetic forms */
+#define LEKF_PREC_F64 0     /* plain: all products FP64 */
+#define LEKF_PREC_F32 1     /* mixed: Y^T R^-1 Y on FP32 MFMA, eigen / W / apply FP64 */
+#define LEKF_PREC_BF16X3 2  /* mixed: Y^T R^-1 Y as split-BF16x3 on MFMA, FP32 accumulation */
+
+/* inflation modes of the das entry points */
+#define LEKF_INFL_ADAPTIVE 0  /* letkf_core's adaptive update kept (speedy/letkf/letkf_tools.f90:160-162) */
+#define LEKF_INFL_FIXED 1     /* reset to infl_value after each point (lorenz96/letkf/letkf.f90:256) */
+
+/* job flags (lekf_job.flags) */
+#define LEKF_F_WLOC 1        /* rdiag already err^2/rho (rdiag_wloc=.true.) */
+#define LEKF_F_MINFL 2       /* minfl present */
+#define LEKF_F_FIXED_INFL 4  /* write infl_fixed as the updated inflation */
+#define LEKF_F_TRANSM 8      /* return Wa in trans and w-bar in transm (transm present) */
+#define LEKF_F_APPLY 16      /* apply Xa = xmean + Xb' T to the point's state rows */
+#define LEKF_F_MIXED 32
+#define LEKF_F_BF16X3 64
+
+/* per-point status bits (lekf_job.status) */
+#define LEKF_PT_RANKDEF 1
+#define LEKF_PT_NOCONV 2
+
+#define LEKF_NVMAX 16        /* state rows per point in the fused apply */
+#define LEKF_KMAX 95         /* largest ensemble size of the LDS-resident eigensolver */
+
+/*
+ * One batched analysis job (device pointers).  The library fills pt_begin.
+ * Y(o, m) = hdxf[o*y_so + m*y_sm]; point b uses local entries
+ * loc_off[b] .. loc_off[b] + cnt (cnt = loc_cnt[b], or loc_off[b+1]-loc_off[b]
+ * when loc_cnt is NULL), each an obs index and a localisation weight.
+ * Fused apply rows: point b = lev*nij + ij owns nv3d rows at
+ * x3b + (b*nv3d + v)*k and, at lev 0, nv2d rows at x2b + (ij*nv2d + v)*k.
+ */
+typedef struct lekf_job {
+    long pt_begin;
+    int npt;
+    int k;
+    int flags;
+    int nij;
+    int nv3d, nv2d;
+    double minfl, infl_fixed;
+    const double *hdxf;
+    long y_so, y_sm;
+    const float *hdxf32;     /* mixed forms: Y(o, m) = hdxf32[o*k + m] */
+    const double *rdiag;
+    const double *dep;
+    const long *loc_off;
+    const int *loc_cnt;      /* nullable */
+    const int *loc_idx;
+    const double *loc_rho;
+    const double *loc_rdiag; /* nullable */
+    const double *infl_in;   /* nullable: infl_fixed used */
+    double *infl_out;        /* nullable */
+    double *trans;           /* nullable, k x k column-major per point */
+    double *transm;          /* nullable */
+    double *pao;             /* nullable */
+    int *status;             /* nullable */
+    const double *x3b;
+    double *x3a;
+    const double *x2b;
+    double *x2a;
+    const uint32_t *upd_mask; /* nullable: bit v of upd_mask[lev] set = 3-D variable v updated */
+} lekf_job;
+
+/* --- 1. per-point drop-in --------------------------------------------------
+ * letkf_core(ne,nobs,nobsl,hdxb,rdiag,rloc,dep,parm_infl,trans,transm,pao,
+ *            rdiag_wloc,minfl)   common/common_letkf.f90:50
+ * hdxb is nobs x ne column-major (first nobsl rows used).  transm, pao and
+ * minfl may be NULL (absent OPTIONAL arguments).  Host memory. */
+int lekf_core_f64(int ne, int nobs, int nobsl, const double *hdxb, const double *rdiag,
+                  const double *rloc, const double *dep, double *parm_infl, double *trans,
+                  double *transm, double *pao, int rdiag_wloc, const double *minfl);
+
+/* --- 2. batched local solve ----------------------------------------------
+ * Solves every point of njobs jobs (device memory) on `stream` (hipStream_t,
+ * NULL = default stream); prec selects the Gram arithmetic.  Asynchronous. */
+int lekf_core_batch(const lekf_job *jobs, int njobs, int prec, void *stream);
+
+/* --- 3. selection on device memory ---------------------------------------
+ * Lorenz-96 rule (lorenz96/letkf/letkf.f90:216-254): obs i of grid index
+ * gidx[i]; point ix keeps |periodic distance| < 2*sqrt(10/3)*xlocal and
+ * rho >= 1e-4.  Lists at loc_off[ix] = ix*ny. */
+int lekf_obs_local_l96(int nx, int ny, const int *gidx, double xlocal, long *loc_off, int *loc_cnt,
+                       int *loc_idx, double *loc_rho, void *stream);
+
+/* --- 4. das_letkf-level handle -------------------------------------------- */
+typedef struct lekf_handle lekf_handle;
+
+typedef struct lekf_global_cfg {
+    int nlon, nlat, nlev;     /* grid; ij = i + nlon*j */
+    const double *lon;        /* nlon longitudes (deg), ascending in [0, 360) */
+    const double *lat;        /* nlat latitudes (deg), ascending */
+    const double *sig;        /* nlev sigma levels (p_full = ps*sig) */
+    int ne;                   /* ensemble size */
+    int nv3d, nv2d;           /* 3-D variables, 2-D variables (updated with the level-1 W) */
+    int ips;                  /* surface pressure among the 2-D variables */
+    int iq;                   /* specific humidity among the 3-D variables, -1: none */
+    int q_top_lev;            /* 1-based level from which q keeps the background; 0: off */
+    double sigma_obs;         /* horizontal localisation scale (m) */
+    double sigma_obsv;        /* vertical localisation scale (ln p) */
+    double gross_error;       /* QC: |d| > gross_error * err is rejected */
+    double var_local[7];      /* localisation factor per obs type u,v,t,q,rh,ps,rain */
+    int prec;                 /* LEKF_PREC_* */
+    int infl_mode;            /* LEKF_INFL_* */
+    double infl_value;        /* fixed value, or the initial adaptive field value */
+} lekf_global_cfg;
+
+typedef struct lekf_l96_cfg {
+    int nx;                   /* grid points */
+    int ne;
+    double xlocal;            /* localisation scale (grid points) */
+    int prec;
+    int infl_mode;
+    double infl_value;
+} lekf_l96_cfg;
+
+lekf_handle *lekf_global_create(const lekf_global_cfg *cfg, int *status);
+lekf_handle *lekf_l96_create(const lekf_l96_cfg *cfg, int *status);
+void lekf_destroy(lekf_handle *h);
+
+/* background ensemble, reference layout: gues3d(nij,nlev,ne,nv3d),
+ * gues2d(nij,ne,nv2d), column-major (L96: nij = nx, nlev = 1, nv3d = 1). */
+int lekf_set_ensemble(lekf_handle *h, const double *gues3d, const double *gues2d);
+/* inflation field infl3d(nij,nlev) (and infl2d(nij,nv2d), may be NULL) */
+int lekf_set_infl(lekf_handle *h, const double *infl3d, const double *infl2d);
+/* set_letkf_obs (speedy/letkf/letkf_obs.f90:48-335): obs records and H(x) of
+ * every member, hxf(nobs,ne) column-major; qc may be NULL (all good). */
+int lekf_set_obs_global(lekf_handle *h, int nobs, const double *elm, const double *lon,
+                        const double *lat, const double *lev, const double *dat,
+                        const double *err, const int *qc, const double *hxf);
+/* Lorenz-96 obs: grid index gidx (0-based), value y, error err, hxf(ny,ne) */
+int lekf_set_obs_l96(lekf_handle *h, int ny, const int *gidx, const double *y, const double *err,
+                     const double *hxf);
+/* one analysis (selection + local solves + apply) on the handle's stream;
+ * asynchronous.  The analysis stays on the device. */
+int lekf_das_run(lekf_handle *h);
+/* batch the local solves of several handles into shared launches */
+int lekf_das_run_many(lekf_handle **hs, int n);
+int lekf_sync(lekf_handle *h);
+int lekf_get_analysis(lekf_handle *h, double *anal3d, double *anal2d);
+int lekf_get_infl(lekf_handle *h, double *infl3d, double *infl2d);
+int lekf_get_nobsl(lekf_handle *h, int *nobsl);
+/* local list of point b (= lev*nij + ij): up to cap entries of sorted-obs index,
+ * rho; returns the count (or a negative status) */
+int lekf_get_local(lekf_handle *h, long b, int cap, int *idx, double *rho);
+/* sorted-obs permutation (input index of each binned obs) and nobsgrd(nlon,nlat) */
+int lekf_get_obs_index(lekf_handle *h, int *perm, int *nobsgrd, int *nbinned);
+/* number of points flagged rank-deficient / not converged in the last run */
+int lekf_get_flags(lekf_handle *h, long *n_rankdef, long *n_noconv);
+/* use the analysis as the next background (cycling without host traffic) */
+int lekf_cycle_swap(lekf_handle *h);
+/* device pointers of the resident ensembles (for device-side pipelines) */
+int lekf_device_state(lekf_handle *h, double **x3b, double **x3a, double **x2b, double **x2a);
+/* bytes of device memory the handle holds */
+long lekf_device_bytes(lekf_handle *h);
+/* hipStream_t of the handle */
+void *lekf_stream(lekf_handle *h);
+
+/* library info */
+int lekf_device_count(void);
+const char *lekf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,978 @@
+// lekf_capi.hip -- C ABI of the LETKF library: per-point drop-in, batched
+// solve, and the das_letkf-level handle that keeps the ensemble resident on
+// the device.  Host code only orchestrates: every array operation of the
+// analysis runs in the kernels of lekf_solve.hip / lekf_cull.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "lekf_cull.hpp"
+#include "lekf_dev.hpp"
+
+#define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
+
+namespace {
+
+thread_local char g_errmsg[256];
+
+void set_err(hipError_t e, const char* what)
+{
+    snprintf(g_errmsg, sizeof(g_errmsg), "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(x)                                    \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) {                      \
+            set_err(e_, #x);                         \
+            return LEKF_ERR_HIP;                     \
+        }                                            \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes == 0) bytes = 8;
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T* as() const
+    {
+        return (T*)p;
+    }
+};
+
+__global__ void iota_kernel(int* a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// reference column-major (nij, nlev, ne, nv) <-> device rows [(lev*nij+ij)*nv + v][m]
+__global__ void ens3_to_dev(const double* ref, double* dev, int nij, int nlev, int ne, int nv)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)nij * nlev * ne * nv;
+    if (t >= tot) return;
+    // t enumerates the reference layout (coalesced reads)
+    const int ij = (int)(t % nij);
+    long r = t / nij;
+    const int lev = (int)(r % nlev);
+    r /= nlev;
+    const int m = (int)(r % ne);
+    const int v = (int)(r / ne);
+    dev[(((long)lev * nij + ij) * nv + v) * ne + m] = ref[t];
+}
+
+__global__ void ens3_from_dev(const double* dev, double* ref, int nij, int nlev, int ne, int nv)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)nij * nlev * ne * nv;
+    if (t >= tot) return;
+    const int ij = (int)(t % nij);
+    long r = t / nij;
+    const int lev = (int)(r % nlev);
+    r /= nlev;
+    const int m = (int)(r % ne);
+    const int v = (int)(r / ne);
+    ref[t] = dev[(((long)lev * nij + ij) * nv + v) * ne + m];
+}
+
+// Lorenz-96 obs set-up: mean of H(x), perturbations, departure, R = err^2
+// (lorenz96/letkf/letkf.f90:185-214)
+__global__ void l96_obs_kernel(int ny, int ne, const double* hxf, const double* y,
+                               const double* err, double* hdxf, float* hdxf32, double* dep,
+                               double* rdiag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ny) return;
+    double m = hxf[i];
+    for (int j = 1; j < ne; ++j) m += hxf[i + (long)j * ny];
+    m /= (double)ne;
+    for (int j = 0; j < ne; ++j) {
+        const double v = hxf[i + (long)j * ny] - m;
+        hdxf[(long)i * ne + j] = v;
+        if (hdxf32) hdxf32[(long)i * ne + j] = (float)v;
+    }
+    dep[i] = y[i] - m;
+    rdiag[i] = err[i] * err[i];
+}
+
+int mode_of_prec(int prec) { return prec == LEKF_PREC_F32 ? 1 : (prec == LEKF_PREC_BF16X3 ? 2 : 0); }
+
+// ---------------------------------------------------------------------------
+// per-thread context of the per-point drop-in and of lekf_core_batch
+// ---------------------------------------------------------------------------
+struct CoreCtx {
+    bool init = false;
+    hipStream_t st = nullptr;
+    DevBuf h, rd, rl, dp, idx, off, infl, tr, tm, pa, stat, job;
+    int iota_n = 0;
+    std::map<hipStream_t, DevBuf*> jobs_by_stream;
+};
+thread_local CoreCtx t_core;
+
+int core_init()
+{
+    if (t_core.init) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return LEKF_ERR_NODEV;
+    HIPCHK(hipStreamCreateWithFlags(&t_core.st, hipStreamNonBlocking));
+    t_core.init = true;
+    return 0;
+}
+
+int ensure_iota(DevBuf& b, int& have, int n, hipStream_t st)
+{
+    if (n <= have) return 0;
+    HIPCHK(b.ensure(sizeof(int) * (size_t)n));
+    hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b.as<int>(), n);
+    HIPCHK(hipGetLastError());
+    have = n;
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* lekf_version(void) { return LEKF_VERSION; }
+
+const char* lekf_last_error(void) { return g_errmsg; }
+
+int lekf_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lekf_core_f64(int ne, int nobs, int nobsl, const double* hdxb, const double* rdiag,
+                  const double* rloc, const double* dep, double* parm_infl, double* trans,
+                  double* transm, double* pao, int rdiag_wloc, const double* minfl)
+{
+    if (ne < 2 || ne > LEKF_KMAX || nobsl < 0 || nobs < nobsl || !parm_infl || !trans)
+        return LEKF_ERR_SIZE;
+    if (nobsl > 0 && (!hdxb || !rdiag || !rloc || !dep)) return LEKF_ERR_ARG;
+    int rc = core_init();
+    if (rc) return rc;
+    CoreCtx& c = t_core;
+    hipStream_t st = c.st;
+    const size_t p = (size_t)nobsl, k = (size_t)ne;
+    HIPCHK(c.h.ensure(sizeof(double) * (p ? p : 1) * k));
+    HIPCHK(c.rd.ensure(sizeof(double) * (p ? p : 1)));
+    HIPCHK(c.rl.ensure(sizeof(double) * (p ? p : 1)));
+    HIPCHK(c.dp.ensure(sizeof(double) * (p ? p : 1)));
+    HIPCHK(c.off.ensure(sizeof(long) * 2));
+    HIPCHK(c.infl.ensure(sizeof(double) * 2));
+    HIPCHK(c.tr.ensure(sizeof(double) * k * k));
+    HIPCHK(c.tm.ensure(sizeof(double) * k));
+    HIPCHK(c.pa.ensure(sizeof(double) * k * k));
+    HIPCHK(c.stat.ensure(sizeof(int)));
+    HIPCHK(c.job.ensure(sizeof(lekf_job)));
+    rc = ensure_iota(c.idx, c.iota_n, (int)(p ? p : 1), st);
+    if (rc) return rc;
+    if (p) {
+        HIPCHK(hipMemcpy2DAsync(c.h.p, p * sizeof(double), hdxb, (size_t)nobs * sizeof(double),
+                                p * sizeof(double), k, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c.rd.p, rdiag, p * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c.rl.p, rloc, p * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c.dp.p, dep, p * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    const long offs[2] = {0, (long)p};
+    HIPCHK(hipMemcpyAsync(c.off.p, offs, sizeof(offs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c.infl.p, parm_infl, sizeof(double), hipMemcpyHostToDevice, st));
+    lekf_job J;
+    memset(&J, 0, sizeof(J));
+    J.pt_begin = 0;
+    J.npt = 1;
+    J.k = ne;
+    J.flags = (rdiag_wloc ? LEKF_F_WLOC : 0) | (minfl ? LEKF_F_MINFL : 0) |
+              (transm ? LEKF_F_TRANSM : 0);
+    J.minfl = minfl ? *minfl : 0.0;
+    J.hdxf = c.h.as<double>();
+    J.y_so = 1;
+    J.y_sm = (long)(p ? p : 1);
+    J.rdiag = c.rd.as<double>();
+    J.dep = c.dp.as<double>();
+    J.loc_off = c.off.as<long>();
+    J.loc_idx = c.idx.as<int>();
+    J.loc_rho = c.rl.as<double>();
+    J.infl_in = c.infl.as<double>();
+    J.infl_out = c.infl.as<double>() + 1;
+    J.trans = c.tr.as<double>();
+    J.transm = transm ? c.tm.as<double>() : nullptr;
+    J.pao = pao ? c.pa.as<double>() : nullptr;
+    J.status = c.stat.as<int>();
+    HIPCHK(hipMemcpyAsync(c.job.p, &J, sizeof(J), hipMemcpyHostToDevice, st));
+    HIPCHK(lekf_solve_launch(c.job.as<lekf_job>(), 1, 1, ne, 0, st));
+    int pst = 0;
+    HIPCHK(hipMemcpyAsync(trans, c.tr.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
+    if (transm) HIPCHK(hipMemcpyAsync(transm, c.tm.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+    if (pao) HIPCHK(hipMemcpyAsync(pao, c.pa.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(parm_infl, c.infl.as<double>() + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&pst, c.stat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (pst & LEKF_PT_NOCONV) return LEKF_ERR_EIGEN;
+    if (pst & LEKF_PT_RANKDEF) return LEKF_WARN_RANKDEF;
+    return LEKF_OK;
+}
+
+int lekf_core_batch(const lekf_job* jobs, int njobs, int prec, void* stream)
+{
+    if (njobs <= 0 || !jobs) return LEKF_ERR_ARG;
+    int rc = core_init();
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<lekf_job> J(jobs, jobs + njobs);
+    long tot = 0;
+    int kmax = 0;
+    const int mode = mode_of_prec(prec);
+    for (auto& j : J) {
+        if (j.k < 2 || j.k > LEKF_KMAX || j.npt < 0) return LEKF_ERR_SIZE;
+        if ((j.flags & LEKF_F_APPLY) && (j.nv3d + j.nv2d > LEKF_NVMAX || j.nij <= 0))
+            return LEKF_ERR_SIZE;
+        if (mode != 0 && !j.hdxf32) return LEKF_ERR_ARG;
+        if (!j.loc_off || (j.npt > 0 && (!j.loc_idx || !j.loc_rho))) return LEKF_ERR_ARG;
+        j.pt_begin = tot;
+        tot += j.npt;
+        kmax = std::max(kmax, j.k);
+    }
+    if (tot == 0) return LEKF_OK;
+    if (lekf_solve_nt(kmax) > 6) return LEKF_ERR_SIZE;
+    DevBuf*& db = t_core.jobs_by_stream[st];
+    if (!db) db = new DevBuf();
+    HIPCHK(db->ensure(sizeof(lekf_job) * J.size()));
+    HIPCHK(hipMemcpyAsync(db->p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(lekf_solve_launch(db->as<lekf_job>(), njobs, tot, kmax, mode, st));
+    return LEKF_OK;
+}
+
+int lekf_obs_local_l96(int nx, int ny, const int* gidx, double xlocal, long* loc_off, int* loc_cnt,
+                       int* loc_idx, double* loc_rho, void* stream)
+{
+    if (nx <= 0 || ny < 0) return LEKF_ERR_SIZE;
+    L96CullArgs A;
+    A.nx = nx;
+    A.ny = ny;
+    A.gidx = gidx;
+    A.xlocal = xlocal;
+    A.xmaxloc = xlocal * 2.0 * std::sqrt(10.0 / 3.0);
+    A.loc_off = loc_off;
+    A.loc_cnt = loc_cnt;
+    A.loc_idx = loc_idx;
+    A.loc_rho = loc_rho;
+    HIPCHK(lekf_cull_l96_launch(A, (hipStream_t)stream));
+    return LEKF_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// das_letkf-level handle
+// ===========================================================================
+struct lekf_handle {
+    int kind = 0;  // 0 global, 1 Lorenz-96
+    hipStream_t st = nullptr;
+    int ne = 0, nij = 0, nlev = 1, nv3d = 1, nv2d = 0;
+    int prec = 0;
+    int infl_mode = 0;
+    double infl_value = 1.0;
+    lekf_global_cfg g{};
+    lekf_l96_cfg l{};
+    std::vector<double> lon, lat, sig;
+    double dist_zero = 0, dist_zerov = 0, dlat_zero = 0;
+    // device grid
+    DevBuf d_lon, d_lat, d_sig, d_col_lon, d_col_lat, d_col_dlz, d_var_local, d_logpfm;
+    // state
+    DevBuf x3b, x3a, x2b, x2a, upd, infl_in, infl_out, infl2, stage;
+    bool have_ens = false;
+    // obs
+    int nobs = 0, nbinned = 0;
+    DevBuf r_elm, r_lon, r_lat, r_lev, r_dat, r_err, r_qc, r_hxf, r_y, r_gidx;
+    DevBuf tmp_mean, tmp_dep, keys, cell_cnt, perm, o_hdxf, o_hdxf32, o_dep, o_rdiag, o_err,
+        o_typ, o_sin, o_cos, o_lonr, o_lnlev, o_lndat, nobsgrd;
+    bool have_obs = false;
+    // local lists
+    DevBuf cand_cnt, cand_off, loc_off, loc_cnt, loc_idx, loc_rho, status;
+    long loc_cap = 0;
+    // job
+    DevBuf djob;
+    lekf_job job{};
+    long bytes = 0;
+};
+
+namespace {
+
+long handle_bytes(const lekf_handle* h)
+{
+    const DevBuf* bs[] = {&h->d_lon, &h->d_lat, &h->d_sig, &h->d_col_lon, &h->d_col_lat,
+                          &h->d_col_dlz, &h->d_var_local, &h->d_logpfm, &h->x3b, &h->x3a,
+                          &h->x2b, &h->x2a, &h->upd, &h->infl_in, &h->infl_out, &h->infl2,
+                          &h->stage, &h->r_elm, &h->r_lon, &h->r_lat, &h->r_lev, &h->r_dat,
+                          &h->r_err, &h->r_qc, &h->r_hxf, &h->r_y, &h->r_gidx, &h->tmp_mean,
+                          &h->tmp_dep, &h->keys, &h->cell_cnt, &h->perm, &h->o_hdxf,
+                          &h->o_hdxf32, &h->o_dep, &h->o_rdiag, &h->o_err, &h->o_typ, &h->o_sin,
+                          &h->o_cos, &h->o_lonr, &h->o_lnlev, &h->o_lndat, &h->nobsgrd,
+                          &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx,
+                          &h->loc_rho, &h->status, &h->djob};
+    long s = 0;
+    for (auto* b : bs) s += (long)b->n;
+    return s;
+}
+
+void free_all(lekf_handle* h)
+{
+    DevBuf* bs[] = {&h->d_lon, &h->d_lat, &h->d_sig, &h->d_col_lon, &h->d_col_lat, &h->d_col_dlz,
+                    &h->d_var_local, &h->d_logpfm, &h->x3b, &h->x3a, &h->x2b, &h->x2a, &h->upd,
+                    &h->infl_in, &h->infl_out, &h->infl2, &h->stage, &h->r_elm, &h->r_lon,
+                    &h->r_lat, &h->r_lev, &h->r_dat, &h->r_err, &h->r_qc, &h->r_hxf, &h->r_y,
+                    &h->r_gidx, &h->tmp_mean, &h->tmp_dep, &h->keys, &h->cell_cnt, &h->perm,
+                    &h->o_hdxf, &h->o_hdxf32, &h->o_dep, &h->o_rdiag, &h->o_err, &h->o_typ,
+                    &h->o_sin, &h->o_cos, &h->o_lonr, &h->o_lnlev, &h->o_lndat, &h->nobsgrd,
+                    &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
+                    &h->status, &h->djob};
+    for (auto* b : bs) b->release();
+}
+
+int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
+{
+    HIPCHK(b.ensure(bytes));
+    if (bytes && src) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+int alloc_state(lekf_handle* h)
+{
+    const size_t n3 = (size_t)h->nij * h->nlev * h->nv3d * h->ne;
+    const size_t n2 = (size_t)h->nij * h->nv2d * h->ne;
+    HIPCHK(h->x3b.ensure(sizeof(double) * n3));
+    HIPCHK(h->x3a.ensure(sizeof(double) * n3));
+    HIPCHK(h->x2b.ensure(sizeof(double) * (n2 ? n2 : 1)));
+    HIPCHK(h->x2a.ensure(sizeof(double) * (n2 ? n2 : 1)));
+    const size_t npt = (size_t)h->nij * h->nlev;
+    HIPCHK(h->infl_in.ensure(sizeof(double) * npt));
+    HIPCHK(h->infl_out.ensure(sizeof(double) * npt));
+    HIPCHK(h->infl2.ensure(sizeof(double) * (size_t)h->nij * (h->nv2d ? h->nv2d : 1)));
+    HIPCHK(h->status.ensure(sizeof(int) * npt));
+    HIPCHK(h->loc_off.ensure(sizeof(long) * (npt + 1)));
+    HIPCHK(h->loc_cnt.ensure(sizeof(int) * npt));
+    // initial inflation field
+    std::vector<double> f(npt, h->infl_value);
+    HIPCHK(hipMemcpyAsync(h->infl_in.p, f.data(), sizeof(double) * npt, hipMemcpyHostToDevice,
+                          h->st));
+    std::vector<double> f2((size_t)h->nij * (h->nv2d ? h->nv2d : 1), h->infl_value);
+    HIPCHK(hipMemcpyAsync(h->infl2.p, f2.data(), sizeof(double) * f2.size(), hipMemcpyHostToDevice,
+                          h->st));
+    // update masks (SPEEDY q rule, speedy/letkf/letkf_tools.f90:172-177)
+    std::vector<uint32_t> m(h->nlev, 0xffffffffu);
+    if (h->kind == 0 && h->g.iq >= 0 && h->g.q_top_lev > 0)
+        for (int l = h->g.q_top_lev - 1; l < h->nlev; ++l) m[l] &= ~(1u << h->g.iq);
+    HIPCHK(h->upd.ensure(sizeof(uint32_t) * h->nlev));
+    HIPCHK(hipMemcpyAsync(h->upd.p, m.data(), sizeof(uint32_t) * h->nlev, hipMemcpyHostToDevice,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+void fill_job(lekf_handle* h)
+{
+    lekf_job& J = h->job;
+    memset(&J, 0, sizeof(J));
+    J.npt = h->nij * h->nlev;
+    J.k = h->ne;
+    J.flags = LEKF_F_APPLY | (h->infl_mode == LEKF_INFL_FIXED ? LEKF_F_FIXED_INFL : 0) |
+              (h->prec == LEKF_PREC_F32 ? LEKF_F_MIXED : 0) |
+              (h->prec == LEKF_PREC_BF16X3 ? LEKF_F_BF16X3 : 0);
+    J.nij = h->nij;
+    J.nv3d = h->nv3d;
+    J.nv2d = h->nv2d;
+    J.infl_fixed = h->infl_value;
+    J.hdxf = h->o_hdxf.as<double>();
+    J.y_so = h->ne;
+    J.y_sm = 1;
+    J.hdxf32 = h->prec != LEKF_PREC_F64 ? h->o_hdxf32.as<float>() : nullptr;
+    J.rdiag = h->o_rdiag.as<double>();
+    J.dep = h->o_dep.as<double>();
+    J.loc_off = h->loc_off.as<long>();
+    J.loc_cnt = h->loc_cnt.as<int>();
+    J.loc_idx = h->loc_idx.as<int>();
+    J.loc_rho = h->loc_rho.as<double>();
+    J.infl_in = h->infl_in.as<double>();
+    J.infl_out = h->infl_out.as<double>();
+    J.status = h->status.as<int>();
+    J.x3b = h->x3b.as<double>();
+    J.x3a = h->x3a.as<double>();
+    J.x2b = h->nv2d ? h->x2b.as<double>() : nullptr;
+    J.x2a = h->nv2d ? h->x2a.as<double>() : nullptr;
+    J.upd_mask = h->upd.as<uint32_t>();
+}
+
+int run_select(lekf_handle* h)
+{
+    hipStream_t st = h->st;
+    if (h->kind == 0) {
+        HIPCHK(lekf_logpfm_launch(h->x2b.as<double>(), h->nij, h->nv2d, h->g.ips, h->ne,
+                                  h->d_sig.as<double>(), h->nlev, h->d_logpfm.as<double>(), st));
+        GlobalCullArgs A;
+        memset(&A, 0, sizeof(A));
+        A.nij = h->nij;
+        A.nlev = h->nlev;
+        A.nlon = h->g.nlon;
+        A.nlat = h->g.nlat;
+        A.lon = h->d_lon.as<double>();
+        A.lat = h->d_lat.as<double>();
+        A.col_lon = h->d_col_lon.as<double>();
+        A.col_lat = h->d_col_lat.as<double>();
+        A.col_dlon_zero = h->d_col_dlz.as<double>();
+        A.dlat_zero = h->dlat_zero;
+        A.dist_zero = h->dist_zero;
+        A.dist_zerov = h->dist_zerov;
+        A.sigma_obs = h->g.sigma_obs;
+        A.sigma_obsv = h->g.sigma_obsv;
+        A.logpfm = h->d_logpfm.as<double>();
+        A.var_local = h->d_var_local.as<double>();
+        A.nobsgrd = h->nobsgrd.as<int>();
+        A.o_typ = h->o_typ.as<int>();
+        A.o_sinlat = h->o_sin.as<double>();
+        A.o_coslat = h->o_cos.as<double>();
+        A.o_lonr = h->o_lonr.as<double>();
+        A.o_lnlev = h->o_lnlev.as<double>();
+        A.o_lndat = h->o_lndat.as<double>();
+        A.cand_cnt = h->cand_cnt.as<int>();
+        A.cand_off = h->cand_off.as<long>();
+        A.loc_off = h->loc_off.as<long>();
+        A.loc_cnt = h->loc_cnt.as<int>();
+        A.loc_idx = h->loc_idx.as<int>();
+        A.loc_rho = h->loc_rho.as<double>();
+        HIPCHK(lekf_cull_global_fill_launch(A, st));
+    } else {
+        L96CullArgs A;
+        A.nx = h->nij;
+        A.ny = h->nobs;
+        A.gidx = h->r_gidx.as<int>();
+        A.xlocal = h->l.xlocal;
+        A.xmaxloc = h->l.xlocal * 2.0 * std::sqrt(10.0 / 3.0);
+        A.loc_off = h->loc_off.as<long>();
+        A.loc_cnt = h->loc_cnt.as<int>();
+        A.loc_idx = h->loc_idx.as<int>();
+        A.loc_rho = h->loc_rho.as<double>();
+        HIPCHK(lekf_cull_l96_launch(A, st));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+lekf_handle* lekf_global_create(const lekf_global_cfg* cfg, int* status)
+{
+    auto fail = [&](int c) -> lekf_handle* {
+        if (status) *status = c;
+        return nullptr;
+    };
+    if (!cfg || cfg->nlon < 2 || cfg->nlat < 3 || cfg->nlev < 1 || cfg->ne < 2 ||
+        cfg->ne > LEKF_KMAX || cfg->nv3d < 1 || cfg->nv2d < 1 || cfg->nv3d + cfg->nv2d > LEKF_NVMAX ||
+        cfg->nlev > LEKF_MAX_LEV || cfg->ips < 0 || cfg->ips >= cfg->nv2d || cfg->nv3d > 32)
+        return fail(LEKF_ERR_SIZE);
+    if (lekf_device_count() < 1) return fail(LEKF_ERR_NODEV);
+    lekf_handle* h = new lekf_handle();
+    h->kind = 0;
+    h->g = *cfg;
+    h->ne = cfg->ne;
+    h->nij = cfg->nlon * cfg->nlat;
+    h->nlev = cfg->nlev;
+    h->nv3d = cfg->nv3d;
+    h->nv2d = cfg->nv2d;
+    h->prec = cfg->prec;
+    h->infl_mode = cfg->infl_mode;
+    h->infl_value = cfg->infl_value;
+    h->lon.assign(cfg->lon, cfg->lon + cfg->nlon);
+    h->lat.assign(cfg->lat, cfg->lat + cfg->nlat);
+    h->sig.assign(cfg->sig, cfg->sig + cfg->nlev);
+    h->g.lon = h->lon.data();
+    h->g.lat = h->lat.data();
+    h->g.sig = h->sig.data();
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(LEKF_ERR_HIP);
+    }
+    // localisation constants (speedy/letkf/letkf_obs.f90:82-88); host libm as in the reference
+    const double pi = LEKF_PI_REF, re = LEKF_RE_REF;
+    h->dist_zero = cfg->sigma_obs * std::sqrt(10.0 / 3.0) * 2.0;
+    h->dist_zerov = cfg->sigma_obsv * std::sqrt(10.0 / 3.0) * 2.0;
+    h->dlat_zero = h->dist_zero / pi / re * 180.0;
+    std::vector<double> clon(h->nij), clat(h->nij), cdz(h->nij);
+    for (int j = 0; j < cfg->nlat; ++j)
+        for (int i = 0; i < cfg->nlon; ++i) {
+            const int ij = i + cfg->nlon * j;
+            clon[ij] = cfg->lon[i];
+            clat[ij] = cfg->lat[j];
+            cdz[ij] = h->dlat_zero / std::cos(pi * cfg->lat[j] / 180.0);
+        }
+    hipStream_t st = h->st;
+    if ((rc = upload(h->d_lon, cfg->lon, sizeof(double) * cfg->nlon, st)) ||
+        (rc = upload(h->d_lat, cfg->lat, sizeof(double) * cfg->nlat, st)) ||
+        (rc = upload(h->d_sig, cfg->sig, sizeof(double) * cfg->nlev, st)) ||
+        (rc = upload(h->d_col_lon, clon.data(), sizeof(double) * h->nij, st)) ||
+        (rc = upload(h->d_col_lat, clat.data(), sizeof(double) * h->nij, st)) ||
+        (rc = upload(h->d_col_dlz, cdz.data(), sizeof(double) * h->nij, st)) ||
+        (rc = upload(h->d_var_local, cfg->var_local, sizeof(double) * 7, st)) ||
+        (rc = (h->d_logpfm.ensure(sizeof(double) * (size_t)h->nij * h->nlev) == hipSuccess
+                   ? 0
+                   : LEKF_ERR_HIP)) ||
+        (rc = (h->cand_cnt.ensure(sizeof(int) * h->nij) == hipSuccess ? 0 : LEKF_ERR_HIP)) ||
+        (rc = (h->cand_off.ensure(sizeof(long) * (h->nij + 1)) == hipSuccess ? 0 : LEKF_ERR_HIP)) ||
+        (rc = alloc_state(h))) {
+        free_all(h);
+        (void)hipStreamDestroy(h->st);
+        delete h;
+        return fail(rc);
+    }
+    if (status) *status = 0;
+    return h;
+}
+
+lekf_handle* lekf_l96_create(const lekf_l96_cfg* cfg, int* status)
+{
+    auto fail = [&](int c) -> lekf_handle* {
+        if (status) *status = c;
+        return nullptr;
+    };
+    if (!cfg || cfg->nx < 1 || cfg->ne < 2 || cfg->ne > LEKF_KMAX) return fail(LEKF_ERR_SIZE);
+    if (lekf_device_count() < 1) return fail(LEKF_ERR_NODEV);
+    lekf_handle* h = new lekf_handle();
+    h->kind = 1;
+    h->l = *cfg;
+    h->ne = cfg->ne;
+    h->nij = cfg->nx;
+    h->nlev = 1;
+    h->nv3d = 1;
+    h->nv2d = 0;
+    h->prec = cfg->prec;
+    h->infl_mode = cfg->infl_mode;
+    h->infl_value = cfg->infl_value;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(LEKF_ERR_HIP);
+    }
+    int rc = alloc_state(h);
+    if (rc) {
+        free_all(h);
+        (void)hipStreamDestroy(h->st);
+        delete h;
+        return fail(rc);
+    }
+    if (status) *status = 0;
+    return h;
+}
+
+void lekf_destroy(lekf_handle* h)
+{
+    if (!h) return;
+    (void)hipStreamSynchronize(h->st);
+    free_all(h);
+    (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+void* lekf_stream(lekf_handle* h) { return h ? (void*)h->st : nullptr; }
+
+long lekf_device_bytes(lekf_handle* h) { return h ? handle_bytes(h) : 0; }
+
+int lekf_set_ensemble(lekf_handle* h, const double* gues3d, const double* gues2d)
+{
+    if (!h || !gues3d || (h->nv2d > 0 && !gues2d)) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const size_t n3 = (size_t)h->nij * h->nlev * h->ne * h->nv3d;
+    const size_t n2 = (size_t)h->nij * h->ne * h->nv2d;
+    HIPCHK(h->stage.ensure(sizeof(double) * std::max(n3, n2)));
+    HIPCHK(hipMemcpyAsync(h->stage.p, gues3d, sizeof(double) * n3, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(ens3_to_dev, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                       h->stage.as<double>(), h->x3b.as<double>(), h->nij, h->nlev, h->ne, h->nv3d);
+    HIPCHK(hipGetLastError());
+    if (n2) {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpyAsync(h->stage.p, gues2d, sizeof(double) * n2, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(ens3_to_dev, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st,
+                           h->stage.as<double>(), h->x2b.as<double>(), h->nij, 1, h->ne, h->nv2d);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    h->have_ens = true;
+    return LEKF_OK;
+}
+
+int lekf_set_infl(lekf_handle* h, const double* infl3d, const double* infl2d)
+{
+    if (!h || !infl3d) return LEKF_ERR_ARG;
+    const size_t npt = (size_t)h->nij * h->nlev;
+    // reference infl3d(nij,nlev) column-major == device point order lev*nij + ij
+    HIPCHK(hipMemcpyAsync(h->infl_in.p, infl3d, sizeof(double) * npt, hipMemcpyHostToDevice, h->st));
+    if (infl2d && h->nv2d)
+        HIPCHK(hipMemcpyAsync(h->infl2.p, infl2d, sizeof(double) * h->nij * h->nv2d,
+                              hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return LEKF_OK;
+}
+
+int lekf_set_obs_global(lekf_handle* h, int nobs, const double* elm, const double* lon,
+                        const double* lat, const double* lev, const double* dat,
+                        const double* err, const int* qc, const double* hxf)
+{
+    if (!h || h->kind != 0 || nobs < 0) return LEKF_ERR_ARG;
+    if (nobs > 0 && (!elm || !lon || !lat || !lev || !dat || !err || !hxf)) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const int ne = h->ne;
+    const size_t n = (size_t)(nobs > 0 ? nobs : 1);
+    int rc;
+    if ((rc = upload(h->r_elm, elm, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lon, lon, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lat, lat, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lev, lev, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_dat, dat, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_err, err, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_hxf, hxf, sizeof(double) * nobs * (size_t)ne, st)))
+        return rc;
+    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
+    long n2 = 2048;
+    while (n2 < nobs) n2 <<= 1;
+    const int ncell = h->g.nlon * h->g.nlat;
+    HIPCHK(h->keys.ensure(sizeof(unsigned long long) * n2));
+    HIPCHK(h->cell_cnt.ensure(sizeof(int) * ncell));
+    HIPCHK(h->nobsgrd.ensure(sizeof(int) * ncell));
+    HIPCHK(h->tmp_mean.ensure(sizeof(double) * n));
+    HIPCHK(h->tmp_dep.ensure(sizeof(double) * n));
+    HIPCHK(h->perm.ensure(sizeof(int) * n));
+    HIPCHK(h->o_hdxf.ensure(sizeof(double) * n * ne));
+    if (h->prec != LEKF_PREC_F64) HIPCHK(h->o_hdxf32.ensure(sizeof(float) * n * ne));
+    HIPCHK(h->o_dep.ensure(sizeof(double) * n));
+    HIPCHK(h->o_rdiag.ensure(sizeof(double) * n));
+    HIPCHK(h->o_err.ensure(sizeof(double) * n));
+    HIPCHK(h->o_typ.ensure(sizeof(int) * n));
+    HIPCHK(h->o_sin.ensure(sizeof(double) * n));
+    HIPCHK(h->o_cos.ensure(sizeof(double) * n));
+    HIPCHK(h->o_lonr.ensure(sizeof(double) * n));
+    HIPCHK(h->o_lnlev.ensure(sizeof(double) * n));
+    HIPCHK(h->o_lndat.ensure(sizeof(double) * n));
+    HIPCHK(hipMemsetAsync(h->cell_cnt.p, 0, sizeof(int) * ncell, st));
+    HIPCHK(hipMemsetAsync(h->keys.p, 0xff, sizeof(unsigned long long) * n2, st));
+    ObsPrepArgs A;
+    memset(&A, 0, sizeof(A));
+    A.nobs = nobs;
+    A.ne = ne;
+    A.nlon = h->g.nlon;
+    A.nlat = h->g.nlat;
+    A.lon = h->d_lon.as<double>();
+    A.lat = h->d_lat.as<double>();
+    A.elm = h->r_elm.as<double>();
+    A.olon = h->r_lon.as<double>();
+    A.olat = h->r_lat.as<double>();
+    A.olev = h->r_lev.as<double>();
+    A.dat = h->r_dat.as<double>();
+    A.err = h->r_err.as<double>();
+    A.qc = qc ? h->r_qc.as<int>() : nullptr;
+    A.hxf = h->r_hxf.as<double>();
+    A.h_so = 1;
+    A.h_sm = nobs;
+    A.gross_error = h->g.gross_error;
+    A.tmp_mean = h->tmp_mean.as<double>();
+    A.tmp_dep = h->tmp_dep.as<double>();
+    A.keys = h->keys.as<unsigned long long>();
+    A.cell_cnt = h->cell_cnt.as<int>();
+    A.perm = h->perm.as<int>();
+    A.o_hdxf = h->o_hdxf.as<double>();
+    A.o_hdxf32 = h->prec != LEKF_PREC_F64 ? h->o_hdxf32.as<float>() : nullptr;
+    A.o_dep = h->o_dep.as<double>();
+    A.o_rdiag = h->o_rdiag.as<double>();
+    A.o_err = h->o_err.as<double>();
+    A.o_typ = h->o_typ.as<int>();
+    A.o_sinlat = h->o_sin.as<double>();
+    A.o_coslat = h->o_cos.as<double>();
+    A.o_lonr = h->o_lonr.as<double>();
+    A.o_lnlev = h->o_lnlev.as<double>();
+    A.o_lndat = h->o_lndat.as<double>();
+    A.nobsgrd = h->nobsgrd.as<int>();
+    if (nobs > 0) HIPCHK(lekf_obs_prep_launch(A, st));
+    HIPCHK(lekf_sort_keys_launch(A.keys, n2, st));
+    // binned count = total of the cell histogram
+    std::vector<int> cc(ncell);
+    HIPCHK(hipMemcpyAsync(cc.data(), h->cell_cnt.p, sizeof(int) * ncell, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    long nb = 0;
+    for (int c : cc) nb += c;
+    h->nbinned = (int)nb;
+    h->nobs = nobs;
+    HIPCHK(lekf_obs_scatter_launch(A, h->nbinned, ncell, st));
+    // candidate capacity per column (depends on obs geometry only)
+    GlobalCullArgs C;
+    memset(&C, 0, sizeof(C));
+    C.nij = h->nij;
+    C.nlev = h->nlev;
+    C.nlon = h->g.nlon;
+    C.nlat = h->g.nlat;
+    C.lon = h->d_lon.as<double>();
+    C.lat = h->d_lat.as<double>();
+    C.col_lon = h->d_col_lon.as<double>();
+    C.col_lat = h->d_col_lat.as<double>();
+    C.col_dlon_zero = h->d_col_dlz.as<double>();
+    C.dlat_zero = h->dlat_zero;
+    C.nobsgrd = h->nobsgrd.as<int>();
+    C.cand_cnt = h->cand_cnt.as<int>();
+    C.cand_off = h->cand_off.as<long>();
+    HIPCHK(lekf_cull_global_count_launch(C, st));
+    long tot = 0;
+    HIPCHK(hipMemcpyAsync(&tot, h->cand_off.as<long>() + h->nij, sizeof(long), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    h->loc_cap = tot * h->nlev;
+    HIPCHK(h->loc_idx.ensure(sizeof(int) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
+    HIPCHK(h->loc_rho.ensure(sizeof(double) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
+    h->have_obs = true;
+    fill_job(h);
+    HIPCHK(h->djob.ensure(sizeof(lekf_job)));
+    HIPCHK(hipMemcpyAsync(h->djob.p, &h->job, sizeof(lekf_job), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LEKF_OK;
+}
+
+int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, const double* err,
+                     const double* hxf)
+{
+    if (!h || h->kind != 1 || ny < 0) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const int ne = h->ne;
+    const size_t n = (size_t)(ny > 0 ? ny : 1);
+    int rc;
+    if ((rc = upload(h->r_gidx, gidx, sizeof(int) * ny, st)) ||
+        (rc = upload(h->r_y, y, sizeof(double) * ny, st)) ||
+        (rc = upload(h->r_err, err, sizeof(double) * ny, st)) ||
+        (rc = upload(h->r_hxf, hxf, sizeof(double) * ny * (size_t)ne, st)))
+        return rc;
+    HIPCHK(h->o_hdxf.ensure(sizeof(double) * n * ne));
+    if (h->prec != LEKF_PREC_F64) HIPCHK(h->o_hdxf32.ensure(sizeof(float) * n * ne));
+    HIPCHK(h->o_dep.ensure(sizeof(double) * n));
+    HIPCHK(h->o_rdiag.ensure(sizeof(double) * n));
+    if (ny > 0) {
+        hipLaunchKernelGGL(l96_obs_kernel, dim3((ny + 255) / 256), dim3(256), 0, st, ny, ne,
+                           h->r_hxf.as<double>(), h->r_y.as<double>(), h->r_err.as<double>(),
+                           h->o_hdxf.as<double>(),
+                           h->prec != LEKF_PREC_F64 ? h->o_hdxf32.as<float>() : nullptr,
+                           h->o_dep.as<double>(), h->o_rdiag.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    h->nobs = ny;
+    h->nbinned = ny;
+    h->loc_cap = (long)h->nij * (ny > 0 ? ny : 1);
+    HIPCHK(h->loc_idx.ensure(sizeof(int) * h->loc_cap));
+    HIPCHK(h->loc_rho.ensure(sizeof(double) * h->loc_cap));
+    h->have_obs = true;
+    fill_job(h);
+    HIPCHK(h->djob.ensure(sizeof(lekf_job)));
+    HIPCHK(hipMemcpyAsync(h->djob.p, &h->job, sizeof(lekf_job), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LEKF_OK;
+}
+
+int lekf_das_run(lekf_handle* h)
+{
+    if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
+    int rc = run_select(h);
+    if (rc) return rc;
+    HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->job.npt, h->ne, mode_of_prec(h->prec),
+                             h->st));
+    return LEKF_OK;
+}
+
+int lekf_das_run_many(lekf_handle** hs, int n)
+{
+    if (!hs || n <= 0) return LEKF_ERR_ARG;
+    static std::mutex mu;
+    static DevBuf jobs_dev[3][8];
+    std::lock_guard<std::mutex> lk(mu);
+    hipStream_t st = hs[0]->st;
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i] || !hs[i]->have_ens || !hs[i]->have_obs) return LEKF_ERR_ARG;
+        if (hs[i]->st != st) {
+            // selection runs on the leading stream so the shared solve sees it
+            hipStream_t keep = hs[i]->st;
+            hs[i]->st = st;
+            int rc = run_select(hs[i]);
+            hs[i]->st = keep;
+            if (rc) return rc;
+        } else {
+            int rc = run_select(hs[i]);
+            if (rc) return rc;
+        }
+    }
+    // group by (precision, Gram tiles)
+    for (int mode = 0; mode < 3; ++mode)
+        for (int nt = 1; nt <= 6; ++nt) {
+            std::vector<lekf_job> J;
+            long tot = 0;
+            int kmax = 0;
+            for (int i = 0; i < n; ++i) {
+                if (mode_of_prec(hs[i]->prec) != mode || lekf_solve_nt(hs[i]->ne) != nt) continue;
+                lekf_job j = hs[i]->job;
+                j.pt_begin = tot;
+                tot += j.npt;
+                kmax = std::max(kmax, j.k);
+                J.push_back(j);
+            }
+            if (J.empty()) continue;
+            DevBuf& db = jobs_dev[mode][nt];
+            HIPCHK(db.ensure(sizeof(lekf_job) * J.size()));
+            HIPCHK(hipMemcpyAsync(db.p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice,
+                                  st));
+            HIPCHK(lekf_solve_launch(db.as<lekf_job>(), (int)J.size(), tot, kmax, mode, st));
+        }
+    HIPCHK(hipStreamSynchronize(st));
+    return LEKF_OK;
+}
+
+int lekf_sync(lekf_handle* h)
+{
+    if (!h) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return LEKF_OK;
+}
+
+int lekf_get_analysis(lekf_handle* h, double* anal3d, double* anal2d)
+{
+    if (!h || !anal3d) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const size_t n3 = (size_t)h->nij * h->nlev * h->ne * h->nv3d;
+    const size_t n2 = (size_t)h->nij * h->ne * h->nv2d;
+    HIPCHK(h->stage.ensure(sizeof(double) * std::max(n3, n2)));
+    hipLaunchKernelGGL(ens3_from_dev, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                       h->x3a.as<double>(), h->stage.as<double>(), h->nij, h->nlev, h->ne, h->nv3d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(anal3d, h->stage.p, sizeof(double) * n3, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (n2 && anal2d) {
+        hipLaunchKernelGGL(ens3_from_dev, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st,
+                           h->x2a.as<double>(), h->stage.as<double>(), h->nij, 1, h->ne, h->nv2d);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(anal2d, h->stage.p, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return LEKF_OK;
+}
+
+int lekf_get_infl(lekf_handle* h, double* infl3d, double* infl2d)
+{
+    if (!h || !infl3d) return LEKF_ERR_ARG;
+    const size_t npt = (size_t)h->nij * h->nlev;
+    HIPCHK(hipMemcpyAsync(infl3d, h->infl_out.p, sizeof(double) * npt, hipMemcpyDeviceToHost, h->st));
+    if (infl2d && h->nv2d)
+        HIPCHK(hipMemcpyAsync(infl2d, h->infl2.p, sizeof(double) * h->nij * h->nv2d,
+                              hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return LEKF_OK;
+}
+
+int lekf_get_nobsl(lekf_handle* h, int* nobsl)
+{
+    if (!h || !nobsl) return LEKF_ERR_ARG;
+    const size_t npt = (size_t)h->nij * h->nlev;
+    HIPCHK(hipMemcpyAsync(nobsl, h->loc_cnt.p, sizeof(int) * npt, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return LEKF_OK;
+}
+
+int lekf_get_local(lekf_handle* h, long b, int cap, int* idx, double* rho)
+{
+    if (!h || b < 0 || b >= (long)h->nij * h->nlev) return -LEKF_ERR_ARG;
+    long off = 0;
+    int cnt = 0;
+    if (hipMemcpy(&off, h->loc_off.as<long>() + b, sizeof(long), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&cnt, h->loc_cnt.as<int>() + b, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return -LEKF_ERR_HIP;
+    const int m = cnt < cap ? cnt : cap;
+    if (m > 0) {
+        if (hipMemcpy(idx, h->loc_idx.as<int>() + off, sizeof(int) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(rho, h->loc_rho.as<double>() + off, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
+            return -LEKF_ERR_HIP;
+    }
+    return cnt;
+}
+
+int lekf_get_obs_index(lekf_handle* h, int* perm, int* nobsgrd, int* nbinned)
+{
+    if (!h || h->kind != 0) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (nbinned) *nbinned = h->nbinned;
+    if (perm && h->nbinned)
+        HIPCHK(hipMemcpy(perm, h->perm.p, sizeof(int) * h->nbinned, hipMemcpyDeviceToHost));
+    if (nobsgrd)
+        HIPCHK(hipMemcpy(nobsgrd, h->nobsgrd.p, sizeof(int) * h->g.nlon * h->g.nlat,
+                         hipMemcpyDeviceToHost));
+    return LEKF_OK;
+}
+
+int lekf_get_flags(lekf_handle* h, long* n_rankdef, long* n_noconv)
+{
+    if (!h) return LEKF_ERR_ARG;
+    const size_t npt = (size_t)h->nij * h->nlev;
+    std::vector<int> s(npt);
+    HIPCHK(hipMemcpyAsync(s.data(), h->status.p, sizeof(int) * npt, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    long a = 0, b = 0;
+    for (int v : s) {
+        a += (v & LEKF_PT_RANKDEF) ? 1 : 0;
+        b += (v & LEKF_PT_NOCONV) ? 1 : 0;
+    }
+    if (n_rankdef) *n_rankdef = a;
+    if (n_noconv) *n_noconv = b;
+    return LEKF_OK;
+}
+
+int lekf_cycle_swap(lekf_handle* h)
+{
+    if (!h) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::swap(h->x3b, h->x3a);
+    std::swap(h->x2b, h->x2a);
+    std::swap(h->infl_in, h->infl_out);
+    if (h->have_obs) {
+        fill_job(h);
+        HIPCHK(hipMemcpyAsync(h->djob.p, &h->job, sizeof(lekf_job), hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    return LEKF_OK;
+}
+
+int lekf_device_state(lekf_handle* h, double** x3b, double** x3a, double** x2b, double** x2a)
+{
+    if (!h) return LEKF_ERR_ARG;
+    if (x3b) *x3b = h->x3b.as<double>();
+    if (x3a) *x3a = h->x3a.as<double>();
+    if (x2b) *x2b = h->x2b.as<double>();
+    if (x2a) *x2a = h->x2a.as<double>();
+    return LEKF_OK;
+}
+
+}  // extern "C"

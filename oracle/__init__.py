@@ -1,0 +1,399 @@
+"""The oracle: CPU restatement of the reference LETKF local analysis.
+
+TEST INFRASTRUCTURE.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker (or the timed CPU baseline).  The product package ``letkf_amd`` never
+imports it.
+
+Two legs:
+
+* ``impl="c"``   -- ``liblekf_oracle.so``, our C restatement (lekf_oracle.c),
+  each function citing the reference file:line it follows.
+* ``impl="ref"`` -- ``_ref/libletkf_ref.so``, the reference Fortran
+  (common/common_letkf.f90, common/common_mtx.f90, common/netlib.f,
+  common/netlibblas.f) compiled with amdflang from /root/reference by
+  oracle/Makefile, reached through our BIND(C) shim ref_bind.f90.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_long
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_C_LIB = os.path.join(_HERE, "liblekf_oracle.so")
+_REF_LIB = os.path.join(_HERE, "_ref", "libletkf_ref.so")
+
+_c = None
+_ref = None
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int)
+_lp = POINTER(c_long)
+
+# speedy/common/common_obs_speedy.f90:17-24
+ID_U, ID_V, ID_T, ID_Q, ID_RH, ID_PS, ID_RAIN = 2819, 2820, 3073, 3330, 3331, 14593, 9999
+PI_REF = 3.1415926535  # common/common.f90:28
+RE_REF = 6371.3e3  # common/common.f90:35
+
+
+def _lib_c():
+    global _c
+    if _c is None:
+        if not os.path.exists(_C_LIB):
+            raise RuntimeError(f"oracle library missing: {_C_LIB} (run make -C oracle)")
+        _c = ctypes.CDLL(_C_LIB)
+        _c.orc_letkf_core.restype = c_int
+        _c.orc_letkf_core.argtypes = [c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, c_int, _dp]
+        _c.orc_mtx_eigen.restype = c_int
+        _c.orc_mtx_eigen.argtypes = [c_int, _dp, _dp, _dp, _ip]
+        _c.orc_distll_1.restype = c_double
+        _c.orc_distll_1.argtypes = [c_double] * 4
+        _c.orc_obs_local_l96.restype = c_int
+        _c.orc_obs_local_l96.argtypes = [c_int, c_int, _ip, c_int, c_double, _ip, _dp]
+        _c.orc_obs_bins_global.restype = c_int
+        _c.orc_obs_bins_global.argtypes = [c_int, _dp, _dp, c_int, c_int, _dp, _dp, _ip, _ip]
+        _c.orc_rect_candidates.restype = c_int
+        _c.orc_rect_candidates.argtypes = [c_double, c_double, c_double, c_double, c_int, c_int,
+                                           _dp, _dp, _ip, _ip]
+        _c.orc_obs_local_global.restype = c_int
+        _c.orc_obs_local_global.argtypes = (
+            [c_double] * 5 + [c_int, c_int, c_int, _dp, _dp, _ip] + [_dp] * 6 +
+            [c_double] * 4 + [_dp, _ip, _ip, _dp, _dp])
+        _c.orc_obs_local_regional.restype = c_int
+        _c.orc_obs_local_regional.argtypes = (
+            [c_int, _ip, _dp, _dp, _dp, _ip, _dp, c_int] + [c_double] * 6 +
+            [c_int, _ip, _dp, _ip, _ip, _dp, _dp])
+        _c.orc_das_global.restype = c_int
+        _c.orc_das_global.argtypes = (
+            [c_int] * 5 + [_dp] * 4 + [_dp, _dp, _dp, c_int, _dp, _dp, c_int, c_int, _dp, _dp,
+                                       c_int, _ip] + [_dp] * 8 + [c_double, c_double, _dp, c_int,
+                                                                  c_int, _ip])
+        _c.orc_phys2ij.restype = c_int
+        _c.orc_phys2ij.argtypes = [c_double, c_double, c_int, c_int, _dp, _dp, _dp]
+        _c.orc_itpl_2d.restype = c_double
+        _c.orc_itpl_2d.argtypes = [_dp, c_int, c_double, c_double]
+        _c.orc_itpl_3d.restype = c_double
+        _c.orc_itpl_3d.argtypes = [_dp, c_int, c_int, c_double, c_double, c_double]
+        _c.orc_trans_xtoy.restype = c_int
+        _c.orc_trans_xtoy.argtypes = ([c_int] + [c_double] * 4 + [c_int] * 3 + [_dp] * 4 +
+                                      [_dp] * 4)
+    return _c
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF_LIB)
+
+
+def _lib_ref():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(_REF_LIB):
+            raise RuntimeError(f"reference library missing: {_REF_LIB} (make -C oracle ref)")
+        _ref = ctypes.CDLL(_REF_LIB)
+        _ref.ref_letkf_core.restype = None
+        _ref.ref_letkf_core.argtypes = [c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        c_int, _dp, c_int, c_int, _dp, c_int]
+        _ref.ref_mtx_eigen.restype = None
+        _ref.ref_mtx_eigen.argtypes = [c_int, _dp, _dp, _dp, _ip]
+        _ref.ref_solve_apply.restype = c_int
+        _ref.ref_solve_apply.argtypes = [c_int, c_int, c_int, c_int, _lp, _ip, _dp, _dp, _dp,
+                                         _dp, c_int, _dp, _dp, _dp, c_double]
+        _ref.ref_max_threads.restype = c_int
+        _ref.ref_max_threads.argtypes = []
+    return _ref
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def letkf_core(hdxb, rdiag, rloc, dep, parm_infl, *, transm=False, pao=False, rdiag_wloc=False,
+               minfl=None, impl="c"):
+    """letkf_core (common/common_letkf.f90:50-243) on one local problem.
+
+    hdxb: (nobsl, ne) array of obs-space perturbations (row = obs).
+    Returns dict(trans, transm, pao, parm_infl, status).
+    """
+    hdxb = np.asarray(hdxb, dtype=np.float64)
+    p, ne = hdxb.shape
+    ld = max(p, 1)
+    h = np.zeros((ne, ld), dtype=np.float64)  # column-major (ld, ne) == row-major (ne, ld)
+    h[:, :p] = hdxb.T
+    rd = np.ones(ld)
+    rl = np.ones(ld)
+    dp = np.zeros(ld)
+    rd[:p] = rdiag
+    rl[:p] = rloc
+    dp[:p] = dep
+    trans = np.zeros((ne, ne))
+    tm = np.zeros(ne)
+    pa = np.zeros((ne, ne))
+    parm = np.array([parm_infl], dtype=np.float64)
+    mf = np.array([minfl if minfl is not None else 0.0])
+    if impl == "c":
+        st = _lib_c().orc_letkf_core(ne, ld, p, _d(h), _d(rd), _d(rl), _d(dp), _d(parm),
+                                     _d(trans), _d(tm) if transm else None,
+                                     _d(pa) if pao else None, int(bool(rdiag_wloc)),
+                                     _d(mf) if minfl is not None else None)
+    elif impl == "ref":
+        _lib_ref().ref_letkf_core(ne, ld, p, _d(h), _d(rd), _d(rl), _d(dp), _d(parm), _d(trans),
+                                  _d(tm), int(bool(transm)), _d(pa), int(bool(pao)),
+                                  int(bool(rdiag_wloc)), _d(mf), int(minfl is not None))
+        st = 0
+    else:
+        raise ValueError(impl)
+    # column-major (ne, ne) buffers: transpose the row-major view back
+    return dict(trans=trans.T.copy(), transm=tm if transm else None,
+                pao=pa.T.copy() if pao else None, parm_infl=float(parm[0]), status=int(st))
+
+
+def mtx_eigen(a, impl="c"):
+    """mtx_eigen (common/common_mtx.f90:41-99): (eival desc, eivec columns, nrank, status)."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    af = _f64(a.T)  # column-major
+    ev = np.zeros(n)
+    vec = np.zeros((n, n))
+    nr = np.zeros(1, dtype=np.int32)
+    if impl == "c":
+        st = _lib_c().orc_mtx_eigen(n, _d(af), _d(ev), _d(vec), _i(nr))
+    else:
+        _lib_ref().ref_mtx_eigen(n, _d(af), _d(ev), _d(vec), _i(nr))
+        st = 0
+    return ev, vec.T.copy(), int(nr[0]), int(st)
+
+
+def distll_1(alon, alat, blon, blat):
+    return _lib_c().orc_distll_1(alon, alat, blon, blat)
+
+
+def obs_local_l96(nx, obs_gidx, ix, xlocal):
+    """lorenz96/letkf/letkf.f90:216-254 selection: (idx, rloc)."""
+    g = np.ascontiguousarray(obs_gidx, dtype=np.int32)
+    ny = g.size
+    idx = np.zeros(max(ny, 1), dtype=np.int32)
+    rl = np.zeros(max(ny, 1))
+    n = _lib_c().orc_obs_local_l96(nx, ny, _i(g), ix, xlocal, _i(idx), _d(rl))
+    return idx[:n].copy(), rl[:n].copy()
+
+
+def obs_bins_global(olon, olat, lon, lat):
+    """set_letkf_obs binning (speedy/letkf/letkf_obs.f90:243-313): (perm, nobsgrd[nlat, nlon])."""
+    olon = _f64(olon)
+    olat = _f64(olat)
+    lon = _f64(lon)
+    lat = _f64(lat)
+    nobs = olon.size
+    perm = np.zeros(max(nobs, 1), dtype=np.int32)
+    grd = np.zeros((lat.size, lon.size), dtype=np.int32)  # column-major (nlon, nlat)
+    n = _lib_c().orc_obs_bins_global(nobs, _d(olon), _d(olat), lon.size, lat.size, _d(lon), _d(lat),
+                                     _i(perm), _i(grd))
+    return perm[:n].copy(), grd
+
+
+def set_letkf_obs_global(obs, lon, lat, gross_error, qc=None):
+    """set_letkf_obs (speedy/letkf/letkf_obs.f90:146-313) on raw obs records.
+
+    obs: dict elm, lon, lat, lev, dat, err (nobs) and hxf (nobs, ne) = H(x) per member.
+    Ensemble mean of H(x) in member order (:160-163), perturbations (:164-166),
+    departure (:167), gross-error QC (:168-170), then the stable lat-band /
+    lon-cell binning (obs_bins_global).  Returns (sorted obs dict with dep and
+    hdxf (ne, nobs) C-order, nobsgrd (nlat, nlon), perm).
+    """
+    hx = np.asarray(obs["hxf"], dtype=np.float64)
+    ne = hx.shape[1]
+    m = hx[:, 0].copy()
+    for i in range(1, ne):
+        m = m + hx[:, i]
+    m = m / float(ne)
+    dep = np.asarray(obs["dat"], dtype=np.float64) - m
+    keep = ~(np.abs(dep) > gross_error * np.asarray(obs["err"]))
+    if qc is not None:
+        keep &= np.asarray(qc) == 1
+    kidx = np.flatnonzero(keep)
+    pk, grd = obs_bins_global(np.asarray(obs["lon"])[kidx], np.asarray(obs["lat"])[kidx], lon, lat)
+    perm = kidx[pk]
+    s = {k: np.asarray(obs[k], dtype=np.float64)[perm] for k in ("elm", "lon", "lat", "lev", "dat",
+                                                                 "err")}
+    s["dep"] = dep[perm]
+    s["hdxf"] = np.ascontiguousarray((hx - m[:, None])[perm].T)
+    return s, grd, perm
+
+
+def rect_candidates(lon1, lat1, dlon_zero, dlat_zero, lon, lat, nobsgrd, nobs):
+    lon = _f64(lon)
+    lat = _f64(lat)
+    g = np.ascontiguousarray(nobsgrd, dtype=np.int32)
+    use = np.zeros(max(nobs, 1) * 2, dtype=np.int32)
+    n = _lib_c().orc_rect_candidates(lon1, lat1, dlon_zero, dlat_zero, lon.size, lat.size, _d(lon),
+                                     _d(lat), _i(g), _i(use))
+    return use[:n].copy()
+
+
+def obs_local_global(lon1, lat1, dlon_zero, dlat_zero, logpfm, ilev, lon, lat, nobsgrd, obs,
+                     dist_zero, dist_zerov, sigma_obs, sigma_obsv, var_local_row=None):
+    """obs_local (speedy/letkf/letkf_tools.f90:272-573): (idx, rloc, rdiag)."""
+    lon = _f64(lon)
+    lat = _f64(lat)
+    g = np.ascontiguousarray(nobsgrd, dtype=np.int32)
+    nobs = obs["elm"].size
+    vl = _f64(np.ones(7) if var_local_row is None else var_local_row)
+    use = np.zeros(max(nobs, 1) * 2, dtype=np.int32)
+    idx = np.zeros(max(nobs, 1), dtype=np.int32)
+    rl = np.zeros(max(nobs, 1))
+    rd = np.zeros(max(nobs, 1))
+    arrs = [_f64(obs[k]) for k in ("elm", "lon", "lat", "lev", "dat", "err")]
+    n = _lib_c().orc_obs_local_global(lon1, lat1, dlon_zero, dlat_zero, logpfm, ilev, lon.size,
+                                      lat.size, _d(lon), _d(lat), _i(g), *[_d(a) for a in arrs],
+                                      dist_zero, dist_zerov, sigma_obs, sigma_obsv, _d(vl), _i(use),
+                                      _i(idx), _d(rl), _d(rd))
+    return idx[:n].copy(), rl[:n].copy(), rd[:n].copy()
+
+
+def obs_local_regional(cand, ox, oy, oz, otyp, oerr, ntyp, px, py, pz, sigma_h, sigma_v, nd_max,
+                       nobs_max):
+    """Regional selection with the per-type cap (scale/letkf/letkf_tools.f90:853-1178)."""
+    cand = np.ascontiguousarray(cand, dtype=np.int32)
+    nc = cand.size
+    cap = max(nobs_max, 1)
+    bucket = np.zeros(ntyp * cap, dtype=np.int32)
+    bkey = np.zeros(ntyp * cap)
+    bcount = np.zeros(ntyp, dtype=np.int32)
+    idx = np.zeros(max(nc, 1), dtype=np.int32)
+    rl = np.zeros(max(nc, 1))
+    rd = np.zeros(max(nc, 1))
+    ox, oy, oz, oerr = (_f64(a) for a in (ox, oy, oz, oerr))
+    ot = np.ascontiguousarray(otyp, dtype=np.int32)
+    n = _lib_c().orc_obs_local_regional(nc, _i(cand), _d(ox), _d(oy), _d(oz), _i(ot), _d(oerr),
+                                        ntyp, px, py, pz, sigma_h, sigma_v, nd_max, nobs_max,
+                                        _i(bucket), _d(bkey), _i(bcount), _i(idx), _d(rl), _d(rd))
+    return idx[:n].copy(), rl[:n].copy(), rd[:n].copy()
+
+
+def das_global(prob, *, q_top_lev=5, iq=3, var_local_row=None, return_nobsl=False):
+    """das_letkf (speedy/letkf/letkf_tools.f90:45-267) on a global problem dict.
+
+    prob keys (numpy, reference column-major shapes expressed C-order reversed):
+      gues3d (nv3d, ne, nlev, nij), gues2d (nv2d, ne, nij), lon1, lat1 (nij), sig (nlev),
+      lon (nlon), lat (nlat), nobsgrd (nlat, nlon), obs dict with sorted arrays and
+      hdxf (ne, nobs), infl3d (nlev, nij), infl2d (nv2d, nij), sigma_obs, sigma_obsv, ips.
+    Returns (anal3d, anal2d, infl3d, infl2d[, nobsl (nlev, nij)]).
+    """
+    g3 = _f64(prob["gues3d"]).copy()
+    g2 = _f64(prob["gues2d"]).copy()
+    nv3d, ne, nlev, nij = g3.shape
+    nv2d = g2.shape[0]
+    a3 = np.zeros_like(g3)
+    a2 = np.zeros_like(g2)
+    i3 = _f64(prob["infl3d"]).copy()
+    i2 = _f64(prob["infl2d"]).copy()
+    obs = prob["obs"]
+    nobs = obs["elm"].size
+    hd = _f64(obs["hdxf"])
+    vl = _f64(np.ones(7) if var_local_row is None else var_local_row)
+    nl = np.zeros((nlev, nij), dtype=np.int32)
+    lon = _f64(prob["lon"])
+    lat = _f64(prob["lat"])
+    arrs = [_f64(obs[k]) for k in ("elm", "lon", "lat", "lev", "dat", "err", "dep")]
+    grd = np.ascontiguousarray(prob["nobsgrd"], dtype=np.int32)
+    st = _lib_c().orc_das_global(nij, nlev, ne, nv3d, nv2d, _d(g3), _d(g2), _d(a3), _d(a2),
+                                 _d(_f64(prob["lon1"])), _d(_f64(prob["lat1"])),
+                                 _d(_f64(prob["sig"])), int(prob.get("ips", 0)), _d(i3), _d(i2),
+                                 lon.size, lat.size, _d(lon), _d(lat), nobs, _i(grd),
+                                 *[_d(a) for a in arrs], _d(hd), float(prob["sigma_obs"]),
+                                 float(prob["sigma_obsv"]), _d(vl), int(q_top_lev), int(iq),
+                                 _i(nl))
+    if st != 0:
+        raise RuntimeError(f"oracle das_global status {st}")
+    out = (a3, a2, i3, i2)
+    return out + (nl,) if return_nobsl else out
+
+
+def das_global_problem(p, *, q_top_lev=5, iq=3, infl3d=None):
+    """set_letkf_obs + das_letkf on a letkf_amd.synth global problem dict.
+
+    Returns dict(anal3d (nij,nlev,ne,nv3d), anal2d (nij,ne,nv2d), infl3d (nij,nlev),
+    nobsl (nij,nlev), perm, nobsgrd (nlat,nlon), sobs) in the reference layouts.
+    """
+    s, grd, perm = set_letkf_obs_global(p["obs"], p["lon"], p["lat"], p["gross_error"])
+    nij, nlev = p["nij"], p["nlev"]
+    nv2d = p["gues2d"].shape[2]
+    i3 = np.full((nlev, nij), p["infl"]) if infl3d is None else np.ascontiguousarray(infl3d.T)
+    prob = dict(gues3d=np.ascontiguousarray(p["gues3d"].transpose(3, 2, 1, 0)),
+                gues2d=np.ascontiguousarray(p["gues2d"].transpose(2, 1, 0)),
+                lon1=np.tile(p["lon"], p["nlat"]), lat1=np.repeat(p["lat"], p["nlon"]),
+                sig=p["sig"], lon=p["lon"], lat=p["lat"], nobsgrd=grd, obs=s, infl3d=i3,
+                infl2d=np.full((nv2d, nij), p["infl"]), sigma_obs=p["sigma_obs"],
+                sigma_obsv=p["sigma_obsv"], ips=p.get("ips", 0))
+    a3, a2, o3, o2, nl = das_global(prob, q_top_lev=q_top_lev, iq=iq, return_nobsl=True)
+    return dict(anal3d=np.asfortranarray(a3.transpose(3, 2, 1, 0)),
+                anal2d=np.asfortranarray(a2.transpose(2, 1, 0)), infl3d=o3.T.copy(),
+                nobsl=nl.T.copy(), perm=perm, nobsgrd=grd, sobs=s)
+
+
+def phys2ij(rlon, rlat, nlon, lat):
+    """phys2ijk lon/lat part (speedy/common/common_obs_speedy.f90:156-175): (status, ri, rj)."""
+    lat = _f64(lat)
+    ri = np.zeros(1)
+    rj = np.zeros(1)
+    st = _lib_c().orc_phys2ij(rlon, rlat, nlon, lat.size, _d(lat), _d(ri), _d(rj))
+    return st, float(ri[0]), float(rj[0])
+
+
+def trans_xtoy(elm, rlon, rlat, rlev, odat, lat, sig, v3d, ps):
+    """Trans_XtoY + phys2ijk + obsope range rules for one member.
+
+    v3d: (4, nlev, nlat, nlon) C-order (== column-major nlon x nlat x nlev x 4), ps (nlat, nlon).
+    Returns (status, yobs, ri, rj, rk).
+    """
+    lat = _f64(lat)
+    sig = _f64(sig)
+    v3d = _f64(v3d)
+    ps = _f64(ps)
+    nlon = ps.shape[1]
+    y = np.zeros(1)
+    ri = np.zeros(1)
+    rj = np.zeros(1)
+    rk = np.zeros(1)
+    st = _lib_c().orc_trans_xtoy(int(elm), rlon, rlat, rlev, odat, nlon, lat.size, sig.size,
+                                 _d(lat), _d(sig), _d(v3d), _d(ps), _d(y), _d(ri), _d(rj), _d(rk))
+    return st, float(y[0]), float(ri[0]), float(rj[0]), float(rk[0])
+
+
+def ref_solve_apply(loc_off, loc_idx, loc_rho, hdxf, rdiag, dep, xb, infl, fixed_infl=0.0):
+    """Reference letkf_core + apply over packed local problems (OpenMP over points).
+
+    hdxf: (nobs, ne) row = obs; xb: (npt, nv, ne).  Returns (xa, infl_out).
+    """
+    lib = _lib_ref()
+    npt = loc_off.size - 1
+    nobs, ne = hdxf.shape
+    nv = xb.shape[1]
+    off = np.ascontiguousarray(loc_off, dtype=np.int64)
+    idx = np.ascontiguousarray(loc_idx, dtype=np.int32)
+    rho = _f64(loc_rho)
+    h = _f64(np.asarray(hdxf).T)  # (ne, nobs) C-order == column-major (nobs, ne)
+    xbf = _f64(xb)
+    xa = np.zeros_like(xbf)
+    inf = _f64(infl).copy()
+    maxp = int(np.max(np.diff(off))) if npt else 1
+    lib.ref_solve_apply(npt, ne, nobs, max(maxp, 1), off.ctypes.data_as(_lp), _i(idx), _d(rho),
+                        _d(h), _d(_f64(rdiag)), _d(_f64(dep)), nv, _d(xbf), _d(xa), _d(inf),
+                        float(fixed_infl))
+    return xa, inf
+
+
+def ref_threads() -> int:
+    return int(_lib_ref().ref_max_threads())
