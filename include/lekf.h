@@ -20,6 +20,7 @@
  */
 #ifndef LEKF_H
 #define LEKF_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -173,6 +174,11 @@ int lekf_set_obs_l96(lekf_handle *h, int ny, const int *gidx, const double *y, c
 /* one analysis (selection + local solves + apply) on the handle's stream;
  * asynchronous.  The analysis stays on the device. */
 int lekf_das_run(lekf_handle *h);
+/* the two halves of lekf_das_run: selection kernels, then the local solves + apply */
+int lekf_das_select(lekf_handle *h);
+int lekf_das_solve(lekf_handle *h);
+/* run the handle's work on a caller stream (hipStream_t; NULL = default stream) */
+int lekf_set_stream(lekf_handle *h, void *stream);
 /* batch the local solves of several handles into shared launches */
 int lekf_das_run_many(lekf_handle **hs, int n);
 int lekf_sync(lekf_handle *h);
@@ -195,9 +201,27 @@ long lekf_device_bytes(lekf_handle *h);
 /* hipStream_t of the handle */
 void *lekf_stream(lekf_handle *h);
 
+/* --- 5. device memory, streams and events ----------------------------------
+ * For hosts without a GPU runtime of their own (Fortran, Python/numpy):
+ * the batched entry points take device pointers from here.  No counterpart
+ * in the reference, which has no device.  kind: 0 host->device, 1
+ * device->host (both synchronous), 2 device->device (asynchronous). */
+void *lekf_dev_malloc(size_t bytes);
+int lekf_dev_free(void *p);
+int lekf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int lekf_memset(void *dst, int value, size_t bytes, void *stream);
+void *lekf_stream_create(void);
+int lekf_stream_destroy(void *stream);
+int lekf_stream_sync(void *stream);
+void *lekf_event_create(void);
+int lekf_event_destroy(void *ev);
+int lekf_event_record(void *ev, void *stream);
+int lekf_event_elapsed_ms(void *ev0, void *ev1, float *ms);
+
 /* library info */
 int lekf_device_count(void);
 const char *lekf_version(void);
+const char *lekf_last_error(void);
 
 #ifdef __cplusplus
 }

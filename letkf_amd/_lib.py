@@ -129,6 +129,12 @@ def lib():
     L.lekf_set_obs_l96.argtypes = [c_void_p, c_int, _ip, _dp, _dp, _dp]
     L.lekf_das_run.restype = c_int
     L.lekf_das_run.argtypes = [c_void_p]
+    L.lekf_das_select.restype = c_int
+    L.lekf_das_select.argtypes = [c_void_p]
+    L.lekf_das_solve.restype = c_int
+    L.lekf_das_solve.argtypes = [c_void_p]
+    L.lekf_set_stream.restype = c_int
+    L.lekf_set_stream.argtypes = [c_void_p, c_void_p]
     L.lekf_das_run_many.restype = c_int
     L.lekf_das_run_many.argtypes = [POINTER(c_void_p), c_int]
     L.lekf_sync.restype = c_int
@@ -153,6 +159,28 @@ def lib():
     L.lekf_device_bytes.argtypes = [c_void_p]
     L.lekf_stream.restype = c_void_p
     L.lekf_stream.argtypes = [c_void_p]
+    L.lekf_dev_malloc.restype = c_void_p
+    L.lekf_dev_malloc.argtypes = [ctypes.c_size_t]
+    L.lekf_dev_free.restype = c_int
+    L.lekf_dev_free.argtypes = [c_void_p]
+    L.lekf_memcpy.restype = c_int
+    L.lekf_memcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int, c_void_p]
+    L.lekf_memset.restype = c_int
+    L.lekf_memset.argtypes = [c_void_p, c_int, ctypes.c_size_t, c_void_p]
+    L.lekf_stream_create.restype = c_void_p
+    L.lekf_stream_create.argtypes = []
+    L.lekf_stream_destroy.restype = c_int
+    L.lekf_stream_destroy.argtypes = [c_void_p]
+    L.lekf_stream_sync.restype = c_int
+    L.lekf_stream_sync.argtypes = [c_void_p]
+    L.lekf_event_create.restype = c_void_p
+    L.lekf_event_create.argtypes = []
+    L.lekf_event_destroy.restype = c_int
+    L.lekf_event_destroy.argtypes = [c_void_p]
+    L.lekf_event_record.restype = c_int
+    L.lekf_event_record.argtypes = [c_void_p, c_void_p]
+    L.lekf_event_elapsed_ms.restype = c_int
+    L.lekf_event_elapsed_ms.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
     _lib = L
     return L
 
@@ -162,10 +190,12 @@ EXPORTED = [
     "lekf_core_f64", "lekf_core_batch", "lekf_obs_local_l96", "lekf_global_create",
     "lekf_l96_create", "lekf_regional_create", "lekf_destroy", "lekf_set_ensemble",
     "lekf_set_infl", "lekf_set_obs_global", "lekf_set_obs_l96", "lekf_set_obs_regional",
-    "lekf_das_run", "lekf_das_run_many", "lekf_sync", "lekf_get_analysis", "lekf_get_infl",
+    "lekf_das_run", "lekf_das_select", "lekf_das_solve", "lekf_set_stream", "lekf_das_run_many", "lekf_sync", "lekf_get_analysis", "lekf_get_infl",
     "lekf_get_nobsl", "lekf_get_local", "lekf_get_obs_index", "lekf_get_flags",
     "lekf_cycle_swap", "lekf_device_state", "lekf_device_bytes", "lekf_stream",
-    "lekf_device_count", "lekf_version", "lekf_last_error",
+    "lekf_device_count", "lekf_version", "lekf_last_error", "lekf_dev_malloc", "lekf_dev_free",
+    "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
+    "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
 ]
 
 

@@ -77,9 +77,16 @@ def build_fortran(verbose: bool = False) -> str | None:
     os.makedirs(mod, exist_ok=True)
     tmp = FLIB + ".tmp"
     cmd = [fc, "-O2", "-fPIC", "-shared", "-module-dir", mod, *srcs, "-L", HERE, "-llekf_hip",
-           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+           "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
     _run(cmd, verbose)
     os.replace(tmp, FLIB)
+    # the Lorenz-96 driver program of the Fortran host side (used by the tests)
+    exe = os.path.join(FSRC, "l96_driver")
+    cmd = [fc, "-O2", "-I", mod, os.path.join(FSRC, "l96_driver.f90"), "-L", FSRC,
+           "-llekf_fortran", "-L", HERE, "-llekf_hip", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(exe + ".tmp", exe)
     return FLIB
 
 

@@ -159,6 +159,81 @@ extern "C" {
 
 const char* lekf_version(void) { return LEKF_VERSION; }
 
+// --- device memory, streams and events for hosts without a GPU runtime of their own
+void* lekf_dev_malloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
+    return p;
+}
+
+int lekf_dev_free(void* p)
+{
+    HIPCHK(hipFree(p));
+    return LEKF_OK;
+}
+
+int lekf_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream)
+{
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+    if (kind != 2) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return LEKF_OK;
+}
+
+int lekf_memset(void* dst, int value, size_t bytes, void* stream)
+{
+    HIPCHK(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return LEKF_OK;
+}
+
+void* lekf_stream_create(void)
+{
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    return (void*)s;
+}
+
+int lekf_stream_destroy(void* stream)
+{
+    HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    return LEKF_OK;
+}
+
+int lekf_stream_sync(void* stream)
+{
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return LEKF_OK;
+}
+
+void* lekf_event_create(void)
+{
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return (void*)e;
+}
+
+int lekf_event_destroy(void* ev)
+{
+    HIPCHK(hipEventDestroy((hipEvent_t)ev));
+    return LEKF_OK;
+}
+
+int lekf_event_record(void* ev, void* stream)
+{
+    HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return LEKF_OK;
+}
+
+int lekf_event_elapsed_ms(void* ev0, void* ev1, float* ms)
+{
+    HIPCHK(hipEventSynchronize((hipEvent_t)ev1));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)ev0, (hipEvent_t)ev1));
+    return LEKF_OK;
+}
+
 const char* lekf_last_error(void) { return g_errmsg; }
 
 int lekf_device_count(void)
@@ -293,8 +368,9 @@ int lekf_obs_local_l96(int nx, int ny, const int* gidx, double xlocal, long* loc
 // das_letkf-level handle
 // ===========================================================================
 struct lekf_handle {
-    int kind = 0;  // 0 global, 1 Lorenz-96
+    int kind = 0;  // 0 global, 1 Lorenz-96, 2 regional
     hipStream_t st = nullptr;
+    bool own_stream = true;
     int ne = 0, nij = 0, nlev = 1, nv3d = 1, nv2d = 0;
     int prec = 0;
     int infl_mode = 0;
@@ -595,7 +671,7 @@ void lekf_destroy(lekf_handle* h)
     if (!h) return;
     (void)hipStreamSynchronize(h->st);
     free_all(h);
-    (void)hipStreamDestroy(h->st);
+    if (h->own_stream) (void)hipStreamDestroy(h->st);
     delete h;
 }
 
@@ -797,13 +873,34 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
     return LEKF_OK;
 }
 
-int lekf_das_run(lekf_handle* h)
+int lekf_das_select(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
-    int rc = run_select(h);
-    if (rc) return rc;
+    return run_select(h);
+}
+
+int lekf_das_solve(lekf_handle* h)
+{
+    if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
     HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->job.npt, h->ne, mode_of_prec(h->prec),
                              h->st));
+    return LEKF_OK;
+}
+
+int lekf_das_run(lekf_handle* h)
+{
+    int rc = lekf_das_select(h);
+    if (rc) return rc;
+    return lekf_das_solve(h);
+}
+
+int lekf_set_stream(lekf_handle* h, void* stream)
+{
+    if (!h) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (h->own_stream) HIPCHK(hipStreamDestroy(h->st));
+    h->own_stream = false;
+    h->st = (hipStream_t)stream;
     return LEKF_OK;
 }
 

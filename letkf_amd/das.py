@@ -56,6 +56,16 @@ class _Handle:
     def run(self):
         check(_lib.lib().lekf_das_run(self.h), "das_run")
 
+    def select(self):
+        check(_lib.lib().lekf_das_select(self.h), "das_select")
+
+    def solve(self):
+        check(_lib.lib().lekf_das_solve(self.h), "das_solve")
+
+    def set_stream(self, stream):
+        """Run on a caller stream (an int hipStream_t, e.g. torch.cuda.current_stream().cuda_stream)."""
+        check(_lib.lib().lekf_set_stream(self.h, stream), "set_stream")
+
     def sync(self):
         check(_lib.lib().lekf_sync(self.h), "sync")
 

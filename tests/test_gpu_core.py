@@ -81,7 +81,7 @@ def test_bad_sizes_raise(gpu):
 
 def test_batch_matches_single_and_reference(gpu):
     """Many uneven local problems in one launch; each equals its lone solve bit for bit."""
-    import torch
+    from letkf_amd.device import DeviceArray, sync
     k = 20
     ps = [0, 1, 3, 17, 64, 150, 333, 5, 200, 90]
     nobs = 2000
@@ -96,23 +96,22 @@ def test_batch_matches_single_and_reference(gpu):
         idx.append(sel)
         rho.append(rng.uniform(0.01, 1.0, p))
         off.append(off[-1] + p)
-    dev = torch.device("cuda:0")
-    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)  # noqa
+    t = DeviceArray.from_numpy
     idx_all = np.concatenate(idx).astype(np.int32)
     rho_all = np.concatenate(rho)
     dY, dR, dD = t(Y), t(R), t(D)
-    dOff = t(np.array(off), torch.int64)
-    dIdx, dRho = t(idx_all, torch.int32), t(rho_all)
+    dOff = t(np.array(off), np.int64)
+    dIdx, dRho = t(idx_all, np.int32), t(rho_all)
     npt = len(ps)
     infl = t(np.full(npt, 1.03))
-    infl_out = torch.zeros(npt, dtype=torch.float64, device=dev)
-    trans = torch.zeros(npt * k * k, dtype=torch.float64, device=dev)
+    infl_out = DeviceArray.zeros(npt)
+    trans = DeviceArray.zeros(npt * k * k)
     job = core.make_job(k=k, npt=npt, hdxf=dY, rdiag=dR, dep=dD, loc_off=dOff, loc_idx=dIdx,
                         loc_rho=dRho, infl_in=infl, infl_out=infl_out, trans=trans)
     core.letkf_core_batch([job])
-    torch.cuda.synchronize()
-    tb = trans.cpu().numpy().reshape(npt, k, k)
-    io = infl_out.cpu().numpy()
+    sync()
+    tb = trans.numpy().reshape(npt, k, k)
+    io = infl_out.numpy()
     for b, p in enumerate(ps):
         sel = idx[b]
         ref = oracle.letkf_core(Y[sel], R[sel], rho[b], D[sel], 1.03, impl=_impl())
