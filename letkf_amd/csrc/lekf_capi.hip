@@ -142,6 +142,34 @@ int core_init()
     return 0;
 }
 
+// Launch the solves of jobs (all of one arithmetic mode) grouped by kernel
+// class: descriptors are uploaded once, sorted by class, and each class is
+// one launch whose points are numbered from 0.
+int launch_grouped(std::vector<lekf_job> J, int mode, DevBuf& db, hipStream_t st)
+{
+    std::stable_sort(J.begin(), J.end(), [](const lekf_job& a, const lekf_job& b) {
+        return lekf_solve_class(a.k) < lekf_solve_class(b.k);
+    });
+    HIPCHK(db.ensure(sizeof(lekf_job) * J.size()));
+    std::vector<size_t> starts;
+    for (size_t i = 0; i < J.size(); ++i)
+        if (i == 0 || lekf_solve_class(J[i].k) != lekf_solve_class(J[i - 1].k)) starts.push_back(i);
+    starts.push_back(J.size());
+    std::vector<long> tot(starts.size(), 0);
+    std::vector<int> kmax(starts.size(), 0);
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        for (size_t i = starts[g]; i < starts[g + 1]; ++i) {
+            J[i].pt_begin = tot[g];
+            tot[g] += J[i].npt;
+            kmax[g] = std::max(kmax[g], J[i].k);
+        }
+    HIPCHK(hipMemcpyAsync(db.p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice, st));
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        HIPCHK(lekf_solve_launch(db.as<lekf_job>() + starts[g], (int)(starts[g + 1] - starts[g]),
+                                 tot[g], kmax[g], mode, st));
+    return 0;
+}
+
 int ensure_iota(DevBuf& b, int& have, int n, hipStream_t st)
 {
     if (n <= have) return 0;
@@ -335,13 +363,9 @@ int lekf_core_batch(const lekf_job* jobs, int njobs, int prec, void* stream)
         kmax = std::max(kmax, j.k);
     }
     if (tot == 0) return LEKF_OK;
-    if (lekf_solve_nt(kmax) > 6) return LEKF_ERR_SIZE;
     DevBuf*& db = t_core.jobs_by_stream[st];
     if (!db) db = new DevBuf();
-    HIPCHK(db->ensure(sizeof(lekf_job) * J.size()));
-    HIPCHK(hipMemcpyAsync(db->p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(lekf_solve_launch(db->as<lekf_job>(), njobs, tot, kmax, mode, st));
-    return LEKF_OK;
+    return launch_grouped(J, mode, *db, st);
 }
 
 int lekf_obs_local_l96(int nx, int ny, const int* gidx, double xlocal, long* loc_off, int* loc_cnt,
@@ -908,7 +932,7 @@ int lekf_das_run_many(lekf_handle** hs, int n)
 {
     if (!hs || n <= 0) return LEKF_ERR_ARG;
     static std::mutex mu;
-    static DevBuf jobs_dev[3][8];
+    static DevBuf jobs_dev[3];
     std::lock_guard<std::mutex> lk(mu);
     hipStream_t st = hs[0]->st;
     for (int i = 0; i < n; ++i) {
@@ -925,27 +949,15 @@ int lekf_das_run_many(lekf_handle** hs, int n)
             if (rc) return rc;
         }
     }
-    // group by (precision, Gram tiles)
-    for (int mode = 0; mode < 3; ++mode)
-        for (int nt = 1; nt <= 6; ++nt) {
-            std::vector<lekf_job> J;
-            long tot = 0;
-            int kmax = 0;
-            for (int i = 0; i < n; ++i) {
-                if (mode_of_prec(hs[i]->prec) != mode || lekf_solve_nt(hs[i]->ne) != nt) continue;
-                lekf_job j = hs[i]->job;
-                j.pt_begin = tot;
-                tot += j.npt;
-                kmax = std::max(kmax, j.k);
-                J.push_back(j);
-            }
-            if (J.empty()) continue;
-            DevBuf& db = jobs_dev[mode][nt];
-            HIPCHK(db.ensure(sizeof(lekf_job) * J.size()));
-            HIPCHK(hipMemcpyAsync(db.p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice,
-                                  st));
-            HIPCHK(lekf_solve_launch(db.as<lekf_job>(), (int)J.size(), tot, kmax, mode, st));
-        }
+    // one grouped launch set per arithmetic mode
+    for (int mode = 0; mode < 3; ++mode) {
+        std::vector<lekf_job> J;
+        for (int i = 0; i < n; ++i)
+            if (mode_of_prec(hs[i]->prec) == mode) J.push_back(hs[i]->job);
+        if (J.empty()) continue;
+        const int rc = launch_grouped(J, mode, jobs_dev[mode], st);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(st));
     return LEKF_OK;
 }

@@ -77,5 +77,6 @@ hipError_t lekf_logpfm_launch(const double* x2b, int nij, int nv2d, int ips, int
 // solve (lekf_solve.hip)
 size_t lekf_solve_lds_bytes(int k);
 int lekf_solve_nt(int k);
+int lekf_solve_class(int k);
 hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int mode,
                              hipStream_t st);

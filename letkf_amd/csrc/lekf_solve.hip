@@ -116,6 +116,26 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Reciprocal and reciprocal square root: hardware estimate refined by Newton
+// steps to full FP64 accuracy (a Jacobi rotation only needs c^2 + s^2 = 1 to
+// rounding, which an accurate c = (1+t^2)^-1/2 and s = t c give).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Gram products.  Lane l holds A[i = l%16][kk = l/16] and B[kk][j = l%16] of
 // each 16x16x4 step (16x16x32 BF16: kk = 8*(l/16) .. +7).
@@ -126,6 +146,7 @@ __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int 
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
+#pragma unroll 2
     for (int q0 = 0; q0 < p; q0 += 4) {
         const int q = q0 + kq;
         const bool valid = q < p;
@@ -175,6 +196,7 @@ __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int 
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
+#pragma unroll 2
     for (int q0 = 0; q0 < p; q0 += 4) {
         const int q = q0 + kq;
         const bool valid = q < p;
@@ -338,10 +360,12 @@ __device__ int jacobi_lds(SolveLds& L, int k, int lane)
                 const double app = L.A[pp * lda + pp];
                 const double aqq = L.A[qq * lda + qq];
                 const double apq = L.A[pp * lda + qq];
-                if (fabs(apq) > LEKF_JACOBI_TOL * sqrt(fabs(app * aqq))) {
-                    const double th = (aqq - app) / (2.0 * apq);
-                    t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-                    c = 1.0 / sqrt(1.0 + t * t);
+                if (apq * apq > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * fabs(app * aqq)) {
+                    const double th = (aqq - app) * rcp_nr(2.0 * apq);
+                    const double q = fma(th, th, 1.0);
+                    const double at = fabs(th) + q * rsq_nr(q);
+                    t = th >= 0.0 ? rcp_nr(at) : -rcp_nr(at);
+                    c = rsq_nr(fma(t, t, 1.0));
                     s = t * c;
                     L.flag[0] = 1;
                 }
@@ -414,7 +438,100 @@ __device__ int jacobi_lds(SolveLds& L, int k, int lane)
     return sweep < LEKF_JACOBI_MAX_SWEEPS ? sweep : -1;
 }
 
-template <int NT, int MODE>  // MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3
+// Partner of index j in round r of the circle-method tournament on n (even)
+// indices: the same pairing as rr_pair, inverted.
+__device__ __forceinline__ int rr_partner(int n, int r, int j)
+{
+    const int m = n - 1;
+    if (j == 0) return 1 + (r % m);
+    const int x = (j - 1 - (r % m) + m) % m;
+    return x == 0 ? 0 : 1 + ((m - x) + r) % m;
+}
+
+// One-sided (Hestenes) cyclic Jacobi with the matrix in registers, for k <= 32.
+// Lanes 0..31 hold the columns of A (initially C, so A V -> U S with S = the
+// eigenvalues of the SPD C); lanes 32..63 hold the columns of V (initially I).
+// Each round pairs every column with its tournament partner; a lane fetches
+// its partner's column with a cross-lane shuffle, the A lanes form
+// [alpha gamma; gamma beta] = the 2x2 Gram of the pair and the rotation that
+// zeroes gamma (the same rotation two-sided Jacobi would apply to C'C), and
+// both halves apply it to their own column.  KR = rows carried (k rounded up
+// to even), rows and columns >= k are zero and never rotate.
+// Writes the eigenvalues to L.L and V to L.V (k x k, lda).  Returns the
+// number of sweeps or -1 if the sweep limit was reached.
+template <int KR>
+__device__ int jacobi_reg32(SolveLds& L, int k, int lane, int lda)
+{
+    const int half = lane >> 5, j = lane & 31;
+    double x[KR];
+#pragma unroll
+    for (int i = 0; i < KR; ++i) {
+        if (half == 0)
+            x[i] = (j < k && i < k) ? L.A[i * lda + j] : 0.0;
+        else
+            x[i] = (i == j && j < k) ? 1.0 : 0.0;
+    }
+    double alpha = 0.0;
+#pragma unroll
+    for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
+    int sweep = 0;
+    for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
+        bool rotated = false;
+        for (int r = 0; r < KR - 1; ++r) {
+            const int pj = rr_partner(KR, r, j);
+            const int src = (half << 5) | pj;
+            double y[KR];
+#pragma unroll
+            for (int i = 0; i < KR; ++i) y[i] = __shfl(x[i], src, 64);
+            double gamma = 0.0;
+#pragma unroll
+            for (int i = 0; i < KR; ++i) gamma += x[i] * y[i];
+            const double beta = __shfl(alpha, pj, 64);
+            const bool lo = j < pj;
+            const double app = lo ? alpha : beta, aqq = lo ? beta : alpha;
+            double c = 1.0, s = 0.0, t = 0.0;
+            const bool rot =
+                gamma * gamma > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * (app * aqq);
+            if (rot) {
+                const double th = (aqq - app) * rcp_nr(2.0 * gamma);
+                const double q = fma(th, th, 1.0);
+                const double at = fabs(th) + q * rsq_nr(q);
+                t = th >= 0.0 ? rcp_nr(at) : -rcp_nr(at);
+                c = rsq_nr(fma(t, t, 1.0));
+                s = t * c;
+            }
+            // the V lanes take the rotation of their column from the A lane
+            c = __shfl(c, j, 64);
+            s = __shfl(s, j, 64);
+            const double sg = lo ? -s : s;
+#pragma unroll
+            for (int i = 0; i < KR; ++i) x[i] = c * x[i] + sg * y[i];
+            if (rot) {
+                alpha = lo ? app - t * gamma : aqq + t * gamma;
+                rotated = true;
+            }
+        }
+        if (__ballot(rotated && half == 0) == 0ull) break;
+        // refresh the column norms once per sweep
+        alpha = 0.0;
+#pragma unroll
+        for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
+    }
+    double nrm = 0.0;
+#pragma unroll
+    for (int i = 0; i < KR; ++i) nrm += x[i] * x[i];
+    if (half == 0 && j < k) L.L[j] = sqrt(nrm);
+    if (half == 1 && j < k) {
+#pragma unroll
+        for (int i = 0; i < KR; ++i)
+            if (i < k) L.V[i * lda + j] = x[i];
+    }
+    return sweep < LEKF_JACOBI_MAX_SWEEPS ? sweep : -1;
+}
+
+// NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32), or
+// 0 for the LDS Jacobi (k > 32); MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3.
+template <int NT, int KR, int MODE>
 __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restrict__ jobs, int njobs)
 {
     extern __shared__ double smem[];
@@ -434,21 +551,24 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     const bool transm_mode = (J.flags & LEKF_F_TRANSM) != 0;
     int status = 0;
 
-    // zero A (dummy row/col) and set V = I; block table
-    for (int w = lane; w < n * lda; w += LEKF_WAVE) L.A[w] = 0.0;
-    for (int w = lane; w < k * lda; w += LEKF_WAVE) {
-        const int r = w / lda, c = w - r * lda;
-        L.V[w] = (r == c) ? 1.0 : 0.0;
-    }
-    for (int w = lane; w < nblk; w += LEKF_WAVE) {
-        int i = 0, rem = w;
-        while (rem >= np - i) {
-            rem -= np - i;
-            ++i;
+    // LDS Jacobi (k > 32): zero A (dummy row/col), V = I, block table
+    constexpr bool reg_jacobi = KR > 0;
+    if (!reg_jacobi) {
+        for (int w = lane; w < n * lda; w += LEKF_WAVE) L.A[w] = 0.0;
+        for (int w = lane; w < k * lda; w += LEKF_WAVE) {
+            const int r = w / lda, c = w - r * lda;
+            L.V[w] = (r == c) ? 1.0 : 0.0;
         }
-        L.blk[w] = (i << 16) | (i + rem);
+        for (int w = lane; w < nblk; w += LEKF_WAVE) {
+            int i = 0, rem = w;
+            while (rem >= np - i) {
+                rem -= np - i;
+                ++i;
+            }
+            L.blk[w] = (i << 16) | (i + rem);
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     if (p > 0) {
         double p3 = 0.0;
@@ -482,19 +602,24 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
         }
         __syncthreads();
         alpha = L.sc[3];
-        const int sweeps = jacobi_lds(L, k, lane);
+        int sweeps;
+        if constexpr (reg_jacobi) {
+            sweeps = jacobi_reg32<(KR > 0 ? KR : 4)>(L, k, lane, lda);
+        } else {
+            sweeps = jacobi_lds(L, k, lane);
+            __syncthreads();
+            for (int j = lane; j < k; j += LEKF_WAVE) L.L[j] = L.A[j * lda + j];
+        }
         if (sweeps < 0) status |= LEKF_PT_NOCONV;
         __syncthreads();
         // eigenvalues; reference truncation test (common_mtx.f90:66-74)
         double lmax = 0.0, lmin = 1.0e308;
         for (int j = 0; j < k; ++j) {
-            const double lj = L.A[j * lda + j];
+            const double lj = L.L[j];
             lmax = lj > lmax ? lj : lmax;
             lmin = lj < lmin ? lj : lmin;
         }
         if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
-        for (int j = lane; j < k; j += LEKF_WAVE) L.L[j] = L.A[j * lda + j];
-        __syncthreads();
         // u = L^-1 V^T b ; w-bar = V u
         for (int j = lane; j < k; j += LEKF_WAVE) {
             double s = 0.0;
@@ -619,38 +744,57 @@ size_t lekf_solve_lds_bytes(int k) { return solve_lds_doubles(k) * sizeof(double
 
 int lekf_solve_nt(int k) { return (k + 1 + 15) / 16; }
 
-template <int NT, int MODE>
-static hipError_t launch_nt(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
+// Kernel class of an ensemble size: KR * 8 + NT.  k <= 32 uses the register
+// Jacobi with KR = k rounded up to a multiple of 4; larger k the LDS Jacobi.
+int lekf_solve_class(int k)
+{
+    if (k <= 32) {
+        const int kr = (k + 3) & ~3;
+        const int nt = kr <= 12 ? 1 : (kr <= 28 ? 2 : 3);
+        return kr * 8 + nt;
+    }
+    return lekf_solve_nt(k);
+}
+
+template <int NT, int KR, int MODE>
+static hipError_t launch_k(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
 {
     const size_t lds = lekf_solve_lds_bytes(kmax);
-    hipLaunchKernelGGL((lekf_solve_kernel<NT, MODE>), dim3((unsigned)npt), dim3(LEKF_WAVE), lds, st,
-                       djobs, njobs);
+    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt), dim3(LEKF_WAVE), lds,
+                       st, djobs, njobs);
     return hipGetLastError();
 }
 
 template <int MODE>
-static hipError_t launch_mode(int nt, const LekfJob* djobs, int njobs, long npt, int kmax,
+static hipError_t launch_mode(int cls, const LekfJob* djobs, int njobs, long npt, int kmax,
                               hipStream_t st)
 {
-    switch (nt) {
-    case 1: return launch_nt<1, MODE>(djobs, njobs, npt, kmax, st);
-    case 2: return launch_nt<2, MODE>(djobs, njobs, npt, kmax, st);
-    case 3: return launch_nt<3, MODE>(djobs, njobs, npt, kmax, st);
-    case 4: return launch_nt<4, MODE>(djobs, njobs, npt, kmax, st);
-    case 5: return launch_nt<5, MODE>(djobs, njobs, npt, kmax, st);
-    case 6: return launch_nt<6, MODE>(djobs, njobs, npt, kmax, st);
+    switch (cls) {
+    case 4 * 8 + 1: return launch_k<1, 4, MODE>(djobs, njobs, npt, kmax, st);
+    case 8 * 8 + 1: return launch_k<1, 8, MODE>(djobs, njobs, npt, kmax, st);
+    case 12 * 8 + 1: return launch_k<1, 12, MODE>(djobs, njobs, npt, kmax, st);
+    case 16 * 8 + 2: return launch_k<2, 16, MODE>(djobs, njobs, npt, kmax, st);
+    case 20 * 8 + 2: return launch_k<2, 20, MODE>(djobs, njobs, npt, kmax, st);
+    case 24 * 8 + 2: return launch_k<2, 24, MODE>(djobs, njobs, npt, kmax, st);
+    case 28 * 8 + 2: return launch_k<2, 28, MODE>(djobs, njobs, npt, kmax, st);
+    case 32 * 8 + 3: return launch_k<3, 32, MODE>(djobs, njobs, npt, kmax, st);
+    case 3: return launch_k<3, 0, MODE>(djobs, njobs, npt, kmax, st);
+    case 4: return launch_k<4, 0, MODE>(djobs, njobs, npt, kmax, st);
+    case 5: return launch_k<5, 0, MODE>(djobs, njobs, npt, kmax, st);
+    case 6: return launch_k<6, 0, MODE>(djobs, njobs, npt, kmax, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 // Launch the solve over npt flat points of njobs jobs (device array djobs).
-// kmax = largest k among the jobs (sets LDS); nt = Gram tiles for kmax.
+// Every job of a launch has the same kernel class; kmax (the largest k) sets
+// the LDS size.
 hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int mode,
                              hipStream_t st)
 {
     if (npt <= 0) return hipSuccess;
-    const int nt = lekf_solve_nt(kmax);
-    if (mode == 0) return launch_mode<0>(nt, djobs, njobs, npt, kmax, st);
-    if (mode == 1) return launch_mode<1>(nt, djobs, njobs, npt, kmax, st);
-    return launch_mode<2>(nt, djobs, njobs, npt, kmax, st);
+    const int cls = lekf_solve_class(kmax);
+    if (mode == 0) return launch_mode<0>(cls, djobs, njobs, npt, kmax, st);
+    if (mode == 1) return launch_mode<1>(cls, djobs, njobs, npt, kmax, st);
+    return launch_mode<2>(cls, djobs, njobs, npt, kmax, st);
 }
