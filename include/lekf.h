@@ -73,6 +73,9 @@ extern "C" {
  */
 typedef struct lekf_job {
     long pt_begin;
+    long pt0;                /* global index of the job's first point: per-point arrays
+                                (loc_off, loc_cnt, infl, outputs, state rows) are
+                                indexed by pt0 + local point */
     int npt;
     int k;
     int flags;
@@ -154,7 +157,26 @@ typedef struct lekf_l96_cfg {
     double infl_value;
 } lekf_l96_cfg;
 
+/* Regional cartesian grid (SCALE-style, one subdomain): point (i, j, lev) at
+ * grid units ri = i, rj = j (1-based) and height zlev[lev]; radar-type
+ * vertical distance |z_o - z| (scale/letkf/letkf_tools.f90:992-1000). */
+typedef struct lekf_regional_cfg {
+    int nx, ny, nlev;
+    double dx, dy;            /* grid spacing (m) */
+    const double *zlev;       /* nlev level heights (m) */
+    int ne;
+    int nv3d;                 /* state variables (all 3-D) */
+    double sigma_h, sigma_v;  /* localisation scales (m) */
+    int nobs_max;             /* MAX_NOBS_PER_GRID per bucket; <= 0: no cap */
+    int prec;
+    int infl_mode;
+    double infl_value;
+    double minfl;             /* INFL_MUL_MIN; <= 0: off */
+    long list_budget;         /* local-list entries held at once (0: library default) */
+} lekf_regional_cfg;
+
 lekf_handle *lekf_global_create(const lekf_global_cfg *cfg, int *status);
+lekf_handle *lekf_regional_create(const lekf_regional_cfg *cfg, int *status);
 lekf_handle *lekf_l96_create(const lekf_l96_cfg *cfg, int *status);
 void lekf_destroy(lekf_handle *h);
 
@@ -168,6 +190,13 @@ int lekf_set_infl(lekf_handle *h, const double *infl3d, const double *infl2d);
 int lekf_set_obs_global(lekf_handle *h, int nobs, const double *elm, const double *lon,
                         const double *lat, const double *lev, const double *dat,
                         const double *err, const int *qc, const double *hxf);
+/* Regional obs after the obs operator (SCALE's obsda): grid-unit position
+ * (ri, rj), height z (m), error, departure dep = y - mean H(x), bucket
+ * (the (element, obs type) index, 0..3), qc (NULL: all good) and the obs-space
+ * perturbations hdxf(nobs,ne) column-major (scale/letkf/letkf_obs.f90:733-830). */
+int lekf_set_obs_regional(lekf_handle *h, int nobs, const double *ri, const double *rj,
+                          const double *z, const double *err, const double *dep, const int *bucket,
+                          const int *qc, const double *hdxf);
 /* Lorenz-96 obs: grid index gidx (0-based), value y, error err, hxf(ny,ne) */
 int lekf_set_obs_l96(lekf_handle *h, int ny, const int *gidx, const double *y, const double *err,
                      const double *hxf);
@@ -186,9 +215,12 @@ int lekf_get_analysis(lekf_handle *h, double *anal3d, double *anal2d);
 int lekf_get_infl(lekf_handle *h, double *infl3d, double *infl2d);
 int lekf_get_nobsl(lekf_handle *h, int *nobsl);
 /* local list of point b (= lev*nij + ij): up to cap entries of sorted-obs index,
- * rho; returns the count (or a negative status) */
+ * rho; returns the count (or a negative status).  A regional handle whose lists
+ * were built in several chunks selects the chunk of b again first. */
 int lekf_get_local(lekf_handle *h, long b, int cap, int *idx, double *rho);
-/* sorted-obs permutation (input index of each binned obs) and nobsgrd(nlon,nlat) */
+/* sorted-obs permutation (input index of each binned obs) and the cumulative
+ * obs count through each cell: nobsgrd(nlon,nlat) for the global grid,
+ * (nx,ny) for the regional grid */
 int lekf_get_obs_index(lekf_handle *h, int *perm, int *nobsgrd, int *nbinned);
 /* number of points flagged rank-deficient / not converged in the last run */
 int lekf_get_flags(lekf_handle *h, long *n_rankdef, long *n_noconv);

@@ -49,7 +49,8 @@ _lp = POINTER(c_long)
 
 class LekfJob(Structure):
     _fields_ = [
-        ("pt_begin", c_long), ("npt", c_int), ("k", c_int), ("flags", c_int), ("nij", c_int),
+        ("pt_begin", c_long), ("pt0", c_long), ("npt", c_int), ("k", c_int), ("flags", c_int),
+        ("nij", c_int),
         ("nv3d", c_int), ("nv2d", c_int), ("minfl", c_double), ("infl_fixed", c_double),
         ("hdxf", c_void_p), ("y_so", c_long), ("y_sm", c_long), ("hdxf32", c_void_p),
         ("rdiag", c_void_p), ("dep", c_void_p), ("loc_off", c_void_p), ("loc_cnt", c_void_p),
@@ -80,7 +81,7 @@ class RegionalCfg(Structure):
         ("nx", c_int), ("ny", c_int), ("nlev", c_int), ("dx", c_double), ("dy", c_double),
         ("zlev", _dp), ("ne", c_int), ("nv3d", c_int), ("sigma_h", c_double),
         ("sigma_v", c_double), ("nobs_max", c_int), ("prec", c_int), ("infl_mode", c_int),
-        ("infl_value", c_double), ("minfl", c_double),
+        ("infl_value", c_double), ("minfl", c_double), ("list_budget", c_long),
     ]
 
 

@@ -14,6 +14,7 @@
 
 #include "lekf_cull.hpp"
 #include "lekf_dev.hpp"
+#include "lekf_regional.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
 
@@ -421,6 +422,16 @@ struct lekf_handle {
     DevBuf djob;
     lekf_job job{};
     long bytes = 0;
+    // regional grid (kind 2)
+    lekf_regional_cfg r{};
+    std::vector<double> zlev;
+    int nbkt = 1;
+    double dlon_zero = 0;  // regional rectangle half-width in grid units (dlat_zero above)
+    DevBuf d_zlev, cellcum, cap_n, cap_off, o_ri, o_rj, o_z, o_bkt, r_dep, r_bkt, r_z;
+    std::vector<long> chunk_pts;  // chunk boundaries (points)
+    std::vector<long> chunk_off;  // list offsets at the boundaries
+    long list_budget = 512L * 1024 * 1024;
+    int held_chunk = -1;  // chunk whose lists the list buffer holds
 };
 
 namespace {
@@ -436,7 +447,9 @@ long handle_bytes(const lekf_handle* h)
                           &h->o_hdxf32, &h->o_dep, &h->o_rdiag, &h->o_err, &h->o_typ, &h->o_sin,
                           &h->o_cos, &h->o_lonr, &h->o_lnlev, &h->o_lndat, &h->nobsgrd,
                           &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx,
-                          &h->loc_rho, &h->status, &h->djob};
+                          &h->loc_rho, &h->status, &h->djob, &h->d_zlev, &h->cellcum,
+                          &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
+                          &h->r_dep, &h->r_bkt, &h->r_z};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -452,7 +465,8 @@ void free_all(lekf_handle* h)
                     &h->o_hdxf, &h->o_hdxf32, &h->o_dep, &h->o_rdiag, &h->o_err, &h->o_typ,
                     &h->o_sin, &h->o_cos, &h->o_lonr, &h->o_lnlev, &h->o_lndat, &h->nobsgrd,
                     &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
-                    &h->status, &h->djob};
+                    &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
+                    &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z};
     for (auto* b : bs) b->release();
 }
 
@@ -527,6 +541,72 @@ void fill_job(lekf_handle* h)
     J.x2b = h->nv2d ? h->x2b.as<double>() : nullptr;
     J.x2a = h->nv2d ? h->x2a.as<double>() : nullptr;
     J.upd_mask = h->upd.as<uint32_t>();
+    if (h->kind == 2 && h->r.minfl > 0.0) {  // INFL_MUL_MIN (scale/letkf/letkf_tools.f90:199-205)
+        J.flags |= LEKF_F_MINFL;
+        J.minfl = h->r.minfl;
+    }
+}
+
+// device job descriptors: one per chunk of points for the regional kind
+int upload_jobs(lekf_handle* h)
+{
+    fill_job(h);
+    std::vector<lekf_job> J;
+    if (h->kind == 2) {
+        for (size_t c = 0; c + 1 < h->chunk_pts.size(); ++c) {
+            lekf_job j = h->job;
+            j.pt0 = h->chunk_pts[c];
+            j.npt = (int)(h->chunk_pts[c + 1] - h->chunk_pts[c]);
+            J.push_back(j);
+        }
+    } else {
+        J.push_back(h->job);
+    }
+    HIPCHK(h->djob.ensure(sizeof(lekf_job) * J.size()));
+    HIPCHK(hipMemcpyAsync(h->djob.p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+RegionalArgs make_reg_args(const lekf_handle* h)
+{
+    RegionalArgs R;
+    memset(&R, 0, sizeof(R));
+    R.nx = h->r.nx;
+    R.ny = h->r.ny;
+    R.nlev = h->nlev;
+    R.nbkt = h->nbkt;
+    R.nobs_max = h->r.nobs_max;
+    R.dx = h->r.dx;
+    R.dy = h->r.dy;
+    R.sigma_h = h->r.sigma_h;
+    R.sigma_v = h->r.sigma_v;
+    R.dlon_zero = h->dlon_zero;
+    R.dlat_zero = h->dlat_zero;
+    R.zlev = h->d_zlev.as<double>();
+    R.cellcum = h->cellcum.as<int>();
+    R.ori = h->o_ri.as<double>();
+    R.orj = h->o_rj.as<double>();
+    R.oz = h->o_z.as<double>();
+    R.oerr = h->o_err.as<double>();
+    R.obkt = h->o_bkt.as<int>();
+    R.list_off = h->cap_off.as<long>();
+    R.loc_off = h->loc_off.as<long>();
+    R.loc_cnt = h->loc_cnt.as<int>();
+    R.loc_idx = h->loc_idx.as<int>();
+    R.loc_rho = h->loc_rho.as<double>();
+    return R;
+}
+
+int reg_select_chunk(lekf_handle* h, int c)
+{
+    RegionalArgs R = make_reg_args(h);
+    R.pt_begin = h->chunk_pts[c];
+    R.list_base = h->chunk_off[c];
+    HIPCHK(lekf_reg_select_launch(R, h->chunk_pts[c + 1] - h->chunk_pts[c], h->st));
+    h->held_chunk = c;
+    return 0;
 }
 
 int run_select(lekf_handle* h)
@@ -900,12 +980,22 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
 int lekf_das_select(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
+    if (h->kind == 2) {
+        if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;  // chunked: use lekf_das_run
+        return reg_select_chunk(h, 0);
+    }
     return run_select(h);
 }
 
 int lekf_das_solve(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
+    if (h->kind == 2) {
+        if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;
+        HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->chunk_pts[1], h->ne,
+                                 mode_of_prec(h->prec), h->st));
+        return LEKF_OK;
+    }
     HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->job.npt, h->ne, mode_of_prec(h->prec),
                              h->st));
     return LEKF_OK;
@@ -913,9 +1003,180 @@ int lekf_das_solve(lekf_handle* h)
 
 int lekf_das_run(lekf_handle* h)
 {
+    if (h && h->kind == 2 && h->have_ens && h->have_obs) {
+        // chunks of points whose local lists fit the list buffer: select, then solve
+        for (size_t c = 0; c + 1 < h->chunk_pts.size(); ++c) {
+            const int rc = reg_select_chunk(h, (int)c);
+            if (rc) return rc;
+            HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>() + c, 1,
+                                     h->chunk_pts[c + 1] - h->chunk_pts[c], h->ne,
+                                     mode_of_prec(h->prec), h->st));
+        }
+        return LEKF_OK;
+    }
     int rc = lekf_das_select(h);
     if (rc) return rc;
     return lekf_das_solve(h);
+}
+
+lekf_handle* lekf_regional_create(const lekf_regional_cfg* cfg, int* status)
+{
+    auto fail = [&](int c) -> lekf_handle* {
+        if (status) *status = c;
+        return nullptr;
+    };
+    if (!cfg || cfg->nx < 1 || cfg->ny < 1 || cfg->nlev < 1 || cfg->ne < 2 || cfg->ne > LEKF_KMAX ||
+        cfg->nv3d < 1 || cfg->nv3d > LEKF_NVMAX || !cfg->zlev || !(cfg->dx > 0) || !(cfg->dy > 0) ||
+        !(cfg->sigma_h > 0) || !(cfg->sigma_v > 0))
+        return fail(LEKF_ERR_SIZE);
+    if (lekf_device_count() < 1) return fail(LEKF_ERR_NODEV);
+    lekf_handle* h = new lekf_handle();
+    h->kind = 2;
+    h->r = *cfg;
+    h->zlev.assign(cfg->zlev, cfg->zlev + cfg->nlev);
+    h->r.zlev = h->zlev.data();
+    h->ne = cfg->ne;
+    h->nij = cfg->nx * cfg->ny;
+    h->nlev = cfg->nlev;
+    h->nv3d = cfg->nv3d;
+    h->nv2d = 0;
+    h->prec = cfg->prec;
+    h->infl_mode = cfg->infl_mode;
+    h->infl_value = cfg->infl_value;
+    if (cfg->list_budget > 0) h->list_budget = cfg->list_budget;
+    // search rectangle half-widths in grid units (scale/letkf/letkf_obs.f90:122-123)
+    const double dist_zero_fac = 3.651483717;
+    h->dlon_zero = cfg->sigma_h * dist_zero_fac / cfg->dx;
+    h->dlat_zero = cfg->sigma_h * dist_zero_fac / cfg->dy;
+    // the selection kernel keeps the rectangle's row segments in LDS
+    if (std::min<double>(cfg->ny, 2.0 * h->dlat_zero + 3.0) > LEKF_REG_SEG) {
+        delete h;
+        return fail(LEKF_ERR_SIZE);
+    }
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(LEKF_ERR_HIP);
+    }
+    int rc = upload(h->d_zlev, cfg->zlev, sizeof(double) * cfg->nlev, h->st);
+    if (!rc) rc = alloc_state(h);
+    if (rc) {
+        free_all(h);
+        (void)hipStreamDestroy(h->st);
+        delete h;
+        return fail(rc);
+    }
+    if (status) *status = 0;
+    return h;
+}
+
+int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const double* rj,
+                          const double* z, const double* err, const double* dep, const int* bucket,
+                          const int* qc, const double* hdxf)
+{
+    if (!h || h->kind != 2 || nobs < 0) return LEKF_ERR_ARG;
+    if (nobs > 0 && (!ri || !rj || !z || !err || !dep || !bucket || !hdxf)) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const int ne = h->ne, nx = h->r.nx, ny = h->r.ny;
+    int nb = 1;
+    for (int n = 0; n < nobs; ++n) {
+        if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
+        nb = std::max(nb, bucket[n] + 1);
+    }
+    h->nbkt = nb;
+    const size_t n1 = (size_t)(nobs > 0 ? nobs : 1);
+    int rc;
+    if ((rc = upload(h->r_lon, ri, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lat, rj, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_z, z, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_err, err, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_dep, dep, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_bkt, bucket, sizeof(int) * nobs, st)) ||
+        (rc = upload(h->r_hxf, hdxf, sizeof(double) * nobs * (size_t)ne, st)))
+        return rc;
+    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
+    long n2 = 2048;
+    while (n2 < nobs) n2 <<= 1;
+    const int ncell = nx * ny;
+    HIPCHK(h->keys.ensure(sizeof(unsigned long long) * n2));
+    HIPCHK(h->cell_cnt.ensure(sizeof(int) * ncell));
+    HIPCHK(h->cellcum.ensure(sizeof(int) * ncell));
+    HIPCHK(h->perm.ensure(sizeof(int) * n1));
+    HIPCHK(h->o_ri.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_rj.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_z.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_err.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_rdiag.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_dep.ensure(sizeof(double) * n1));
+    HIPCHK(h->o_bkt.ensure(sizeof(int) * n1));
+    HIPCHK(h->o_hdxf.ensure(sizeof(double) * n1 * ne));
+    if (h->prec != LEKF_PREC_F64) HIPCHK(h->o_hdxf32.ensure(sizeof(float) * n1 * ne));
+    HIPCHK(hipMemsetAsync(h->cell_cnt.p, 0, sizeof(int) * ncell, st));
+    HIPCHK(hipMemsetAsync(h->keys.p, 0xff, sizeof(unsigned long long) * n2, st));
+    RegObsArgs A;
+    memset(&A, 0, sizeof(A));
+    A.nobs = nobs;
+    A.ne = ne;
+    A.nx = nx;
+    A.ny = ny;
+    A.ri = h->r_lon.as<double>();
+    A.rj = h->r_lat.as<double>();
+    A.z = h->r_z.as<double>();
+    A.err = h->r_err.as<double>();
+    A.dep = h->r_dep.as<double>();
+    A.bkt = h->r_bkt.as<int>();
+    A.qc = qc ? h->r_qc.as<int>() : nullptr;
+    A.hdxf = h->r_hxf.as<double>();
+    A.h_so = 1;
+    A.h_sm = nobs;
+    A.keys = h->keys.as<unsigned long long>();
+    A.cell_cnt = h->cell_cnt.as<int>();
+    A.perm = h->perm.as<int>();
+    A.o_ri = h->o_ri.as<double>();
+    A.o_rj = h->o_rj.as<double>();
+    A.o_z = h->o_z.as<double>();
+    A.o_err = h->o_err.as<double>();
+    A.o_rdiag = h->o_rdiag.as<double>();
+    A.o_dep = h->o_dep.as<double>();
+    A.o_hdxf = h->o_hdxf.as<double>();
+    A.o_hdxf32 = h->prec != LEKF_PREC_F64 ? h->o_hdxf32.as<float>() : nullptr;
+    A.o_bkt = h->o_bkt.as<int>();
+    HIPCHK(lekf_reg_keys_launch(A, st));
+    HIPCHK(lekf_sort_keys_launch(A.keys, n2, st));
+    std::vector<int> cc(ncell);
+    HIPCHK(hipMemcpyAsync(cc.data(), h->cell_cnt.p, sizeof(int) * ncell, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    long nbin = 0;
+    for (int c : cc) nbin += c;
+    h->nbinned = (int)nbin;
+    h->nobs = nobs;
+    HIPCHK(lekf_reg_scatter_launch(A, h->nbinned, ncell, h->cellcum.as<int>(), st));
+    // list capacity per point -> chunks of points whose lists fit the budget
+    const long npt = (long)h->nij * h->nlev;
+    HIPCHK(h->cap_n.ensure(sizeof(int) * npt));
+    HIPCHK(h->cap_off.ensure(sizeof(long) * (npt + 1)));
+    RegionalArgs R = make_reg_args(h);
+    HIPCHK(lekf_reg_capacity_launch(R, h->cap_n.as<int>(), h->cap_off.as<long>(), st));
+    std::vector<long> off(npt + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), h->cap_off.p, sizeof(long) * (npt + 1), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    h->chunk_pts.assign(1, 0);
+    h->chunk_off.assign(1, 0);
+    h->held_chunk = -1;
+    long maxlist = 1;
+    for (long p = 0; p < npt;) {
+        long q = p + 1;
+        while (q < npt && off[q + 1] - off[p] <= h->list_budget) ++q;
+        h->chunk_pts.push_back(q);
+        h->chunk_off.push_back(off[q]);
+        maxlist = std::max(maxlist, off[q] - off[p]);
+        p = q;
+    }
+    HIPCHK(h->loc_idx.ensure(sizeof(int) * maxlist));
+    HIPCHK(h->loc_rho.ensure(sizeof(double) * maxlist));
+    h->loc_cap = maxlist;
+    h->have_obs = true;
+    return upload_jobs(h);
 }
 
 int lekf_set_stream(lekf_handle* h, void* stream)
@@ -1015,6 +1276,17 @@ int lekf_get_nobsl(lekf_handle* h, int* nobsl)
 int lekf_get_local(lekf_handle* h, long b, int cap, int* idx, double* rho)
 {
     if (!h || b < 0 || b >= (long)h->nij * h->nlev) return -LEKF_ERR_ARG;
+    if (h->kind == 2 && h->have_obs) {
+        // the list buffer holds one chunk of points: select the chunk of b again if needed
+        int c = 0;
+        while (h->chunk_pts[c + 1] <= b) ++c;
+        if (c != h->held_chunk) {
+            if (hipStreamSynchronize(h->st) != hipSuccess) return -LEKF_ERR_HIP;
+            const int rc = reg_select_chunk(h, c);
+            if (rc) return -rc;
+        }
+        if (hipStreamSynchronize(h->st) != hipSuccess) return -LEKF_ERR_HIP;
+    }
     long off = 0;
     int cnt = 0;
     if (hipMemcpy(&off, h->loc_off.as<long>() + b, sizeof(long), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -1031,14 +1303,16 @@ int lekf_get_local(lekf_handle* h, long b, int cap, int* idx, double* rho)
 
 int lekf_get_obs_index(lekf_handle* h, int* perm, int* nobsgrd, int* nbinned)
 {
-    if (!h || h->kind != 0) return LEKF_ERR_ARG;
+    if (!h || (h->kind != 0 && h->kind != 2)) return LEKF_ERR_ARG;
     HIPCHK(hipStreamSynchronize(h->st));
     if (nbinned) *nbinned = h->nbinned;
     if (perm && h->nbinned)
         HIPCHK(hipMemcpy(perm, h->perm.p, sizeof(int) * h->nbinned, hipMemcpyDeviceToHost));
-    if (nobsgrd)
+    if (nobsgrd && h->kind == 0)
         HIPCHK(hipMemcpy(nobsgrd, h->nobsgrd.p, sizeof(int) * h->g.nlon * h->g.nlat,
                          hipMemcpyDeviceToHost));
+    if (nobsgrd && h->kind == 2 && h->have_obs)
+        HIPCHK(hipMemcpy(nobsgrd, h->cellcum.p, sizeof(int) * h->nij, hipMemcpyDeviceToHost));
     return LEKF_OK;
 }
 
@@ -1066,11 +1340,7 @@ int lekf_cycle_swap(lekf_handle* h)
     std::swap(h->x3b, h->x3a);
     std::swap(h->x2b, h->x2a);
     std::swap(h->infl_in, h->infl_out);
-    if (h->have_obs) {
-        fill_job(h);
-        HIPCHK(hipMemcpyAsync(h->djob.p, &h->job, sizeof(lekf_job), hipMemcpyHostToDevice, h->st));
-        HIPCHK(hipStreamSynchronize(h->st));
-    }
+    if (h->have_obs) return upload_jobs(h);
     return LEKF_OK;
 }
 

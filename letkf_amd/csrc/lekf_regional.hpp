@@ -1,0 +1,48 @@
+// lekf_regional.hpp -- argument blocks of the regional obs-index and selection kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lekf_types.h"
+
+#define LEKF_REG_NB 4      // buckets (the reference's (element, obs type) pairs)
+#define LEKF_REG_LIST 256  // boundary-bin candidates sorted exactly per bucket
+#define LEKF_REG_SEG 512   // rectangle rows per point
+
+struct RegObsArgs {
+    int nobs, ne, nx, ny;
+    const double *ri, *rj, *z, *err, *dep;  // ri, rj: 1-based grid units; z: metres
+    const int* bkt;
+    const int* qc;                          // nullable (1 = good)
+    const double* hdxf;                     // obs perturbation of member i at hdxf[n*h_so + i*h_sm]
+    long h_so, h_sm;
+    unsigned long long* keys;               // padded to a power of two
+    int* cell_cnt;                          // nx * ny, zeroed before
+    // sorted outputs
+    int* perm;
+    double *o_ri, *o_rj, *o_z, *o_err, *o_rdiag, *o_dep, *o_hdxf;
+    float* o_hdxf32;                        // nullable
+    int* o_bkt;
+};
+
+struct RegionalArgs {
+    int nx, ny, nlev, nbkt, nobs_max;
+    double dx, dy, sigma_h, sigma_v, dlon_zero, dlat_zero;
+    const double* zlev;                     // nlev level heights (m)
+    const int* cellcum;                     // nx * ny inclusive cumulative obs counts, (j, i) order
+    const double *ori, *orj, *oz, *oerr;    // sorted obs
+    const int* obkt;
+    long pt_begin;                          // first global point of this launch
+    const long* list_off;                   // per point list offsets (global numbering)
+    long list_base;                         // list_off of the launch's first point
+    long* loc_off;                          // per point, relative to the launch's list buffer
+    int* loc_cnt;
+    int* loc_idx;
+    double* loc_rho;
+};
+
+hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
+hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
+                                   hipStream_t st);
+hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_off,
+                                    hipStream_t st);
+hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, hipStream_t st);

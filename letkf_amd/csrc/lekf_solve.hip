@@ -142,12 +142,12 @@ __device__ __forceinline__ double rsq_nr(double x)
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int k, int lane,
-                                         lekf_d4* acc, double& p3)
+                                         lekf_d4* acc, double& p3, int wv, int nw)
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
 #pragma unroll 2
-    for (int q0 = 0; q0 < p; q0 += 4) {
+    for (int q0 = 4 * wv; q0 < p; q0 += 4 * nw) {
         const int q = q0 + kq;
         const bool valid = q < p;
         double w = 0.0, dd = 0.0;
@@ -192,12 +192,12 @@ __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int 
 // A*B ~ Ah*Bh + Ah*Bl + Al*Bh on v_mfma_f32_16x16x32_bf16.
 template <int NT>
 __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int k, int lane,
-                                         lekf_f4* acc, double& p3)
+                                         lekf_f4* acc, double& p3, int wv, int nw)
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
 #pragma unroll 2
-    for (int q0 = 0; q0 < p; q0 += 4) {
+    for (int q0 = 4 * wv; q0 < p; q0 += 4 * nw) {
         const int q = q0 + kq;
         const bool valid = q < p;
         float w = 0.0f, dd = 0.0f;
@@ -240,11 +240,11 @@ __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int 
 // values (kk = 8*(l/16) .. +7) of row/col l%16.
 template <int NT>
 __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, int k, int lane,
-                                            lekf_f4* acc, double& p3)
+                                            lekf_f4* acc, double& p3, int wv, int nw)
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
-    for (int q0 = 0; q0 < p; q0 += 32) {
+    for (int q0 = 32 * wv; q0 < p; q0 += 32 * nw) {
         lekf_bf8 ah[NT], al[NT], bh[NT], bl[NT];
         float wv[8];
         long ov[8];
@@ -305,12 +305,14 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
 }
 
 // Write the upper tiles of G into LDS: C (k x k, exactly symmetric), b and
-// sum w d^2.  Only (row <= col) elements are taken so C is symmetric bit for
-// bit.  Result-tile layouts (measured on gfx950): the FP64 16x16x4 MFMA puts
-// element (row = l/16 + 4r, col = l%16) in register r of lane l; the FP32 and
-// BF16 16x16 forms put (row = 4*(l/16) + r, col = l%16) there.
+// sum w d^2; with add, accumulate onto what an earlier wave stored.  Only
+// (row <= col) elements are taken so C is symmetric bit for bit.  Result-tile
+// layouts (measured on gfx950): the FP64 16x16x4 MFMA puts element
+// (row = l/16 + 4r, col = l%16) in register r of lane l; the FP32 and BF16
+// 16x16 forms put (row = 4*(l/16) + r, col = l%16) there.
 template <int NT, bool F64, typename ACC>
-__device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int lane, SolveLds& L)
+__device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int lane, SolveLds& L,
+                                           bool add)
 {
     const int lr = lane & 15, kq = lane >> 4;
     int a = 0;
@@ -325,13 +327,14 @@ __device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int l
                 const double val = (double)acc[a][r];
                 if (row <= col) {
                     if (col < k) {
-                        L.A[row * lda + col] = val;
-                        L.A[col * lda + row] = val;
+                        const double v = add ? L.A[row * lda + col] + val : val;
+                        L.A[row * lda + col] = v;
+                        L.A[col * lda + row] = v;
                     } else if (col == k) {
                         if (row < k)
-                            L.B[row] = val;
+                            L.B[row] = add ? L.B[row] + val : val;
                         else
-                            L.sc[0] = val;  // sum w d^2
+                            L.sc[0] = add ? L.sc[0] + val : val;  // sum w d^2
                     }
                 }
             }
@@ -341,13 +344,15 @@ __device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int l
 }
 
 // Parallel cyclic Jacobi on the n x n symmetric matrix in L.A (dummy index
-// k when k is odd stays zero and never rotates).  V accumulates the
-// rotations.  Returns the number of sweeps, or -1 if the limit was reached.
+// k when k is odd stays zero and never rotates) by a block of BS threads.
+// V accumulates the rotations.  Returns the number of sweeps, or -1 if the
+// limit was reached.
+template <int BS>
 __device__ int jacobi_lds(SolveLds& L, int k, int lane)
 {
     const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
     const int nblk = np * (np + 1) / 2;
-    const int dr = LEKF_WAVE / np, dj = LEKF_WAVE % np;
+    const int dr = BS / np, dj = BS % np;
     int sweep = 0;
     for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
         if (lane == 0) L.flag[0] = 0;
@@ -377,7 +382,7 @@ __device__ int jacobi_lds(SolveLds& L, int k, int lane)
             }
             __syncthreads();
             // A <- R^T A R on 2x2 blocks of rotation pairs (i <= j), mirrored
-            for (int w = lane; w < nblk; w += LEKF_WAVE) {
+            for (int w = lane; w < nblk; w += BS) {
                 const int e = L.blk[w];
                 const int i = e >> 16, j = e & 0xffff;
                 const double si = L.S[i];
@@ -531,15 +536,22 @@ __device__ int jacobi_reg32(SolveLds& L, int k, int lane, int lda)
 
 // NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32), or
 // 0 for the LDS Jacobi (k > 32); MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3.
+// The register-Jacobi classes run one wave per point; the LDS-Jacobi classes
+// run four: the Gram is split over the waves by obs and summed into LDS in
+// wave order, and the Jacobi, W assembly and apply stride over all threads.
 template <int NT, int KR, int MODE>
-__global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restrict__ jobs, int njobs)
+__global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
+    const LekfJob* __restrict__ jobs, int njobs)
 {
+    constexpr int BS = KR > 0 ? 64 : 256;  // threads per point
+    constexpr int NW = BS / LEKF_WAVE;
     extern __shared__ double smem[];
     const long gp = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (LEKF_WAVE - 1), wv = tid / LEKF_WAVE;
     const int ji = lekf_job_of(jobs, njobs, gp);
     const LekfJob& J = jobs[ji];
-    const int b = (int)(gp - J.pt_begin);
+    const int b = (int)(gp - J.pt_begin + J.pt0);
     const int k = J.k;
     const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
     const int nblk = np * (np + 1) / 2;
@@ -554,12 +566,12 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     // LDS Jacobi (k > 32): zero A (dummy row/col), V = I, block table
     constexpr bool reg_jacobi = KR > 0;
     if (!reg_jacobi) {
-        for (int w = lane; w < n * lda; w += LEKF_WAVE) L.A[w] = 0.0;
-        for (int w = lane; w < k * lda; w += LEKF_WAVE) {
+        for (int w = tid; w < n * lda; w += BS) L.A[w] = 0.0;
+        for (int w = tid; w < k * lda; w += BS) {
             const int r = w / lda, c = w - r * lda;
             L.V[w] = (r == c) ? 1.0 : 0.0;
         }
-        for (int w = lane; w < nblk; w += LEKF_WAVE) {
+        for (int w = tid; w < nblk; w += BS) {
             int i = 0, rem = w;
             while (rem >= np - i) {
                 rem -= np - i;
@@ -576,21 +588,30 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
             lekf_d4 acc[NT * (NT + 1) / 2];
 #pragma unroll
             for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
-            gram_f64<NT>(J, off, p, k, lane, acc, p3);
-            store_gram<NT, true>(acc, k, lda, lane, L);
+            gram_f64<NT>(J, off, p, k, lane, acc, p3, wv, NW);
+            for (int w = 0; w < NW; ++w) {
+                if (wv == w) store_gram<NT, true>(acc, k, lda, lane, L, w > 0);
+                if (NW > 1) __syncthreads();
+            }
         } else {
             lekf_f4 acc[NT * (NT + 1) / 2];
 #pragma unroll
             for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
             if (MODE == 1)
-                gram_f32<NT>(J, off, p, k, lane, acc, p3);
+                gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, NW);
             else
-                gram_bf16x3<NT>(J, off, p, k, lane, acc, p3);
-            store_gram<NT, false>(acc, k, lda, lane, L);
+                gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, NW);
+            for (int w = 0; w < NW; ++w) {
+                if (wv == w) store_gram<NT, false>(acc, k, lda, lane, L, w > 0);
+                if (NW > 1) __syncthreads();
+            }
         }
         p3 = wave_sum(p3);
+        if (lane == 0) L.sc[4 + wv] = p3;
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
+            p3 = L.sc[4];
+            for (int w = 1; w < NW; ++w) p3 += L.sc[4 + w];
             double tr = 0.0;
             for (int i = 0; i < k; ++i) tr += L.A[i * lda + i];
             L.sc[1] = tr / (double)(k - 1);  // P2
@@ -606,9 +627,9 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
         if constexpr (reg_jacobi) {
             sweeps = jacobi_reg32<(KR > 0 ? KR : 4)>(L, k, lane, lda);
         } else {
-            sweeps = jacobi_lds(L, k, lane);
+            sweeps = jacobi_lds<BS>(L, k, tid);
             __syncthreads();
-            for (int j = lane; j < k; j += LEKF_WAVE) L.L[j] = L.A[j * lda + j];
+            for (int j = tid; j < k; j += BS) L.L[j] = L.A[j * lda + j];
         }
         if (sweeps < 0) status |= LEKF_PT_NOCONV;
         __syncthreads();
@@ -621,23 +642,23 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
         }
         if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
         // u = L^-1 V^T b ; w-bar = V u
-        for (int j = lane; j < k; j += LEKF_WAVE) {
+        for (int j = tid; j < k; j += BS) {
             double s = 0.0;
             for (int i = 0; i < k; ++i) s += L.V[i * lda + j] * L.B[i];
             L.U[j] = s / L.L[j];
         }
         __syncthreads();
-        for (int i = lane; i < k; i += LEKF_WAVE) {
+        for (int i = tid; i < k; i += BS) {
             double s = 0.0;
             for (int j = 0; j < k; ++j) s += L.V[i * lda + j] * L.U[j];
             L.Wb[i] = s;
         }
         // U <- sqrt((k-1)/lambda) (U no longer needed)
         __syncthreads();
-        for (int j = lane; j < k; j += LEKF_WAVE) L.U[j] = sqrt((double)(k - 1) / L.L[j]);
+        for (int j = tid; j < k; j += BS) L.U[j] = sqrt((double)(k - 1) / L.L[j]);
         __syncthreads();
         // T (into A) = V diag(U) V^T (+ w-bar)
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
+        for (int w = tid; w < k * k; w += BS) {
             const int i = w / k, m = w - i * k;
             double s = 0.0;
             for (int l = 0; l < k; ++l) s += L.V[i * lda + l] * L.U[l] * L.V[m * lda + l];
@@ -646,7 +667,7 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
-            for (int w = lane; w < k * k; w += LEKF_WAVE) {
+            for (int w = tid; w < k * k; w += BS) {
                 const int i = w / k, m = w - i * k;
                 double s = 0.0;
                 for (int l = 0; l < k; ++l) s += L.V[i * lda + l] / L.L[l] * L.V[m * lda + l];
@@ -654,9 +675,9 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
             }
         }
         if (J.transm)
-            for (int i = lane; i < k; i += LEKF_WAVE) J.transm[(size_t)b * k + i] = L.Wb[i];
+            for (int i = tid; i < k; i += BS) J.transm[(size_t)b * k + i] = L.Wb[i];
         // adaptive inflation (common_letkf.f90:218-239)
-        if (lane == 0 && J.infl_out) {
+        if (tid == 0 && J.infl_out) {
             double na = alpha;
             if (J.flags & LEKF_F_FIXED_INFL) {
                 na = J.infl_fixed;
@@ -673,27 +694,27 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     } else {
         // no local obs (common_letkf.f90:82-96)
         const double sa = sqrt(alpha);
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
+        for (int w = tid; w < k * k; w += BS) {
             const int i = w / k, m = w - i * k;
             L.A[i * lda + m] = (i == m) ? sa : 0.0;
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
-            for (int w = lane; w < k * k; w += LEKF_WAVE) {
+            for (int w = tid; w < k * k; w += BS) {
                 const int i = w / k, m = w - i * k;
                 pa[i + (size_t)m * k] = (i == m) ? alpha / (double)(k - 1) : 0.0;
             }
         }
         if (J.transm)
-            for (int i = lane; i < k; i += LEKF_WAVE) J.transm[(size_t)b * k + i] = 0.0;
-        if (lane == 0 && J.infl_out)
+            for (int i = tid; i < k; i += BS) J.transm[(size_t)b * k + i] = 0.0;
+        if (tid == 0 && J.infl_out)
             J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
     }
-    if (lane == 0 && J.status) J.status[b] = status;
+    if (tid == 0 && J.status) J.status[b] = status;
     __syncthreads();
     if (J.trans) {
         double* tr = J.trans + (size_t)b * k * k;
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
+        for (int w = tid; w < k * k; w += BS) {
             const int m = w / k, i = w - m * k;  // column-major: coalesced over i
             tr[i + (size_t)m * k] = L.A[i * lda + m];
         }
@@ -704,7 +725,7 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     const int n3 = J.nv3d, n2 = (lev == 0) ? J.nv2d : 0;
     const int nrow = n3 + n2;
     const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
-    for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
+    for (int w = tid; w < nrow * k; w += BS) {
         const int r = w / k, m = w - r * k;
         const double* src = (r < n3) ? J.x3b + ((size_t)b * n3 + r) * k
                                      : J.x2b + ((size_t)ij * J.nv2d + (r - n3)) * k;
@@ -712,15 +733,15 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     }
     __syncthreads();
     // ensemble mean in member order (speedy/common/common_speedy.f90:335-370)
-    for (int r = lane; r < nrow; r += LEKF_WAVE) {
+    for (int r = tid; r < nrow; r += BS) {
         double s = L.X[r * k];
         for (int m = 1; m < k; ++m) s += L.X[r * k + m];
         L.M[r] = s / (double)k;
     }
     __syncthreads();
-    for (int w = lane; w < nrow * k; w += LEKF_WAVE) L.X[w] -= L.M[w / k];
+    for (int w = tid; w < nrow * k; w += BS) L.X[w] -= L.M[w / k];
     __syncthreads();
-    for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
+    for (int w = tid; w < nrow * k; w += BS) {
         const int r = w / k, m = w - r * k;
         const bool upd = (r >= n3) || ((mask >> r) & 1u);
         double a = L.M[r];
@@ -760,8 +781,8 @@ template <int NT, int KR, int MODE>
 static hipError_t launch_k(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
 {
     const size_t lds = lekf_solve_lds_bytes(kmax);
-    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt), dim3(LEKF_WAVE), lds,
-                       st, djobs, njobs);
+    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt),
+                       dim3(KR > 0 ? LEKF_WAVE : 4 * LEKF_WAVE), lds, st, djobs, njobs);
     return hipGetLastError();
 }
 

@@ -182,7 +182,7 @@ class L96Letkf(_Handle):
 
 class RegionalLetkf(_Handle):
     def __init__(self, *, nx, ny, zlev, dx, dy, ne, nv3d, sigma_h, sigma_v, nobs_max=1000,
-                 prec="f64", infl_mode="adaptive", infl_value=1.0, minfl=0.0):
+                 prec="f64", infl_mode="adaptive", infl_value=1.0, minfl=0.0, list_budget=0):
         self._z = np.ascontiguousarray(zlev, dtype=np.float64)
         c = _lib.RegionalCfg()
         c.nx, c.ny, c.nlev = nx, ny, self._z.size
@@ -194,11 +194,13 @@ class RegionalLetkf(_Handle):
         c.infl_mode = _lib.LEKF_INFL_FIXED if infl_mode == "fixed" else _lib.LEKF_INFL_ADAPTIVE
         c.infl_value = infl_value
         c.minfl = minfl
+        c.list_budget = int(list_budget)
         st = ctypes.c_int(0)
         self.h = _lib.lib().lekf_regional_create(ctypes.byref(c), ctypes.byref(st))
         if not self.h:
             raise _lib.LetkfError(st.value, "lekf_regional_create")
         self.nij, self.nlev, self.ne, self.nv3d, self.nv2d = nx * ny, c.nlev, ne, nv3d, 0
+        self.nx, self.ny = nx, ny
 
     def set_obs(self, ri, rj, z, err, dep, bucket, hdxf, qc=None):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (ri, rj, z, err, dep)]
@@ -208,6 +210,17 @@ class RegionalLetkf(_Handle):
         check(_lib.lib().lekf_set_obs_regional(self.h, a[0].size, *[dptr(x) for x in a], iptr(b),
                                                iptr(q) if q is not None else None,
                                                hx.ctypes.data_as(_lib._dp)), "set_obs_regional")
+
+    def obs_index(self):
+        """(perm, cum): input index of each binned obs and the cumulative obs count
+        through each cell, shape (ny, nx) (cell (i, j) at [j-1, i-1])."""
+        nb = ctypes.c_int(0)
+        cum = np.zeros(self.nij, dtype=np.int32)
+        check(_lib.lib().lekf_get_obs_index(self.h, None, iptr(cum), ctypes.byref(nb)),
+              "get_obs_index")
+        perm = np.zeros(max(nb.value, 1), dtype=np.int32)
+        check(_lib.lib().lekf_get_obs_index(self.h, iptr(perm), None, None), "get_obs_index")
+        return perm[:nb.value], cum.reshape(self.ny, self.nx)
 
 
 def run_many(handles):

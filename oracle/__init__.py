@@ -63,10 +63,12 @@ def _lib_c():
         _c.orc_obs_local_global.argtypes = (
             [c_double] * 5 + [c_int, c_int, c_int, _dp, _dp, _ip] + [_dp] * 6 +
             [c_double] * 4 + [_dp, _ip, _ip, _dp, _dp])
+        _c.orc_obs_bins_regional.restype = c_int
+        _c.orc_obs_bins_regional.argtypes = [c_int, _dp, _dp, _ip, c_int, c_int, _ip, _ip]
         _c.orc_obs_local_regional.restype = c_int
         _c.orc_obs_local_regional.argtypes = (
-            [c_int, _ip, _dp, _dp, _dp, _ip, _dp, c_int] + [c_double] * 6 +
-            [c_int, _ip, _dp, _ip, _ip, _dp, _dp])
+            [c_double] * 3 + [c_int, c_int, _ip] + [_dp] * 4 + [_ip, c_int] + [c_double] * 6 +
+            [c_int, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _dp])
         _c.orc_das_global.restype = c_int
         _c.orc_das_global.argtypes = (
             [c_int] * 5 + [_dp] * 4 + [_dp, _dp, _dp, c_int, _dp, _dp, c_int, c_int, _dp, _dp,
@@ -262,24 +264,48 @@ def obs_local_global(lon1, lat1, dlon_zero, dlat_zero, logpfm, ilev, lon, lat, n
     return idx[:n].copy(), rl[:n].copy(), rd[:n].copy()
 
 
-def obs_local_regional(cand, ox, oy, oz, otyp, oerr, ntyp, px, py, pz, sigma_h, sigma_v, nd_max,
-                       nobs_max):
-    """Regional selection with the per-type cap (scale/letkf/letkf_tools.f90:853-1178)."""
-    cand = np.ascontiguousarray(cand, dtype=np.int32)
-    nc = cand.size
+DIST_ZERO_FAC_SCALE = 3.651483717  # scale/letkf/letkf_obs.f90:29
+
+
+def obs_bins_regional(ri, rj, nx, ny, qc=None):
+    """Regional obs index (scale/letkf/letkf_obs.f90:740-830): (perm, nobsgrd (ny, nx+1))."""
+    ri = _f64(ri)
+    rj = _f64(rj)
+    q = np.ascontiguousarray(qc, dtype=np.int32) if qc is not None else None
+    perm = np.zeros(max(ri.size, 1), dtype=np.int32)
+    grd = np.zeros((ny, nx + 1), dtype=np.int32)  # column-major (0:nx, ny)
+    n = _lib_c().orc_obs_bins_regional(ri.size, _d(ri), _d(rj), _i(q) if q is not None else None,
+                                       nx, ny, _i(perm), _i(grd))
+    return perm[:n].copy(), grd
+
+
+def obs_local_regional(ri, rj, rz, nx, ny, nobsgrd, obs, dx, dy, sigma_h, sigma_v, nobs_max,
+                       nbkt):
+    """Regional obs_local with the per-bucket cap (scale/letkf/letkf_tools.f90:853-1178).
+
+    obs: sorted arrays ri, rj, z, err (float) and bucket (int).  Returns (idx, rloc, rdiag)
+    in the reference's order (scan order uncapped; bucket then slot order capped).
+    """
+    dlon_zero = sigma_h * DIST_ZERO_FAC_SCALE / dx
+    dlat_zero = sigma_h * DIST_ZERO_FAC_SCALE / dy
+    ori, orj, oz, oe = (_f64(obs[k]) for k in ("ri", "rj", "z", "err"))
+    ob = np.ascontiguousarray(obs["bucket"], dtype=np.int32)
+    n = ori.size
     cap = max(nobs_max, 1)
-    bucket = np.zeros(ntyp * cap, dtype=np.int32)
-    bkey = np.zeros(ntyp * cap)
-    bcount = np.zeros(ntyp, dtype=np.int32)
-    idx = np.zeros(max(nc, 1), dtype=np.int32)
-    rl = np.zeros(max(nc, 1))
-    rd = np.zeros(max(nc, 1))
-    ox, oy, oz, oerr = (_f64(a) for a in (ox, oy, oz, oerr))
-    ot = np.ascontiguousarray(otyp, dtype=np.int32)
-    n = _lib_c().orc_obs_local_regional(nc, _i(cand), _d(ox), _d(oy), _d(oz), _i(ot), _d(oerr),
-                                        ntyp, px, py, pz, sigma_h, sigma_v, nd_max, nobs_max,
-                                        _i(bucket), _d(bkey), _i(bcount), _i(idx), _d(rl), _d(rd))
-    return idx[:n].copy(), rl[:n].copy(), rd[:n].copy()
+    isort = np.zeros(nbkt * cap, dtype=np.int32)
+    sobs = np.zeros(nbkt * cap, dtype=np.int32)
+    srd = np.zeros(nbkt * cap)
+    srl = np.zeros(nbkt * cap)
+    cnt = np.zeros(nbkt, dtype=np.int32)
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    rl = np.zeros(max(n, 1))
+    rd = np.zeros(max(n, 1))
+    g = np.ascontiguousarray(nobsgrd, dtype=np.int32)
+    m = _lib_c().orc_obs_local_regional(ri, rj, rz, nx, ny, _i(g), _d(ori), _d(orj), _d(oz), _d(oe),
+                                        _i(ob), nbkt, dx, dy, sigma_h, sigma_v, dlon_zero,
+                                        dlat_zero, nobs_max, _i(isort), _i(sobs), _d(srd), _d(srl),
+                                        _i(cnt), _i(idx), _d(rl), _d(rd))
+    return idx[:m].copy(), rl[:m].copy(), rd[:m].copy()
 
 
 def das_global(prob, *, q_top_lev=5, iq=3, var_local_row=None, return_nobsl=False):

@@ -673,76 +673,150 @@ int orc_obs_local_global(double lon1, double lat1, double dlon_zero, double dlat
 /* ------------------------------------------------------------------------- */
 
 /*
- * Cartesian regional selection restated from scale/letkf/letkf_tools.f90:
- *   normalised distance nd = (dh/sigma_h)^2 + (dv/sigma_v)^2      (:991-1018)
- *   reject nd > dist_zero_fac_square (13.33333333)                 (:1022-1023)
- *   rho = exp(-0.5 nd); rdiag = err^2 / rho (rdiag_wloc form)      (:1027-1037)
- *   cap: per obs-type bucket keep the nobs_max smallest rdiag by
- *        insertion sort, strict < so earlier-scanned obs win ties  (:1041-1156)
- * Candidates arrive in scan order (cand[], ncand); the caller supplies obs
- * coordinates in metres (ox, oy, oz), the point (px, py, pz), per-obs type
- * index otyp (0..ntyp-1) and error oerr.  Output order: by type, then rank.
- * Returns nobsl.  Scratch `bucket` needs ntyp*nobs_max ints, `bkey` doubles.
+ * Regional obs index (scale/letkf/letkf_obs.f90:740-830, one subdomain):
+ * obs n with qc good goes to cell i = ceiling(ri-0.5), j = ceiling(rj-0.5)
+ * (1-based grid units), clamped to [1,nx] x [1,ny]; obs keep their input
+ * order inside a cell.  nobsgrd is (0:nx) x ny column-major, cumulative,
+ * with nobsgrd(0,j) = nobsgrd(nx,j-1).  perm[m] = input index of sorted obs
+ * m.  Returns the number of binned obs.
  */
-int orc_obs_local_regional(int ncand, const int *cand, const double *ox, const double *oy,
-                           const double *oz, const int *otyp, const double *oerr, int ntyp,
-                           double px, double py, double pz, double sigma_h, double sigma_v,
-                           double nd_max, int nobs_max, int *bucket, double *bkey,
-                           int *bcount, int *idx_out, double *rloc_out, double *rdiag_out)
+int orc_obs_bins_regional(int nobs, const double *ri, const double *rj, const int *qc, int nx,
+                          int ny, int *perm, int *nobsgrd)
 {
-    for (int t = 0; t < ntyp; ++t) bcount[t] = 0;
+    const int ld = nx + 1;
+    int *cell = (int *)malloc(sizeof(int) * (nobs > 0 ? nobs : 1));
+    int *cnt = (int *)calloc((size_t)nx * ny, sizeof(int));
+    for (int n = 0; n < nobs; ++n) {
+        cell[n] = -1;
+        if (qc && qc[n] != 1) continue;
+        int i = (int)ceil(ri[n] - 0.5), j = (int)ceil(rj[n] - 0.5);
+        if (i < 1) i = 1;
+        if (i > nx) i = nx;
+        if (j < 1) j = 1;
+        if (j > ny) j = ny;
+        cell[n] = (i - 1) + nx * (j - 1);
+        cnt[cell[n]]++;
+    }
+    memset(nobsgrd, 0, sizeof(int) * (size_t)ld * ny);
+    for (int j = 0; j < ny; ++j) {
+        if (j > 0) nobsgrd[0 + ld * j] = nobsgrd[nx + ld * (j - 1)];
+        for (int i = 1; i <= nx; ++i)
+            nobsgrd[i + ld * j] = nobsgrd[(i - 1) + ld * j] + cnt[(i - 1) + nx * j];
+    }
+    int *next = (int *)malloc(sizeof(int) * (size_t)nx * ny);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 1; i <= nx; ++i) next[(i - 1) + nx * j] = nobsgrd[(i - 1) + ld * j];
+    int tot = 0;
+    for (int n = 0; n < nobs; ++n) {
+        if (cell[n] < 0) continue;
+        perm[next[cell[n]]++] = n;
+        ++tot;
+    }
+    free(cell);
+    free(cnt);
+    free(next);
+    return tot;
+}
+
+/*
+ * Regional obs_local (scale/letkf/letkf_tools.f90:853-1178, one subdomain, radar-type
+ * vertical distance):
+ *   rectangle imin = max(1, floor(ri - dlon_zero)), imax = min(nx, ceiling(ri + dlon_zero)),
+ *   likewise j (:906-909); candidates scanned row by row through nobsgrd (obs_choose,
+ *   scale/letkf/letkf_obs.f90:1054-1119);
+ *   rdx = (ri - ri_o) DX, rdy = (rj - rj_o) DY, nd_h = sqrt(rdx^2 + rdy^2) / sigma_h,
+ *   nd_v = |z_o - rz| / sigma_v (:972-1004), ndist = nd_h^2 + nd_v^2 rejected when above
+ *   the literal 13.33333333 (scale/letkf/letkf_obs.f90:29-30; :1022-1023);
+ *   rloc = exp(-0.5 ndist), rdiag = err^2 / rloc (:1027-1037);
+ *   nobs_max > 0: per bucket (the reference's (element, obs type) pair) keep the
+ *   nobs_max smallest rdiag with the reference's slot/sort structure (:1041-1128) and
+ *   emit bucket by bucket in slot order (:1134-1156); nobs_max <= 0: scan order.
+ * Scratch: isort, slot_obs (nbkt*nobs_max ints), slot_rd, slot_rl (doubles), cnt (nbkt).
+ * Returns nobsl.
+ */
+int orc_obs_local_regional(double ri, double rj, double rz, int nx, int ny, const int *nobsgrd,
+                           const double *ori, const double *orj, const double *oz,
+                           const double *oerr, const int *obkt, int nbkt, double dx, double dy,
+                           double sigma_h, double sigma_v, double dlon_zero, double dlat_zero,
+                           int nobs_max, int *isort, int *slot_obs, double *slot_rd,
+                           double *slot_rl, int *cnt, int *idx_out, double *rloc_out,
+                           double *rdiag_out)
+{
+    const double nd_max = 13.33333333;
+    const int ld = nx + 1;
+    int imin = (int)floor(ri - dlon_zero), imax = (int)ceil(ri + dlon_zero);
+    int jmin = (int)floor(rj - dlat_zero), jmax = (int)ceil(rj + dlat_zero);
+    if (imin < 1) imin = 1;
+    if (imax > nx) imax = nx;
+    if (jmin < 1) jmin = 1;
+    if (jmax > ny) jmax = ny;
     int nsel = 0;
-    for (int q = 0; q < ncand; ++q) {
-        const int o = cand[q];
-        const double dx = ox[o] - px, dy = oy[o] - py, dz = oz[o] - pz;
-        const double dh2 = dx * dx + dy * dy;
-        const double nd = dh2 / (sigma_h * sigma_h) + (dz * dz) / (sigma_v * sigma_v);
-        if (nd > nd_max) continue;
-        const double rho = exp(-0.5 * nd);
-        const double rd = oerr[o] * oerr[o] / rho;
-        if (nobs_max <= 0) {
-            idx_out[nsel] = o;
-            rloc_out[nsel] = rho;
-            rdiag_out[nsel] = rd;
-            ++nsel;
-            continue;
-        }
-        const int t = otyp[o];
-        int *bk = bucket + (size_t)t * nobs_max;
-        double *kk = bkey + (size_t)t * nobs_max;
-        int c = bcount[t];
-        if (c < nobs_max) {
-            int pos = c;
-            while (pos > 0 && rd < kk[pos - 1]) {
-                kk[pos] = kk[pos - 1];
-                bk[pos] = bk[pos - 1];
-                --pos;
+    const int cap = nobs_max;
+    if (cap > 0) {
+        for (int b = 0; b < nbkt; ++b) cnt[b] = 0;
+        for (size_t t = 0; t < (size_t)nbkt * cap; ++t) isort[t] = -1;
+    }
+    if (imin > imax || jmin > jmax) return 0;
+    for (int j = jmin; j <= jmax; ++j) {
+        const int beg = nobsgrd[(imin - 1) + ld * (j - 1)];
+        const int end = nobsgrd[imax + ld * (j - 1)];
+        for (int o = beg; o < end; ++o) {
+            const double rdx = (ri - ori[o]) * dx;
+            const double rdy = (rj - orj[o]) * dy;
+            const double nd_h = sqrt(rdx * rdx + rdy * rdy) / sigma_h;
+            const double nd_v = fabs(oz[o] - rz) / sigma_v;
+            const double ndist = nd_h * nd_h + nd_v * nd_v;
+            if (ndist > nd_max) continue;
+            const double nrloc = exp(-0.5 * ndist);
+            const double nrdiag = oerr[o] * oerr[o] / nrloc;
+            if (cap <= 0) {
+                idx_out[nsel] = o;
+                rloc_out[nsel] = nrloc;
+                rdiag_out[nsel] = nrdiag;
+                ++nsel;
+                continue;
             }
-            kk[pos] = rd;
-            bk[pos] = o;
-            bcount[t] = c + 1;
-        } else if (rd < kk[nobs_max - 1]) {
-            int pos = nobs_max - 1;
-            while (pos > 0 && rd < kk[pos - 1]) {
-                kk[pos] = kk[pos - 1];
-                bk[pos] = bk[pos - 1];
-                --pos;
+            const int b = obkt[o];
+            int *is = isort + (size_t)b * cap;
+            int *so = slot_obs + (size_t)b * cap;
+            double *sr = slot_rd + (size_t)b * cap;
+            double *sl = slot_rl + (size_t)b * cap;
+            /* case 0: full and not better than the last-priority obs */
+            if (cnt[b] >= cap && nrdiag >= sr[is[cap - 1]]) continue;
+            for (int s2 = 0; s2 < cap; ++s2) {
+                if (is[s2] < 0) { /* case 1: spare slot at the end */
+                    const int slot = cnt[b]++;
+                    is[s2] = slot;
+                    so[slot] = o;
+                    sr[slot] = nrdiag;
+                    sl[slot] = nrloc;
+                    break;
+                } else if (nrdiag < sr[is[s2]]) {
+                    int slot;
+                    if (cnt[b] < cap) { /* case 2: insert, spare slot */
+                        slot = cnt[b]++;
+                        for (int ss = cnt[b] - 1; ss > s2; --ss) is[ss] = is[ss - 1];
+                    } else { /* case 3: overwrite the last-priority slot */
+                        slot = is[cap - 1];
+                        for (int ss = cap - 1; ss > s2; --ss) is[ss] = is[ss - 1];
+                    }
+                    is[s2] = slot;
+                    so[slot] = o;
+                    sr[slot] = nrdiag;
+                    sl[slot] = nrloc;
+                    break;
+                }
             }
-            kk[pos] = rd;
-            bk[pos] = o;
         }
     }
-    if (nobs_max <= 0) return nsel;
-    for (int t = 0; t < ntyp; ++t) {
-        const int *bk = bucket + (size_t)t * nobs_max;
-        const double *kk = bkey + (size_t)t * nobs_max;
-        for (int r = 0; r < bcount[t]; ++r) {
-            idx_out[nsel] = bk[r];
-            rdiag_out[nsel] = kk[r];
-            rloc_out[nsel] = oerr[bk[r]] * oerr[bk[r]] / kk[r];
+    if (cap <= 0) return nsel;
+    for (int b = 0; b < nbkt; ++b)
+        for (int s2 = 0; s2 < cnt[b]; ++s2) {
+            idx_out[nsel] = slot_obs[(size_t)b * cap + s2];
+            rdiag_out[nsel] = slot_rd[(size_t)b * cap + s2];
+            rloc_out[nsel] = slot_rl[(size_t)b * cap + s2];
             ++nsel;
         }
-    }
     return nsel;
 }
 
