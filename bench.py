@@ -41,6 +41,20 @@ def flops_per_point(p, k, nv):
     return np.where(p > 0, full, empty)
 
 
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel/workload from its rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a plain copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def percentile(a, q):
     return float(np.percentile(np.asarray(a), q))
 
@@ -139,6 +153,19 @@ def cpu_baseline(p, ref, max_threads=16):
                        f"(oracle restatement) not timed")}
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
+    give the scale factor between the counters and HBM bytes."""
+    from letkf_amd.device import DeviceArray, sync
+    a = DeviceArray((nbytes,), np.uint8)
+    b = DeviceArray((nbytes,), np.uint8)
+    a.memset(1)
+    for _ in range(reps):
+        b.copy_from_device(a)
+    sync()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--profile", action="store_true",
                     help="only the plain FP64 timed loop (for rocprofv3)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only a 1 GiB device-to-device copy, 5 times (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
@@ -160,6 +189,9 @@ def main():
     if lib().lekf_device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calib:
+        copy_calibration()
+        return
     t0 = time.perf_counter()
     p = synth.speedy()
     gen_s = time.perf_counter() - t0
@@ -190,7 +222,8 @@ def main():
                       "solve_apply": float(np.mean(r["solve_ms"])), "obs_setup_once": obs_ms},
         "roofline": {"bound": "mfma", "kernel": "lekf_solve_kernel (FP64)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("lekf_solve_speedy_f64.md"),
                      "flops_per_launch": flops},
     }
     if args.profile:

@@ -202,8 +202,26 @@ int lekf_dev_free(void* p)
     return LEKF_OK;
 }
 
+// streaming device copy in 16-byte words (one read and one write of each byte)
+__global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 int lekf_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream)
 {
+    if (kind == 2 && bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
+        const size_t n = bytes / 16;
+        if (n) {
+            const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+            hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                               (const uint4*)src, (uint4*)dst, n);
+            HIPCHK(hipGetLastError());
+        }
+        return LEKF_OK;
+    }
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
                           : kind == 1 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;

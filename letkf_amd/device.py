@@ -49,6 +49,15 @@ class DeviceArray:
     def data_ptr(self):
         return self.ptr
 
+    def memset(self, value, stream=None):
+        _lib.check(_lib.lib().lekf_memset(self.ptr, int(value), max(self.nbytes, 8), stream), "memset")
+
+    def copy_from_device(self, other, stream=None):
+        """Asynchronous device-to-device copy of other's bytes (a streaming copy kernel
+        when sizes and addresses are 16-byte aligned)."""
+        n = min(self.nbytes, other.nbytes)
+        _lib.check(_lib.lib().lekf_memcpy(self.ptr, other.ptr, n, 2, stream), "memcpy")
+
     def free(self):
         if self.ptr:
             _lib.lib().lekf_dev_free(self.ptr)
