@@ -127,9 +127,9 @@ int mode_of_prec(int prec) { return prec == LEKF_PREC_F32 ? 1 : (prec == LEKF_PR
 struct CoreCtx {
     bool init = false;
     hipStream_t st = nullptr;
-    DevBuf h, rd, rl, dp, idx, off, infl, tr, tm, pa, stat, job;
+    DevBuf h, rd, rl, dp, idx, off, infl, tr, tm, pa, stat, job, gws;
     int iota_n = 0;
-    std::map<hipStream_t, DevBuf*> jobs_by_stream;
+    std::map<hipStream_t, DevBuf*> jobs_by_stream, ws_by_stream;
 };
 thread_local CoreCtx t_core;
 
@@ -143,10 +143,24 @@ int core_init()
     return 0;
 }
 
+// One solve launch; ws is grown to the Gram workspace the class needs.
+int run_solve(const lekf_job* djobs, int njobs, long npt, int kmax, int nvmax, int mode,
+              DevBuf& ws, hipStream_t st)
+{
+    const size_t need = lekf_solve_ws_bytes(npt, kmax);
+    if (need) HIPCHK(ws.ensure(need));
+    HIPCHK(lekf_solve_launch(djobs, njobs, npt, kmax, nvmax, mode, need ? ws.p : nullptr,
+                             need ? ws.n : 0, st));
+    return 0;
+}
+
+// state rows a job's points apply their transform to (0 without the apply)
+int job_rows(const lekf_job& j) { return (j.flags & LEKF_F_APPLY) ? j.nv3d + j.nv2d : 0; }
+
 // Launch the solves of jobs (all of one arithmetic mode) grouped by kernel
 // class: descriptors are uploaded once, sorted by class, and each class is
 // one launch whose points are numbered from 0.
-int launch_grouped(std::vector<lekf_job> J, int mode, DevBuf& db, hipStream_t st)
+int launch_grouped(std::vector<lekf_job> J, int mode, DevBuf& db, DevBuf& ws, hipStream_t st)
 {
     std::stable_sort(J.begin(), J.end(), [](const lekf_job& a, const lekf_job& b) {
         return lekf_solve_class(a.k) < lekf_solve_class(b.k);
@@ -157,17 +171,20 @@ int launch_grouped(std::vector<lekf_job> J, int mode, DevBuf& db, hipStream_t st
         if (i == 0 || lekf_solve_class(J[i].k) != lekf_solve_class(J[i - 1].k)) starts.push_back(i);
     starts.push_back(J.size());
     std::vector<long> tot(starts.size(), 0);
-    std::vector<int> kmax(starts.size(), 0);
+    std::vector<int> kmax(starts.size(), 0), nvmax(starts.size(), 0);
     for (size_t g = 0; g + 1 < starts.size(); ++g)
         for (size_t i = starts[g]; i < starts[g + 1]; ++i) {
             J[i].pt_begin = tot[g];
             tot[g] += J[i].npt;
             kmax[g] = std::max(kmax[g], J[i].k);
+            nvmax[g] = std::max(nvmax[g], job_rows(J[i]));
         }
     HIPCHK(hipMemcpyAsync(db.p, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice, st));
-    for (size_t g = 0; g + 1 < starts.size(); ++g)
-        HIPCHK(lekf_solve_launch(db.as<lekf_job>() + starts[g], (int)(starts[g + 1] - starts[g]),
-                                 tot[g], kmax[g], mode, st));
+    for (size_t g = 0; g + 1 < starts.size(); ++g) {
+        const int rc = run_solve(db.as<lekf_job>() + starts[g], (int)(starts[g + 1] - starts[g]),
+                                 tot[g], kmax[g], nvmax[g], mode, ws, st);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -348,7 +365,8 @@ int lekf_core_f64(int ne, int nobs, int nobsl, const double* hdxb, const double*
     J.pao = pao ? c.pa.as<double>() : nullptr;
     J.status = c.stat.as<int>();
     HIPCHK(hipMemcpyAsync(c.job.p, &J, sizeof(J), hipMemcpyHostToDevice, st));
-    HIPCHK(lekf_solve_launch(c.job.as<lekf_job>(), 1, 1, ne, 0, st));
+    rc = run_solve(c.job.as<lekf_job>(), 1, 1, ne, 0, 0, c.gws, st);
+    if (rc) return rc;
     int pst = 0;
     HIPCHK(hipMemcpyAsync(trans, c.tr.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
     if (transm) HIPCHK(hipMemcpyAsync(transm, c.tm.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
@@ -384,7 +402,9 @@ int lekf_core_batch(const lekf_job* jobs, int njobs, int prec, void* stream)
     if (tot == 0) return LEKF_OK;
     DevBuf*& db = t_core.jobs_by_stream[st];
     if (!db) db = new DevBuf();
-    return launch_grouped(J, mode, *db, st);
+    DevBuf*& wb = t_core.ws_by_stream[st];
+    if (!wb) wb = new DevBuf();
+    return launch_grouped(J, mode, *db, *wb, st);
 }
 
 int lekf_obs_local_l96(int nx, int ny, const int* gidx, double xlocal, long* loc_off, int* loc_cnt,
@@ -446,6 +466,7 @@ struct lekf_handle {
     int nbkt = 1;
     double dlon_zero = 0;  // regional rectangle half-width in grid units (dlat_zero above)
     DevBuf d_zlev, cellcum, cap_n, cap_off, o_ri, o_rj, o_z, o_bkt, r_dep, r_bkt, r_z;
+    DevBuf gws;  // Gram workspace of the k > 32 solve
     std::vector<long> chunk_pts;  // chunk boundaries (points)
     std::vector<long> chunk_off;  // list offsets at the boundaries
     long list_budget = 512L * 1024 * 1024;
@@ -467,7 +488,7 @@ long handle_bytes(const lekf_handle* h)
                           &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx,
                           &h->loc_rho, &h->status, &h->djob, &h->d_zlev, &h->cellcum,
                           &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
-                          &h->r_dep, &h->r_bkt, &h->r_z};
+                          &h->r_dep, &h->r_bkt, &h->r_z, &h->gws};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -484,7 +505,8 @@ void free_all(lekf_handle* h)
                     &h->o_sin, &h->o_cos, &h->o_lonr, &h->o_lnlev, &h->o_lndat, &h->nobsgrd,
                     &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
                     &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
-                    &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z};
+                    &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
+                    &h->gws};
     for (auto* b : bs) b->release();
 }
 
@@ -630,6 +652,10 @@ int reg_select_chunk(lekf_handle* h, int c)
 int run_select(lekf_handle* h)
 {
     hipStream_t st = h->st;
+    if (h->kind == 2) {
+        if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;  // chunked: lekf_das_run only
+        return reg_select_chunk(h, 0);
+    }
     if (h->kind == 0) {
         HIPCHK(lekf_logpfm_launch(h->x2b.as<double>(), h->nij, h->nv2d, h->g.ips, h->ne,
                                   h->d_sig.as<double>(), h->nlev, h->d_logpfm.as<double>(), st));
@@ -998,25 +1024,19 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
 int lekf_das_select(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
-    if (h->kind == 2) {
-        if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;  // chunked: use lekf_das_run
-        return reg_select_chunk(h, 0);
-    }
     return run_select(h);
 }
 
 int lekf_das_solve(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
+    long npt = h->job.npt;
     if (h->kind == 2) {
         if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;
-        HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->chunk_pts[1], h->ne,
-                                 mode_of_prec(h->prec), h->st));
-        return LEKF_OK;
+        npt = h->chunk_pts[1];
     }
-    HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>(), 1, h->job.npt, h->ne, mode_of_prec(h->prec),
-                             h->st));
-    return LEKF_OK;
+    return run_solve(h->djob.as<lekf_job>(), 1, npt, h->ne, job_rows(h->job), mode_of_prec(h->prec),
+                     h->gws, h->st);
 }
 
 int lekf_das_run(lekf_handle* h)
@@ -1024,11 +1044,11 @@ int lekf_das_run(lekf_handle* h)
     if (h && h->kind == 2 && h->have_ens && h->have_obs) {
         // chunks of points whose local lists fit the list buffer: select, then solve
         for (size_t c = 0; c + 1 < h->chunk_pts.size(); ++c) {
-            const int rc = reg_select_chunk(h, (int)c);
+            int rc = reg_select_chunk(h, (int)c);
             if (rc) return rc;
-            HIPCHK(lekf_solve_launch(h->djob.as<lekf_job>() + c, 1,
-                                     h->chunk_pts[c + 1] - h->chunk_pts[c], h->ne,
-                                     mode_of_prec(h->prec), h->st));
+            rc = run_solve(h->djob.as<lekf_job>() + c, 1, h->chunk_pts[c + 1] - h->chunk_pts[c],
+                           h->ne, job_rows(h->job), mode_of_prec(h->prec), h->gws, h->st);
+            if (rc) return rc;
         }
         return LEKF_OK;
     }
@@ -1210,12 +1230,24 @@ int lekf_set_stream(lekf_handle* h, void* stream)
 int lekf_das_run_many(lekf_handle** hs, int n)
 {
     if (!hs || n <= 0) return LEKF_ERR_ARG;
+    struct Bufs {
+        DevBuf jobs[3], ws;
+    };
     static std::mutex mu;
-    static DevBuf jobs_dev[3];
+    static std::map<hipStream_t, Bufs> bufs;  // per leading stream: calls on other streams overlap
     std::lock_guard<std::mutex> lk(mu);
     hipStream_t st = hs[0]->st;
+    Bufs& B = bufs[st];
+    std::vector<lekf_handle*> batched;
     for (int i = 0; i < n; ++i) {
         if (!hs[i] || !hs[i]->have_ens || !hs[i]->have_obs) return LEKF_ERR_ARG;
+        if (hs[i]->kind == 2 && hs[i]->chunk_pts.size() != 2) {
+            // lists built chunk by chunk: runs on its own (select and solve per chunk)
+            const int rc = lekf_das_run(hs[i]);
+            if (rc) return rc;
+            continue;
+        }
+        batched.push_back(hs[i]);
         if (hs[i]->st != st) {
             // selection runs on the leading stream so the shared solve sees it
             hipStream_t keep = hs[i]->st;
@@ -1231,13 +1263,14 @@ int lekf_das_run_many(lekf_handle** hs, int n)
     // one grouped launch set per arithmetic mode
     for (int mode = 0; mode < 3; ++mode) {
         std::vector<lekf_job> J;
-        for (int i = 0; i < n; ++i)
-            if (mode_of_prec(hs[i]->prec) == mode) J.push_back(hs[i]->job);
+        for (lekf_handle* h : batched)
+            if (mode_of_prec(h->prec) == mode) J.push_back(h->job);
         if (J.empty()) continue;
-        const int rc = launch_grouped(J, mode, jobs_dev[mode], st);
+        const int rc = launch_grouped(J, mode, B.jobs[mode], B.ws, st);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(hs[i]->st));
     return LEKF_OK;
 }
 

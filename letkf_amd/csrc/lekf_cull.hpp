@@ -75,8 +75,9 @@ hipError_t lekf_logpfm_launch(const double* x2b, int nij, int nv2d, int ips, int
                               const double* sig, int nlev, double* logpfm, hipStream_t st);
 
 // solve (lekf_solve.hip)
-size_t lekf_solve_lds_bytes(int k);
+size_t lekf_solve_lds_bytes(int k, int nvmax);
+size_t lekf_solve_ws_bytes(long npt, int kmax);
 int lekf_solve_nt(int k);
 int lekf_solve_class(int k);
-hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int mode,
-                             hipStream_t st);
+hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
+                             int mode, void* ws, size_t ws_bytes, hipStream_t st);

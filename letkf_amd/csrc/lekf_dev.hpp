@@ -36,9 +36,17 @@ typedef __bf16 lekf_bf8 __attribute__((ext_vector_type(8)));
 #define LEKF_JACOBI_TOL 1.0e-15
 #define LEKF_JACOBI_MAX_SWEEPS 40
 
+// job of flat point gp: the last job whose pt_begin <= gp (binary search;
+// pt_begin increases with the job index)
 __device__ __forceinline__ int lekf_job_of(const LekfJob* jobs, int njobs, long gp)
 {
-    int j = 0;
-    while (j + 1 < njobs && jobs[j + 1].pt_begin <= gp) ++j;
-    return j;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].pt_begin <= gp)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }

@@ -1,119 +1,48 @@
 // lekf_solve.hip -- batched LETKF local solve on gfx950.
 //
-// One 64-lane wavefront per local analysis ("point"):
+// A local analysis ("point") is one letkf_core solve plus its apply:
 //   1. gather the point's local obs rows of Y from the obs table and form the
 //      Gram matrix G = [w.Y  w.d]^T [Y d] on MFMA (v_mfma_f64_16x16x4f64, or
 //      FP32 / split-BF16x3 MFMA in the mixed form).  G holds C = Y^T R^-1 Y,
 //      b = Y^T R^-1 d and sum w d^2 in one product
 //      (common/common_letkf.f90:101-118, 177-182, 219-227);
 //   2. C += (k-1)/alpha I (common_letkf.f90:130-138);
-//   3. symmetric eigendecomposition by parallel cyclic Jacobi in LDS
-//      (stands in for mtx_eigen / EISPACK rs, common/common_mtx.f90:41-99);
+//   3. symmetric eigendecomposition in FP64 (stands in for mtx_eigen / EISPACK
+//      rs, common/common_mtx.f90:41-99);
 //   4. W assembly: w-bar = V L^-1 V^T b, Wa = V sqrt((k-1)/L) V^T,
 //      T = Wa + w-bar 1^T, Pa = V L^-1 V^T (common_letkf.f90:146-214);
 //   5. adaptive inflation update (common_letkf.f90:218-239);
 //   6. optionally the fused apply Xa = xmean + Xb' T over the point's state
 //      rows (speedy/letkf/letkf_tools.f90:94-112,164-177,192-197).
-// A point's arithmetic order depends only on its own data, so its result is
-// the same whatever else shares the launch.
+//
+// Two kernel families, chosen by the ensemble size k:
+//   * k <= 32: one fused kernel, one wave per point; the eigendecomposition is
+//     a one-sided Jacobi with the matrix in registers.
+//   * 32 < k <= LEKF_KMAX: a Gram kernel (four waves per point, MFMA) writes C,
+//     b and the sums to a workspace; a second kernel (one wave per point)
+//     reduces C to tridiagonal form by Householder reflections, diagonalises
+//     it by implicit QL with shifts (the method of EISPACK tred2/tql2, the
+//     reference's own eigensolver), and assembles W and the apply.
+// A point's arithmetic depends only on its own data, so its result is the
+// same whatever else shares the launch.
 #include "lekf_dev.hpp"
 
 namespace {
-
-// Circle-method pairing of n (even) indices: round r in [0, n-1), pair i in
-// [0, n/2).  Over the n-1 rounds every index pair meets exactly once.
-__device__ __forceinline__ void rr_pair(int n, int r, int i, int& p, int& q)
-{
-    const int m = n - 1;
-    int a, b;
-    if (i == 0) {
-        a = 0;
-        b = 1 + (r % m);
-    } else {
-        a = 1 + (i + r) % m;
-        b = 1 + (m - i + r) % m;
-    }
-    p = a < b ? a : b;
-    q = a < b ? b : a;
-}
-
-struct SolveLds {
-    double* A;    // n x lda   (matrix being diagonalised, later T)
-    double* V;    // k x lda   (eigenvectors, columns)
-    double* C;    // np rotation cosines
-    double* S;    // np sines
-    double* T;    // np tangents
-    int* P;       // np
-    int* Q;       // np
-    int* blk;     // nblk packed (i << 16 | j)
-    double* B;    // k   (Y^T R^-1 d)
-    double* L;    // n   eigenvalues
-    double* U;    // k
-    double* Wb;   // k   w-bar
-    double* X;    // LEKF_NVMAX * k   state perturbation rows (aliases V)
-    double* M;    // LEKF_NVMAX       row means (aliases V)
-    double* sc;   // 8 scalars
-    int* flag;    // 2 ints
-};
-
-__host__ __device__ inline size_t solve_lds_doubles(int k)
-{
-    const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
-    const int nblk = np * (np + 1) / 2;
-    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;  // apply rows reuse the V region
-    size_t d = 0;
-    d += (size_t)n * lda;                                       // A
-    d += (size_t)k * lda > vx ? (size_t)k * lda : vx;           // V, later X and M
-    d += 3 * (size_t)np;                                        // C S T
-    d += (size_t)np;                                            // P,Q (2 ints = 1 double each pair)
-    d += ((size_t)nblk + 1) / 2;                                // blk ints
-    d += (size_t)k + n + 2 * (size_t)k;                         // B L U Wb
-    d += 8 + 1;                                                 // scalars, flag
-    return d;
-}
-
-__device__ inline SolveLds carve(double* s, int k)
-{
-    const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
-    const int nblk = np * (np + 1) / 2;
-    SolveLds L;
-    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;
-    L.A = s;
-    s += (size_t)n * lda;
-    L.V = s;
-    L.X = s;                      // used only after T has been formed
-    L.M = s + (size_t)LEKF_NVMAX * k;
-    s += (size_t)k * lda > vx ? (size_t)k * lda : vx;
-    L.C = s;
-    s += np;
-    L.S = s;
-    s += np;
-    L.T = s;
-    s += np;
-    L.P = (int*)s;
-    L.Q = L.P + np;
-    s += np;  // np doubles hold the 2*np ints of P and Q
-    L.blk = (int*)s;
-    s += (nblk + 1) / 2;
-    L.B = s;
-    s += k;
-    L.L = s;
-    s += n;
-    L.U = s;
-    s += k;
-    L.Wb = s;
-    s += k;
-    L.sc = s;
-    s += 8;
-    L.flag = (int*)s;
-    return L;
-}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// value of x in lane src (src uniform across the wave)
+__device__ __forceinline__ double lane_bcast(double x, int src)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Reciprocal and reciprocal square root: hardware estimate refined by Newton
@@ -138,7 +67,8 @@ __device__ __forceinline__ double rsq_nr(double x)
 
 // ---------------------------------------------------------------------------
 // Gram products.  Lane l holds A[i = l%16][kk = l/16] and B[kk][j = l%16] of
-// each 16x16x4 step (16x16x32 BF16: kk = 8*(l/16) .. +7).
+// each 16x16x4 step (16x16x32 BF16: kk = 8*(l/16) .. +7).  Wave wv of nw
+// takes every nw-th group of obs.
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int k, int lane,
@@ -246,7 +176,7 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
     for (int q0 = 32 * wv; q0 < p; q0 += 32 * nw) {
         lekf_bf8 ah[NT], al[NT], bh[NT], bl[NT];
-        float wv[8];
+        float wv8[8];
         long ov[8];
         bool vv[8];
         float dv[8];
@@ -254,14 +184,14 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
         for (int e = 0; e < 8; ++e) {
             const int q = q0 + 8 * kq + e;
             vv[e] = q < p;
-            wv[e] = 0.0f;
+            wv8[e] = 0.0f;
             dv[e] = 0.0f;
             ov[e] = 0;
             if (vv[e]) {
                 const long o = J.loc_idx[off + q];
                 const double rho = J.loc_rho[off + q];
                 const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
-                wv[e] = (float)((wloc ? 1.0 : rho) / rd);
+                wv8[e] = (float)((wloc ? 1.0 : rho) / rd);
                 dv[e] = (float)J.dep[o];
                 ov[e] = o;
                 if (lr == 0) p3 += rho;
@@ -279,7 +209,7 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
                     else if (c == k)
                         x = dv[e];
                 }
-                const float xa = wv[e] * x;
+                const float xa = wv8[e] * x;
                 const __bf16 bh1 = (__bf16)x;
                 const __bf16 bl1 = (__bf16)(x - (float)bh1);
                 const __bf16 ah1 = (__bf16)xa;
@@ -304,15 +234,16 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
     }
 }
 
-// Write the upper tiles of G into LDS: C (k x k, exactly symmetric), b and
-// sum w d^2; with add, accumulate onto what an earlier wave stored.  Only
-// (row <= col) elements are taken so C is symmetric bit for bit.  Result-tile
-// layouts (measured on gfx950): the FP64 16x16x4 MFMA puts element
-// (row = l/16 + 4r, col = l%16) in register r of lane l; the FP32 and BF16
-// 16x16 forms put (row = 4*(l/16) + r, col = l%16) there.
+// Write the upper tiles of G into LDS: C (k x k at A, leading dimension lda,
+// exactly symmetric), b and sum w d^2; with add, accumulate onto what an
+// earlier wave stored.  Only (row <= col) elements are taken so C is
+// symmetric bit for bit.  Result-tile layouts (measured on gfx950): the FP64
+// 16x16x4 MFMA puts element (row = l/16 + 4r, col = l%16) in register r of
+// lane l; the FP32 and BF16 16x16 forms put (row = 4*(l/16) + r, col = l%16)
+// there.
 template <int NT, bool F64, typename ACC>
-__device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int lane, SolveLds& L,
-                                           bool add)
+__device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int lane, double* A,
+                                           double* B, double* sumwd2, bool add)
 {
     const int lr = lane & 15, kq = lane >> 4;
     int a = 0;
@@ -327,14 +258,14 @@ __device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int l
                 const double val = (double)acc[a][r];
                 if (row <= col) {
                     if (col < k) {
-                        const double v = add ? L.A[row * lda + col] + val : val;
-                        L.A[row * lda + col] = v;
-                        L.A[col * lda + row] = v;
+                        const double v = add ? A[row * lda + col] + val : val;
+                        A[row * lda + col] = v;
+                        A[col * lda + row] = v;
                     } else if (col == k) {
                         if (row < k)
-                            L.B[row] = add ? L.B[row] + val : val;
+                            B[row] = add ? B[row] + val : val;
                         else
-                            L.sc[0] = add ? L.sc[0] + val : val;  // sum w d^2
+                            *sumwd2 = add ? *sumwd2 + val : val;
                     }
                 }
             }
@@ -343,108 +274,70 @@ __device__ __forceinline__ void store_gram(const ACC* acc, int k, int lda, int l
     }
 }
 
-// Parallel cyclic Jacobi on the n x n symmetric matrix in L.A (dummy index
-// k when k is odd stays zero and never rotates) by a block of BS threads.
-// V accumulates the rotations.  Returns the number of sweeps, or -1 if the
-// limit was reached.
-template <int BS>
-__device__ int jacobi_lds(SolveLds& L, int k, int lane)
+// Adaptive inflation update (common_letkf.f90:218-239)
+__device__ __forceinline__ double adaptive_alpha(double alpha, double p1, double p2, double p3)
 {
-    const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
-    const int nblk = np * (np + 1) / 2;
-    const int dr = BS / np, dj = BS % np;
-    int sweep = 0;
-    for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
-        if (lane == 0) L.flag[0] = 0;
-        __syncthreads();
-        for (int r = 0; r < n - 1; ++r) {
-            if (lane < np) {
-                int pp, qq;
-                rr_pair(n, r, lane, pp, qq);
-                double c = 1.0, s = 0.0, t = 0.0;
-                const double app = L.A[pp * lda + pp];
-                const double aqq = L.A[qq * lda + qq];
-                const double apq = L.A[pp * lda + qq];
-                if (apq * apq > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * fabs(app * aqq)) {
-                    const double th = (aqq - app) * rcp_nr(2.0 * apq);
-                    const double q = fma(th, th, 1.0);
-                    const double at = fabs(th) + q * rsq_nr(q);
-                    t = th >= 0.0 ? rcp_nr(at) : -rcp_nr(at);
-                    c = rsq_nr(fma(t, t, 1.0));
-                    s = t * c;
-                    L.flag[0] = 1;
-                }
-                L.C[lane] = c;
-                L.S[lane] = s;
-                L.T[lane] = t;
-                L.P[lane] = pp;
-                L.Q[lane] = qq;
-            }
-            __syncthreads();
-            // A <- R^T A R on 2x2 blocks of rotation pairs (i <= j), mirrored
-            for (int w = lane; w < nblk; w += BS) {
-                const int e = L.blk[w];
-                const int i = e >> 16, j = e & 0xffff;
-                const double si = L.S[i];
-                const double sj = L.S[j];
-                if (si == 0.0 && sj == 0.0) continue;
-                const int pi = L.P[i], qi = L.Q[i];
-                if (i == j) {
-                    const double t = L.T[i];
-                    const double apq = L.A[pi * lda + qi];
-                    L.A[pi * lda + pi] -= t * apq;
-                    L.A[qi * lda + qi] += t * apq;
-                    L.A[pi * lda + qi] = 0.0;
-                    L.A[qi * lda + pi] = 0.0;
-                } else {
-                    const double ci = L.C[i], cj = L.C[j];
-                    const int pj = L.P[j], qj = L.Q[j];
-                    const double b00 = L.A[pi * lda + pj], b01 = L.A[pi * lda + qj];
-                    const double b10 = L.A[qi * lda + pj], b11 = L.A[qi * lda + qj];
-                    const double x0 = cj * b00 - sj * b01, x1 = sj * b00 + cj * b01;
-                    const double y0 = cj * b10 - sj * b11, y1 = sj * b10 + cj * b11;
-                    const double n00 = ci * x0 - si * y0, n10 = si * x0 + ci * y0;
-                    const double n01 = ci * x1 - si * y1, n11 = si * x1 + ci * y1;
-                    L.A[pi * lda + pj] = n00;
-                    L.A[pi * lda + qj] = n01;
-                    L.A[qi * lda + pj] = n10;
-                    L.A[qi * lda + qj] = n11;
-                    L.A[pj * lda + pi] = n00;
-                    L.A[qj * lda + pi] = n01;
-                    L.A[pj * lda + qi] = n10;
-                    L.A[qj * lda + qi] = n11;
-                }
-            }
-            // V <- V R (rows r < k, pair j)
-            {
-                int rr = lane / np, j = lane % np;
-                for (; rr < k;) {
-                    const double s = L.S[j];
-                    if (s != 0.0) {
-                        const double c = L.C[j];
-                        const int pj = L.P[j], qj = L.Q[j];
-                        const double v0 = L.V[rr * lda + pj], v1 = L.V[rr * lda + qj];
-                        L.V[rr * lda + pj] = c * v0 - s * v1;
-                        L.V[rr * lda + qj] = s * v0 + c * v1;
-                    }
-                    rr += dr;
-                    j += dj;
-                    if (j >= np) {
-                        j -= np;
-                        ++rr;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (L.flag[0] == 0) break;
-        __syncthreads();
-    }
-    return sweep < LEKF_JACOBI_MAX_SWEEPS ? sweep : -1;
+    const double p4 = (p1 - p3) / p2 - alpha;
+    const double qv = (alpha * p2 + p3) / p2;
+    const double so = 2.0 / p3 * (qv * qv);
+    const double sb2 = 0.04 * 0.04;
+    return alpha + sb2 / (so + sb2) * p4;
+}
+
+// ===========================================================================
+// k <= 32: fused kernel, one wave per point, register Jacobi
+// ===========================================================================
+
+struct SolveLds {
+    double* A;    // n x lda   (C, later T)
+    double* V;    // k x lda   (eigenvectors, columns)
+    double* B;    // k   (Y^T R^-1 d)
+    double* L;    // n   eigenvalues
+    double* U;    // k
+    double* Wb;   // k   w-bar
+    double* X;    // LEKF_NVMAX * k   state perturbation rows (aliases V)
+    double* M;    // LEKF_NVMAX       row means (aliases V)
+    double* sc;   // 8 scalars
+};
+
+__host__ __device__ inline size_t solve_lds_doubles(int k)
+{
+    const int n = (k + 1) & ~1, lda = n + 1;
+    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;  // apply rows reuse the V region
+    size_t d = 0;
+    d += (size_t)n * lda;                                    // A
+    d += (size_t)k * lda > vx ? (size_t)k * lda : vx;        // V, later X and M
+    d += (size_t)k + n + 2 * (size_t)k;                      // B L U Wb
+    d += 8;                                                  // scalars
+    return d;
+}
+
+__device__ inline SolveLds carve(double* s, int k)
+{
+    const int n = (k + 1) & ~1, lda = n + 1;
+    SolveLds L;
+    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;
+    L.A = s;
+    s += (size_t)n * lda;
+    L.V = s;
+    L.X = s;  // used only after T has been formed
+    L.M = s + (size_t)LEKF_NVMAX * k;
+    s += (size_t)k * lda > vx ? (size_t)k * lda : vx;
+    L.B = s;
+    s += k;
+    L.L = s;
+    s += n;
+    L.U = s;
+    s += k;
+    L.Wb = s;
+    s += k;
+    L.sc = s;
+    return L;
 }
 
 // Partner of index j in round r of the circle-method tournament on n (even)
-// indices: the same pairing as rr_pair, inverted.
+// indices: player 0 stays, the others rotate; over the n-1 rounds every pair
+// meets once.
 __device__ __forceinline__ int rr_partner(int n, int r, int j)
 {
     const int m = n - 1;
@@ -534,27 +427,19 @@ __device__ int jacobi_reg32(SolveLds& L, int k, int lane, int lda)
     return sweep < LEKF_JACOBI_MAX_SWEEPS ? sweep : -1;
 }
 
-// NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32), or
-// 0 for the LDS Jacobi (k > 32); MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3.
-// The register-Jacobi classes run one wave per point; the LDS-Jacobi classes
-// run four: the Gram is split over the waves by obs and summed into LDS in
-// wave order, and the Jacobi, W assembly and apply stride over all threads.
+// NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32);
+// MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3.
 template <int NT, int KR, int MODE>
-__global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
-    const LekfJob* __restrict__ jobs, int njobs)
+__global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restrict__ jobs, int njobs)
 {
-    constexpr int BS = KR > 0 ? 64 : 256;  // threads per point
-    constexpr int NW = BS / LEKF_WAVE;
     extern __shared__ double smem[];
     const long gp = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int lane = tid & (LEKF_WAVE - 1), wv = tid / LEKF_WAVE;
+    const int lane = threadIdx.x;
     const int ji = lekf_job_of(jobs, njobs, gp);
     const LekfJob& J = jobs[ji];
     const int b = (int)(gp - J.pt_begin + J.pt0);
     const int k = J.k;
-    const int n = (k + 1) & ~1, lda = n + 1, np = n / 2;
-    const int nblk = np * (np + 1) / 2;
+    const int n = (k + 1) & ~1, lda = n + 1;
     SolveLds L = carve(smem, k);
 
     const long off = J.loc_off[b];
@@ -563,55 +448,27 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
     const bool transm_mode = (J.flags & LEKF_F_TRANSM) != 0;
     int status = 0;
 
-    // LDS Jacobi (k > 32): zero A (dummy row/col), V = I, block table
-    constexpr bool reg_jacobi = KR > 0;
-    if (!reg_jacobi) {
-        for (int w = tid; w < n * lda; w += BS) L.A[w] = 0.0;
-        for (int w = tid; w < k * lda; w += BS) {
-            const int r = w / lda, c = w - r * lda;
-            L.V[w] = (r == c) ? 1.0 : 0.0;
-        }
-        for (int w = tid; w < nblk; w += BS) {
-            int i = 0, rem = w;
-            while (rem >= np - i) {
-                rem -= np - i;
-                ++i;
-            }
-            L.blk[w] = (i << 16) | (i + rem);
-        }
-        __syncthreads();
-    }
-
     if (p > 0) {
         double p3 = 0.0;
         if (MODE == 0) {
             lekf_d4 acc[NT * (NT + 1) / 2];
 #pragma unroll
             for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
-            gram_f64<NT>(J, off, p, k, lane, acc, p3, wv, NW);
-            for (int w = 0; w < NW; ++w) {
-                if (wv == w) store_gram<NT, true>(acc, k, lda, lane, L, w > 0);
-                if (NW > 1) __syncthreads();
-            }
+            gram_f64<NT>(J, off, p, k, lane, acc, p3, 0, 1);
+            store_gram<NT, true>(acc, k, lda, lane, L.A, L.B, &L.sc[0], false);
         } else {
             lekf_f4 acc[NT * (NT + 1) / 2];
 #pragma unroll
             for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
             if (MODE == 1)
-                gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, NW);
+                gram_f32<NT>(J, off, p, k, lane, acc, p3, 0, 1);
             else
-                gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, NW);
-            for (int w = 0; w < NW; ++w) {
-                if (wv == w) store_gram<NT, false>(acc, k, lda, lane, L, w > 0);
-                if (NW > 1) __syncthreads();
-            }
+                gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, 0, 1);
+            store_gram<NT, false>(acc, k, lda, lane, L.A, L.B, &L.sc[0], false);
         }
         p3 = wave_sum(p3);
-        if (lane == 0) L.sc[4 + wv] = p3;
         __syncthreads();
-        if (tid == 0) {
-            p3 = L.sc[4];
-            for (int w = 1; w < NW; ++w) p3 += L.sc[4 + w];
+        if (lane == 0) {
             double tr = 0.0;
             for (int i = 0; i < k; ++i) tr += L.A[i * lda + i];
             L.sc[1] = tr / (double)(k - 1);  // P2
@@ -623,14 +480,7 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
         }
         __syncthreads();
         alpha = L.sc[3];
-        int sweeps;
-        if constexpr (reg_jacobi) {
-            sweeps = jacobi_reg32<(KR > 0 ? KR : 4)>(L, k, lane, lda);
-        } else {
-            sweeps = jacobi_lds<BS>(L, k, tid);
-            __syncthreads();
-            for (int j = tid; j < k; j += BS) L.L[j] = L.A[j * lda + j];
-        }
+        const int sweeps = jacobi_reg32<KR>(L, k, lane, lda);
         if (sweeps < 0) status |= LEKF_PT_NOCONV;
         __syncthreads();
         // eigenvalues; reference truncation test (common_mtx.f90:66-74)
@@ -642,23 +492,23 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
         }
         if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
         // u = L^-1 V^T b ; w-bar = V u
-        for (int j = tid; j < k; j += BS) {
+        for (int j = lane; j < k; j += LEKF_WAVE) {
             double s = 0.0;
             for (int i = 0; i < k; ++i) s += L.V[i * lda + j] * L.B[i];
             L.U[j] = s / L.L[j];
         }
         __syncthreads();
-        for (int i = tid; i < k; i += BS) {
+        for (int i = lane; i < k; i += LEKF_WAVE) {
             double s = 0.0;
             for (int j = 0; j < k; ++j) s += L.V[i * lda + j] * L.U[j];
             L.Wb[i] = s;
         }
         // U <- sqrt((k-1)/lambda) (U no longer needed)
         __syncthreads();
-        for (int j = tid; j < k; j += BS) L.U[j] = sqrt((double)(k - 1) / L.L[j]);
+        for (int j = lane; j < k; j += LEKF_WAVE) L.U[j] = sqrt((double)(k - 1) / L.L[j]);
         __syncthreads();
         // T (into A) = V diag(U) V^T (+ w-bar)
-        for (int w = tid; w < k * k; w += BS) {
+        for (int w = lane; w < k * k; w += LEKF_WAVE) {
             const int i = w / k, m = w - i * k;
             double s = 0.0;
             for (int l = 0; l < k; ++l) s += L.V[i * lda + l] * L.U[l] * L.V[m * lda + l];
@@ -667,7 +517,7 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
-            for (int w = tid; w < k * k; w += BS) {
+            for (int w = lane; w < k * k; w += LEKF_WAVE) {
                 const int i = w / k, m = w - i * k;
                 double s = 0.0;
                 for (int l = 0; l < k; ++l) s += L.V[i * lda + l] / L.L[l] * L.V[m * lda + l];
@@ -675,46 +525,35 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
             }
         }
         if (J.transm)
-            for (int i = tid; i < k; i += BS) J.transm[(size_t)b * k + i] = L.Wb[i];
-        // adaptive inflation (common_letkf.f90:218-239)
-        if (tid == 0 && J.infl_out) {
-            double na = alpha;
-            if (J.flags & LEKF_F_FIXED_INFL) {
-                na = J.infl_fixed;
-            } else {
-                const double p1 = L.sc[0], p2 = L.sc[1], p3v = L.sc[2];
-                const double p4 = (p1 - p3v) / p2 - alpha;
-                const double qv = (alpha * p2 + p3v) / p2;
-                const double so = 2.0 / p3v * (qv * qv);
-                const double sb2 = 0.04 * 0.04;
-                na = alpha + sb2 / (so + sb2) * p4;
-            }
-            J.infl_out[b] = na;
-        }
+            for (int i = lane; i < k; i += LEKF_WAVE) J.transm[(size_t)b * k + i] = L.Wb[i];
+        if (lane == 0 && J.infl_out)
+            J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL)
+                                ? J.infl_fixed
+                                : adaptive_alpha(alpha, L.sc[0], L.sc[1], L.sc[2]);
     } else {
         // no local obs (common_letkf.f90:82-96)
         const double sa = sqrt(alpha);
-        for (int w = tid; w < k * k; w += BS) {
+        for (int w = lane; w < k * k; w += LEKF_WAVE) {
             const int i = w / k, m = w - i * k;
             L.A[i * lda + m] = (i == m) ? sa : 0.0;
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
-            for (int w = tid; w < k * k; w += BS) {
+            for (int w = lane; w < k * k; w += LEKF_WAVE) {
                 const int i = w / k, m = w - i * k;
                 pa[i + (size_t)m * k] = (i == m) ? alpha / (double)(k - 1) : 0.0;
             }
         }
         if (J.transm)
-            for (int i = tid; i < k; i += BS) J.transm[(size_t)b * k + i] = 0.0;
-        if (tid == 0 && J.infl_out)
+            for (int i = lane; i < k; i += LEKF_WAVE) J.transm[(size_t)b * k + i] = 0.0;
+        if (lane == 0 && J.infl_out)
             J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
     }
-    if (tid == 0 && J.status) J.status[b] = status;
+    if (lane == 0 && J.status) J.status[b] = status;
     __syncthreads();
     if (J.trans) {
         double* tr = J.trans + (size_t)b * k * k;
-        for (int w = tid; w < k * k; w += BS) {
+        for (int w = lane; w < k * k; w += LEKF_WAVE) {
             const int m = w / k, i = w - m * k;  // column-major: coalesced over i
             tr[i + (size_t)m * k] = L.A[i * lda + m];
         }
@@ -725,7 +564,7 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
     const int n3 = J.nv3d, n2 = (lev == 0) ? J.nv2d : 0;
     const int nrow = n3 + n2;
     const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
-    for (int w = tid; w < nrow * k; w += BS) {
+    for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
         const int r = w / k, m = w - r * k;
         const double* src = (r < n3) ? J.x3b + ((size_t)b * n3 + r) * k
                                      : J.x2b + ((size_t)ij * J.nv2d + (r - n3)) * k;
@@ -733,15 +572,15 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
     }
     __syncthreads();
     // ensemble mean in member order (speedy/common/common_speedy.f90:335-370)
-    for (int r = tid; r < nrow; r += BS) {
+    for (int r = lane; r < nrow; r += LEKF_WAVE) {
         double s = L.X[r * k];
         for (int m = 1; m < k; ++m) s += L.X[r * k + m];
         L.M[r] = s / (double)k;
     }
     __syncthreads();
-    for (int w = tid; w < nrow * k; w += BS) L.X[w] -= L.M[w / k];
+    for (int w = lane; w < nrow * k; w += LEKF_WAVE) L.X[w] -= L.M[w / k];
     __syncthreads();
-    for (int w = tid; w < nrow * k; w += BS) {
+    for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
         const int r = w / k, m = w - r * k;
         const bool upd = (r >= n3) || ((mask >> r) & 1u);
         double a = L.M[r];
@@ -756,17 +595,706 @@ __global__ __launch_bounds__(KR > 0 ? 64 : 256) void lekf_solve_kernel(
     }
 }
 
+// ===========================================================================
+// 32 < k <= LEKF_KMAX: Gram kernel + eigen/assembly/apply kernel
+// ===========================================================================
+
+// workspace record of one point: C (k x k, row-major), b (k), sum w d^2, P3
+__host__ __device__ inline long gram_stride(int kmax)
+{
+    return ((long)kmax * kmax + kmax + 2 + 1) & ~1L;
+}
+
+constexpr int kGramWaves = 4;
+
+// Four waves per point; wave w takes every fourth group of obs, and the
+// partial Grams are summed in LDS in wave order (deterministic).
+template <int NT, int MODE>
+__global__ __launch_bounds__(64 * kGramWaves) void lekf_gram_kernel(
+    const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (LEKF_WAVE - 1), wv = tid / LEKF_WAVE;
+    const int ji = lekf_job_of(jobs, njobs, gp);
+    const LekfJob& J = jobs[ji];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int k = J.k, lda = k + 1;
+    const long off = J.loc_off[b];
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
+    if (p <= 0) return;
+    double* A = smem;
+    double* B = A + (size_t)k * lda;
+    double* sc = B + k;  // [0] sum w d^2, [4 + w] P3 of wave w
+    double p3 = 0.0;
+    if (MODE == 0) {
+        lekf_d4 acc[NT * (NT + 1) / 2];
+#pragma unroll
+        for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
+        gram_f64<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
+        for (int w = 0; w < kGramWaves; ++w) {
+            if (wv == w) store_gram<NT, true>(acc, k, lda, lane, A, B, &sc[0], w > 0);
+            __syncthreads();
+        }
+    } else {
+        lekf_f4 acc[NT * (NT + 1) / 2];
+#pragma unroll
+        for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+        if (MODE == 1)
+            gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
+        else
+            gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
+        for (int w = 0; w < kGramWaves; ++w) {
+            if (wv == w) store_gram<NT, false>(acc, k, lda, lane, A, B, &sc[0], w > 0);
+            __syncthreads();
+        }
+    }
+    p3 = wave_sum(p3);
+    if (lane == 0) sc[4 + wv] = p3;
+    __syncthreads();
+    double* out = ws + (gp - gp0) * gstride;
+    for (int t = tid; t < k * k; t += 64 * kGramWaves) {
+        const int i = t / k, j = t - i * k;
+        out[t] = A[i * lda + j];
+    }
+    for (int i = tid; i < k; i += 64 * kGramWaves) out[(size_t)k * k + i] = B[i];
+    if (tid == 0) {
+        double s = sc[4];
+        for (int w = 1; w < kGramWaves; ++w) s += sc[4 + w];
+        out[(size_t)k * k + k] = sc[0];
+        out[(size_t)k * k + k + 1] = s;
+    }
+}
+
+// LDS of the eigen/assembly kernel (doubles)
+struct EigLds {
+    double* Z;    // k x lda: C, then Householder vectors + Q, then eigenvectors (columns)
+    double* d;    // k  diagonal -> eigenvalues
+    double* e;    // k  sub-diagonal
+    double* tau;  // k  Householder scalars
+    double* rot;  // 2k (c, s) of one QL sweep
+    double* B;    // k  Y^T R^-1 d
+    double* U;    // k
+    double* Wb;   // k  w-bar
+    double* sc;   // 8
+    double* X;    // nvmax * k  perturbation rows
+    double* R;    // nvmax * k
+    double* M;    // nvmax  row means
+    double* Cw;   // nvmax  X' w-bar
+};
+
+__host__ __device__ inline int eig_lda(int k) { return k | 1; }
+
+__host__ __device__ inline size_t eig_lds_doubles(int k, int nvmax)
+{
+    return (size_t)k * eig_lda(k) + 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 2 * (size_t)nvmax;
+}
+
+__device__ inline EigLds eig_carve(double* s, int k, int nvmax)
+{
+    EigLds L;
+    L.Z = s;
+    s += (size_t)k * eig_lda(k);
+    L.d = s;
+    s += k;
+    L.e = s;
+    s += k;
+    L.tau = s;
+    s += k;
+    L.rot = s;
+    s += 2 * k;
+    L.B = s;
+    s += k;
+    L.U = s;
+    s += k;
+    L.Wb = s;
+    s += k;
+    L.sc = s;
+    s += 8;
+    L.X = s;
+    s += (size_t)nvmax * k;
+    L.R = s;
+    s += (size_t)nvmax * k;
+    L.M = s;
+    s += nvmax;
+    L.Cw = s;
+    return L;
+}
+
+// Householder reduction of the symmetric n x n matrix in Z to tridiagonal
+// T = Q^T A Q (diagonal d, sub-diagonal e), then Q accumulated in place in Z.
+// Step j reflects x = A[j+1.., j] onto beta e_1 with H = I - tau v v^T,
+// v_0 = 1, beta = -sign(x_0) |x|; the trailing block is updated as
+// A -= v w^T + w v^T with p = tau A v, w = p - (tau/2)(v.p) v.  One wave;
+// lane r owns rows/columns r and r + 64 (n <= 128).
+__device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
+{
+    double* A = L.Z;
+    for (int j = 0; j + 2 < n; ++j) {
+        const int o = j + 1, m = n - o;
+        double x0 = 0.0, x1 = 0.0;  // x_r for r = lane, lane + 64
+        if (lane < m) x0 = A[(o + lane) * lda + j];
+        if (lane + 64 < m) x1 = A[(o + lane + 64) * lda + j];
+        const double sigma = wave_sum((lane > 0 ? x0 * x0 : 0.0) + x1 * x1);
+        const double alpha = lane_bcast(x0, 0);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (sigma > 0.0) {
+            const double nrm = sqrt(fma(alpha, alpha, sigma));
+            beta = alpha <= 0.0 ? nrm : -nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        double v0 = lane == 0 ? 1.0 : x0 * scal, v1 = x1 * scal;
+        if (tau == 0.0) {
+            v0 = lane == 0 ? 1.0 : 0.0;
+            v1 = 0.0;
+        }
+        if (lane == 0) {
+            L.d[j] = A[j * lda + j];
+            L.e[j] = beta;
+            L.tau[j] = tau;
+        }
+        if (lane < m) A[(o + lane) * lda + j] = v0;
+        if (lane + 64 < m) A[(o + lane + 64) * lda + j] = v1;
+        __syncthreads();
+        if (tau == 0.0) continue;
+        // p = tau A22 v (A22 symmetric: lane r reads column r, consecutive over lanes)
+        double p0 = 0.0, p1 = 0.0;
+#pragma unroll 4
+        for (int c = 0; c < m; ++c) {
+            const double vc = lane_bcast(c < 64 ? v0 : v1, c & 63);
+            const double* row = A + (o + c) * lda + o;
+            if (lane < m) p0 = fma(row[lane], vc, p0);
+            if (lane + 64 < m) p1 = fma(row[lane + 64], vc, p1);
+        }
+        p0 *= tau;
+        p1 *= tau;
+        const double K = 0.5 * tau * wave_sum(p0 * v0 + p1 * v1);
+        const double w0 = fma(-K, v0, p0), w1 = fma(-K, v1, p1);
+        // A22 -= v w^T + w v^T (lane r updates column r).  Both products are
+        // rounded and then added, so elements (c, r) and (r, c) get the same
+        // value and A22 stays exactly symmetric.
+#pragma unroll 4
+        for (int c = 0; c < m; ++c) {
+            const double vc = lane_bcast(c < 64 ? v0 : v1, c & 63);
+            const double wc = lane_bcast(c < 64 ? w0 : w1, c & 63);
+            double* row = A + (o + c) * lda + o;
+            if (lane < m)
+                row[lane] = __dsub_rn(row[lane], __dadd_rn(__dmul_rn(vc, w0), __dmul_rn(wc, v0)));
+            if (lane + 64 < m)
+                row[lane + 64] =
+                    __dsub_rn(row[lane + 64], __dadd_rn(__dmul_rn(vc, w1), __dmul_rn(wc, v1)));
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        if (n >= 2) {
+            L.d[n - 2] = A[(n - 2) * lda + n - 2];
+            L.e[n - 2] = A[(n - 1) * lda + n - 2];
+        }
+        L.d[n - 1] = A[(n - 1) * lda + n - 1];
+        L.e[n - 1] = 0.0;
+    }
+    __syncthreads();
+    // Q = H_0 H_1 ... H_{n-3}, accumulated from the right end in place: before
+    // step j, row and column j+1 become unit vectors (the reflector stored
+    // there has been used), then Q[j+1.., j+1..] -= tau v (v^T Q[j+1.., j+1..]).
+    const int t0 = n >= 2 ? n - 2 : 0;
+    for (int r = lane; r < n - t0; r += 64)
+        for (int c = 0; c < n - t0; ++c) A[(t0 + c) * lda + t0 + r] = (r == c) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int j = n - 3; j >= 0; --j) {
+        const int o = j + 1, m = n - o;
+        // unit row/column o
+        for (int r = lane; r < m; r += 64) {
+            A[o * lda + o + r] = r == 0 ? 1.0 : 0.0;
+            A[(o + r) * lda + o] = r == 0 ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        const double tau = L.tau[j];
+        if (tau == 0.0) continue;
+        double v0 = 0.0, v1 = 0.0;
+        if (lane < m) v0 = A[(o + lane) * lda + j];
+        if (lane + 64 < m) v1 = A[(o + lane + 64) * lda + j];
+        double y0 = 0.0, y1 = 0.0;
+#pragma unroll 4
+        for (int c = 0; c < m; ++c) {
+            const double vc = lane_bcast(c < 64 ? v0 : v1, c & 63);
+            const double* row = A + (o + c) * lda + o;
+            if (lane < m) y0 = fma(vc, row[lane], y0);
+            if (lane + 64 < m) y1 = fma(vc, row[lane + 64], y1);
+        }
+        y0 *= tau;
+        y1 *= tau;
+#pragma unroll 4
+        for (int c = 0; c < m; ++c) {
+            const double vc = lane_bcast(c < 64 ? v0 : v1, c & 63);
+            double* row = A + (o + c) * lda + o;
+            if (lane < m) row[lane] = fma(-vc, y0, row[lane]);
+            if (lane + 64 < m) row[lane + 64] = fma(-vc, y1, row[lane + 64]);
+        }
+        __syncthreads();
+    }
+    // row/column 0
+    for (int r = lane; r < n; r += 64) {
+        A[r] = r == 0 ? 1.0 : 0.0;
+        A[r * lda] = r == 0 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+}
+
+constexpr int kQlMaxIter = 60;
+
+// Rotations (c_i, s_i), i = m-1 down to ilo, of one QL sweep applied to the
+// columns (i, i+1) of Z, row by row (lane = row): the new z_{i+1} is stored,
+// the new z_i carried to the next step.
+__device__ __forceinline__ void ql_apply(EigLds& L, int n, int lda, int lane, int ilo, int m)
+{
+    for (int r = lane; r < n; r += 64) {
+        double* z = L.Z + r * lda;
+        double z1 = z[m];
+#pragma unroll 4
+        for (int i = m - 1; i >= ilo; --i) {
+            const double c = L.rot[2 * i], s = L.rot[2 * i + 1];
+            const double z0 = z[i];
+            z[i + 1] = fma(s, z0, c * z1);
+            z1 = fma(c, z0, -s * z1);
+        }
+        z[ilo] = z1;
+    }
+}
+
+// Eigenvalues (into d) and eigenvectors (columns of Z, which holds Q on
+// entry) of the symmetric tridiagonal (d, e) by implicit QL with shifts
+// (Bowdler, Martin, Reinsch and Wilkinson 1968, the method of EISPACK tql2).
+// A sub-diagonal element is negligible when adding it to the running norm
+// estimate tst = max(|d_l| + |e_l|) does not change it.  Every lane runs
+// the scalar recurrence; the rotations of a sweep are kept in L.rot and
+// then applied to Z by all lanes.  Returns nonzero if an eigenvalue needed
+// more than kQlMaxIter sweeps.
+__device__ int eig_ql(EigLds& L, int n, int lda, int lane)
+{
+    double* d = L.d;
+    double* e = L.e;
+    double tst = 0.0;
+    for (int l = 0; l < n; ++l) {
+        tst = fmax(tst, fabs(d[l]) + fabs(e[l]));
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m < n - 1; ++m)
+                if (tst + fabs(e[m]) == tst) break;
+            if (m == l) break;
+            if (iter++ == kQlMaxIter) return 1;
+            const double dl = d[l], el = e[l];
+            double g = (d[l + 1] - dl) / (2.0 * el);
+            double r = fabs(g) > 1.0e100 ? fabs(g) : sqrt(fma(g, g, 1.0));
+            g = d[m] - dl + el / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = m - 1;
+            bool split = false;
+            double ei = e[i], di1 = d[i + 1];
+            for (; i >= l; --i) {
+                const double di = d[i];
+                const double eprev = i > l ? e[i - 1] : 0.0;  // next step's e, loaded early
+                const double f = s * ei, bb = c * ei;
+                r = sqrt(fma(f, f, g * g));
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = di1 - p;
+                    e[m] = 0.0;
+                    split = true;
+                    break;
+                }
+                const double ri = rcp_nr(r);
+                s = f * ri;
+                c = g * ri;
+                g = di1 - p;
+                r = fma(di - g, s, 2.0 * c * bb);
+                p = s * r;
+                d[i + 1] = g + p;
+                g = fma(c, r, -bb);
+                if (lane == 0) {
+                    L.rot[2 * i] = c;
+                    L.rot[2 * i + 1] = s;
+                }
+                ei = eprev;
+                di1 = di;
+            }
+            __syncthreads();
+            ql_apply(L, n, lda, lane, split ? i + 1 : l, m);
+            if (!split) {
+                d[l] = d[l] - p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+    return 0;
+}
+
+// The same Householder + implicit-QL eigendecomposition with the matrices in
+// registers, for n <= KR <= 64.  Lane r holds column r of A (a[t] = A[t][r])
+// and row r of Q, later of Z (q[t] = Z[r][t]); d_i and e_i live in lane i
+// and are read by the others with readlane.  Loops over matrix indices run
+// over the static index t with uniform branches, so every register index is
+// a compile-time constant.
+//   Tridiagonalisation, step j: lane j's column gives x = A[j+1.., j] and is
+//   overwritten by v; p = tau A v and y = tau Q v are lane-local dot products
+//   with the broadcast v_t; A -= v w^T + w v^T and Q -= y v^T (Q accumulates
+//   H_0 H_1 ... forward, row by row).
+//   QL: a sweep runs the scalar recurrence in every lane and rotates columns
+//   (i, i+1) of the lane's Z row as it goes.
+// Reads C (+ the diagonal shift) from Z in LDS, writes the eigenvectors back to
+// Z (columns) and the eigenvalues to d.  Returns nonzero if QL did not
+// converge.
+template <int KR>
+__device__ int eig_reg(EigLds& L, int n, int lda, int lane)
+{
+    double a[KR], q[KR];
+#pragma unroll
+    for (int t = 0; t < KR; ++t) {
+        a[t] = (t < n && lane < n) ? L.Z[t * lda + lane] : 0.0;
+        q[t] = (t == lane) ? 1.0 : 0.0;
+    }
+    double dreg = 0.0, ereg = 0.0;
+    for (int j = 0; j + 2 < n; ++j) {
+        // alpha = A[j+1][j], sigma = sum_{t > j+1} A[t][j]^2 (from lane j)
+        double al = 0.0, sg = 0.0, dj = 0.0;
+#pragma unroll
+        for (int t = 0; t < KR; ++t) {
+            if (t == j) dj = a[t];
+            if (t == j + 1) al = a[t];
+            if (t > j + 1 && t < n) sg = fma(a[t], a[t], sg);
+        }
+        const double alpha = lane_bcast(al, j), sigma = lane_bcast(sg, j);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (sigma > 0.0) {
+            const double nrm = sqrt(fma(alpha, alpha, sigma));
+            beta = alpha <= 0.0 ? nrm : -nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        if (lane == j) {
+            dreg = dj;
+            ereg = beta;
+        }
+        if (tau == 0.0) continue;
+        // lane j: column j becomes v (v_{j+1} = 1)
+        if (lane == j) {
+#pragma unroll
+            for (int t = 0; t < KR; ++t) {
+                if (t == j + 1) a[t] = 1.0;
+                if (t > j + 1 && t < n) a[t] *= scal;
+            }
+        }
+        // p = tau A v (lane = column), y = tau Q v (lane = row), own v_r
+        double pr = 0.0, yr = 0.0, vr = 0.0;
+#pragma unroll
+        for (int t = 0; t < KR; ++t) {
+            if (t > j && t < n) {
+                const double vt = lane_bcast(a[t], j);
+                pr = fma(a[t], vt, pr);
+                yr = fma(q[t], vt, yr);
+                if (t == lane) vr = vt;
+            }
+        }
+        pr *= tau;
+        yr *= tau;
+        const bool act = lane > j && lane < n;
+        const double K = 0.5 * tau * wave_sum(act ? vr * pr : 0.0);
+        const double wr = fma(-K, vr, pr);
+        // A -= v w^T + w v^T on the trailing block (products rounded, then
+        // added: elements (t, r) and (r, t) stay equal); Q -= y v^T
+#pragma unroll
+        for (int t = 0; t < KR; ++t) {
+            if (t > j && t < n) {
+                const double vt = lane_bcast(a[t], j);
+                const double wt = lane_bcast(wr, t);
+                if (act) a[t] = __dsub_rn(a[t], __dadd_rn(__dmul_rn(vt, wr), __dmul_rn(wt, vr)));
+                q[t] = fma(-yr, vt, q[t]);
+            }
+        }
+    }
+    // last 2 x 2 block
+    {
+        double d0 = 0.0, d1 = 0.0, e0 = 0.0;
+#pragma unroll
+        for (int t = 0; t < KR; ++t) {
+            if (t == n - 2) d0 = a[t];
+            if (t == n - 1) {
+                d1 = a[t];
+                e0 = a[t];
+            }
+        }
+        if (n >= 2) {
+            const double dn2 = lane_bcast(d0, n - 2), en2 = lane_bcast(e0, n - 2);
+            if (lane == n - 2) {
+                dreg = dn2;
+                ereg = en2;
+            }
+        }
+        const double dn1 = lane_bcast(d1, n - 1);
+        if (lane == n - 1) {
+            dreg = dn1;
+            ereg = 0.0;
+        }
+    }
+    // implicit QL
+    double tst = 0.0;
+    int bad = 0;
+    for (int l = 0; l < n && !bad; ++l) {
+        tst = fmax(tst, fabs(lane_bcast(dreg, l)) + fabs(lane_bcast(ereg, l)));
+        for (int iter = 0;; ++iter) {
+            const bool negl = lane >= l && lane < n - 1 && (tst + fabs(ereg) == tst);
+            const unsigned long long mask =
+                (__ballot(negl) | (1ull << (n - 1))) & (~0ull << l);
+            const int m = __builtin_ctzll(mask);
+            if (m == l) break;
+            if (iter == kQlMaxIter) {
+                bad = 1;
+                break;
+            }
+            const double dl = lane_bcast(dreg, l), el = lane_bcast(ereg, l);
+            double g = (lane_bcast(dreg, l + 1) - dl) / (2.0 * el);
+            double r = fabs(g) > 1.0e100 ? fabs(g) : sqrt(fma(g, g, 1.0));
+            g = lane_bcast(dreg, m) - dl + el / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0, di1 = lane_bcast(dreg, m);
+            bool split = false;
+#pragma unroll
+            for (int i = KR - 2; i >= 0; --i) {
+                if (i >= l && i < m && !split) {
+                    const double ei = lane_bcast(ereg, i), di = lane_bcast(dreg, i);
+                    const double f = s * ei, bb = c * ei;
+                    const double x = fma(f, f, g * g);
+                    if (x == 0.0) {
+                        if (lane == i + 1) dreg = di1 - p;
+                        if (lane == m) ereg = 0.0;
+                        split = true;
+                    } else {
+                        const double y = rsq_nr(x);
+                        r = x * y;  // hypot(f, g)
+                        s = f * y;
+                        c = g * y;
+                        if (lane == i + 1) ereg = r;
+                        g = di1 - p;
+                        r = fma(di - g, s, 2.0 * c * bb);
+                        p = s * r;
+                        if (lane == i + 1) dreg = g + p;
+                        g = fma(c, r, -bb);
+                        di1 = di;
+                        const double h = q[i + 1];
+                        q[i + 1] = fma(s, q[i], c * h);
+                        q[i] = fma(c, q[i], -s * h);
+                    }
+                }
+            }
+            if (!split) {
+                if (lane == l) {
+                    dreg -= p;
+                    ereg = g;
+                }
+                if (lane == m) ereg = 0.0;
+            }
+        }
+    }
+    // eigenvalues and eigenvectors (columns of Z) to LDS
+    if (lane < n) {
+        L.d[lane] = dreg;
+#pragma unroll
+        for (int t = 0; t < KR; ++t)
+            if (t < n) L.Z[lane * lda + t] = q[t];
+    }
+    __syncthreads();
+    return bad;
+}
+
+// One wave per point: C and sums from the Gram workspace, eigendecomposition
+// (KR > 0: in registers for k <= KR; KR = 0: in LDS), W assembly, inflation,
+// the optional trans/transm/pao outputs and the fused apply
+// Xa = x-mean + X' T evaluated as x-mean + (X' V) diag(sqrt((k-1)/L)) V^T
+// + (X' w-bar) 1^T.
+template <int KR>
+__global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                      long gp0, const double* __restrict__ ws,
+                                                      long gstride, int nvmax)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const int lane = threadIdx.x;
+    const int ji = lekf_job_of(jobs, njobs, gp);
+    const LekfJob& J = jobs[ji];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int k = J.k, lda = eig_lda(k);
+    EigLds L = eig_carve(smem, k, nvmax);
+    const long off = J.loc_off[b];
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
+    double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
+    const bool transm_mode = (J.flags & LEKF_F_TRANSM) != 0;
+    int status = 0;
+    const double sa = sqrt(alpha);  // p = 0 branch: T = sqrt(alpha) I
+
+    if (p > 0) {
+        const double* g = ws + (gp - gp0) * gstride;
+        for (int t = lane; t < k * k; t += 64) {
+            const int i = t / k, j = t - i * k;
+            L.Z[i * lda + j] = g[t];
+        }
+        for (int i = lane; i < k; i += 64) L.B[i] = g[(size_t)k * k + i];
+        const double p1 = g[(size_t)k * k + k], p3 = g[(size_t)k * k + k + 1];
+        __syncthreads();
+        // P2 = trace(C) / (k-1), summed in index order as the reference does
+        double tr = 0.0;
+        for (int i = 0; i < k; ++i) tr += L.Z[i * lda + i];
+        const double p2 = tr / (double)(k - 1);
+        if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
+        const double rho = 1.0 / alpha;
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += (double)(k - 1) * rho;
+        __syncthreads();
+        if constexpr (KR > 0) {
+            if (eig_reg<(KR > 0 ? KR : 4)>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
+        } else {
+            eig_tridiag(L, k, lda, lane);
+            if (eig_ql(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
+        }
+        __syncthreads();
+        // reference truncation test (common_mtx.f90:66-74)
+        double lmax = 0.0, lmin = 1.0e308;
+        for (int j = 0; j < k; ++j) {
+            const double lj = L.d[j];
+            lmax = lj > lmax ? lj : lmax;
+            lmin = lj < lmin ? lj : lmin;
+        }
+        if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
+        // u = L^-1 V^T b ; w-bar = V u ; U <- sqrt((k-1)/lambda)
+        for (int j = lane; j < k; j += 64) {
+            double s = 0.0;
+            for (int i = 0; i < k; ++i) s = fma(L.Z[i * lda + j], L.B[i], s);
+            L.U[j] = s / L.d[j];
+        }
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s = fma(L.Z[i * lda + j], L.U[j], s);
+            L.Wb[i] = s;
+        }
+        __syncthreads();
+        for (int j = lane; j < k; j += 64) L.U[j] = sqrt((double)(k - 1) / L.d[j]);
+        __syncthreads();
+        if (J.trans) {
+            double* tr = J.trans + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += 64) {
+                const int m = w / k, i = w - m * k;  // column-major, coalesced over i
+                double s = 0.0;
+                for (int l = 0; l < k; ++l) s = fma(L.Z[i * lda + l] * L.U[l], L.Z[m * lda + l], s);
+                if (!transm_mode) s += L.Wb[i];
+                tr[i + (size_t)m * k] = s;
+            }
+        }
+        if (J.pao) {
+            double* pa = J.pao + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += 64) {
+                const int m = w / k, i = w - m * k;
+                double s = 0.0;
+                for (int l = 0; l < k; ++l) s = fma(L.Z[i * lda + l] / L.d[l], L.Z[m * lda + l], s);
+                pa[i + (size_t)m * k] = s;
+            }
+        }
+        if (J.transm)
+            for (int i = lane; i < k; i += 64) J.transm[(size_t)b * k + i] = L.Wb[i];
+        if (lane == 0 && J.infl_out)
+            J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed
+                                                          : adaptive_alpha(alpha, p1, p2, p3);
+    } else {
+        // no local obs (common_letkf.f90:82-96)
+        if (J.trans) {
+            double* tr = J.trans + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += 64) {
+                const int m = w / k, i = w - m * k;
+                tr[i + (size_t)m * k] = (i == m) ? sa : 0.0;
+            }
+        }
+        if (J.pao) {
+            double* pa = J.pao + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += 64) {
+                const int m = w / k, i = w - m * k;
+                pa[i + (size_t)m * k] = (i == m) ? alpha / (double)(k - 1) : 0.0;
+            }
+        }
+        if (J.transm)
+            for (int i = lane; i < k; i += 64) J.transm[(size_t)b * k + i] = 0.0;
+        if (lane == 0 && J.infl_out)
+            J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
+    }
+    if (lane == 0 && J.status) J.status[b] = status;
+    if (!(J.flags & LEKF_F_APPLY)) return;
+    // fused apply: rows of the point (3-D at every level, 2-D at level 0)
+    const int lev = b / J.nij, ij = b - lev * J.nij;
+    const int n3 = J.nv3d, n2 = (lev == 0) ? J.nv2d : 0;
+    const int nrow = n3 + n2;
+    const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
+    for (int w = lane; w < nrow * k; w += 64) {
+        const int r = w / k, m = w - r * k;
+        const double* src = (r < n3) ? J.x3b + ((size_t)b * n3 + r) * k
+                                     : J.x2b + ((size_t)ij * J.nv2d + (r - n3)) * k;
+        L.X[r * k + m] = src[m];
+    }
+    __syncthreads();
+    // ensemble mean in member order (speedy/common/common_speedy.f90:335-370)
+    for (int r = lane; r < nrow; r += 64) {
+        double s = L.X[r * k];
+        for (int m = 1; m < k; ++m) s += L.X[r * k + m];
+        L.M[r] = s / (double)k;
+    }
+    __syncthreads();
+    for (int w = lane; w < nrow * k; w += 64) L.X[w] -= L.M[w / k];
+    __syncthreads();
+    if (p > 0) {
+        // R = (X' V) diag(U), Cw = X' w-bar
+        for (int w = lane; w < nrow * k; w += 64) {
+            const int r = w / k, l = w - r * k;
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s = fma(L.X[r * k + j], L.Z[j * lda + l], s);
+            L.R[r * k + l] = s * L.U[l];
+        }
+        for (int r = lane; r < nrow; r += 64) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s = fma(L.X[r * k + j], L.Wb[j], s);
+            L.Cw[r] = s;
+        }
+        __syncthreads();
+    }
+    for (int w = lane; w < nrow * k; w += 64) {
+        const int r = w / k, m = w - r * k;
+        const bool upd = (r >= n3) || ((mask >> r) & 1u);
+        double a;
+        if (!upd) {
+            a = L.M[r] + L.X[r * k + m];
+        } else if (p > 0) {
+            double s = 0.0;
+            for (int l = 0; l < k; ++l) s = fma(L.R[r * k + l], L.Z[m * lda + l], s);
+            a = L.M[r] + (L.Cw[r] + s);
+        } else {
+            a = L.M[r] + sa * L.X[r * k + m];
+        }
+        double* dst = (r < n3) ? J.x3a + ((size_t)b * n3 + r) * k
+                               : J.x2a + ((size_t)ij * J.nv2d + (r - n3)) * k;
+        dst[m] = a;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// host-side launcher (called from lekf_capi.hip)
+// host-side launchers (called from lekf_capi.hip)
 // ---------------------------------------------------------------------------
-size_t lekf_solve_lds_bytes(int k) { return solve_lds_doubles(k) * sizeof(double); }
-
 int lekf_solve_nt(int k) { return (k + 1 + 15) / 16; }
 
-// Kernel class of an ensemble size: KR * 8 + NT.  k <= 32 uses the register
-// Jacobi with KR = k rounded up to a multiple of 4; larger k the LDS Jacobi.
+// Kernel class of an ensemble size: KR * 8 + NT for k <= 32 (register
+// Jacobi, KR = k rounded up to a multiple of 4); NT for larger k.
 int lekf_solve_class(int k)
 {
     if (k <= 32) {
@@ -777,45 +1305,114 @@ int lekf_solve_class(int k)
     return lekf_solve_nt(k);
 }
 
-template <int NT, int KR, int MODE>
-static hipError_t launch_k(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
+size_t lekf_solve_lds_bytes(int k, int nvmax)
 {
-    const size_t lds = lekf_solve_lds_bytes(kmax);
-    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt),
-                       dim3(KR > 0 ? LEKF_WAVE : 4 * LEKF_WAVE), lds, st, djobs, njobs);
+    if (k <= 32) return solve_lds_doubles(k) * sizeof(double);
+    return eig_lds_doubles(k, nvmax) * sizeof(double);
+}
+
+static constexpr size_t kGramWsCap = 1ull << 30;
+
+size_t lekf_solve_ws_bytes(long npt, int kmax)
+{
+    if (kmax <= 32 || npt <= 0) return 0;
+    const size_t rec = (size_t)gram_stride(kmax) * sizeof(double);
+    size_t per = kGramWsCap / rec;
+    if (per < 1) per = 1;
+    if ((size_t)npt < per) per = (size_t)npt;
+    return per * rec;
+}
+
+template <int NT, int KR, int MODE>
+static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
+{
+    const size_t lds = solve_lds_doubles(kmax) * sizeof(double);
+    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt), dim3(LEKF_WAVE), lds,
+                       st, djobs, njobs);
     return hipGetLastError();
+}
+
+// register eigensolver variant of an ensemble size: k rounded up to a multiple
+// of 4 for k <= 64, 0 (LDS eigensolver) above
+static int eig_kr(int kmax) { return kmax <= 64 ? (kmax + 3) & ~3 : 0; }
+
+static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const LekfJob* djobs,
+                             int njobs, long g0, const double* ws, long gs, int nvmax)
+{
+    switch (kr) {
+#define LEKF_EIG_CASE(K)                                                                       \
+    case K:                                                                                    \
+        hipLaunchKernelGGL(lekf_eig_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
+                           djobs, njobs, g0, ws, gs, nvmax);                                  \
+        break;
+        LEKF_EIG_CASE(36)
+        LEKF_EIG_CASE(40)
+        LEKF_EIG_CASE(44)
+        LEKF_EIG_CASE(48)
+        LEKF_EIG_CASE(52)
+        LEKF_EIG_CASE(56)
+        LEKF_EIG_CASE(60)
+        LEKF_EIG_CASE(64)
+        LEKF_EIG_CASE(0)
+#undef LEKF_EIG_CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int NT, int MODE>
+static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
+                               void* ws, size_t ws_bytes, hipStream_t st)
+{
+    const long gs = gram_stride(kmax);
+    const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
+    if (per < 1 || !ws) return hipErrorInvalidValue;
+    const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 8) * sizeof(double);
+    const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
+    for (long g0 = 0; g0 < npt; g0 += per) {
+        const long nb = per < npt - g0 ? per : npt - g0;
+        hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE>), dim3((unsigned)nb), dim3(64 * kGramWaves),
+                           glds, st, djobs, njobs, g0, (double*)ws, gs);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (const double*)ws, gs, nvmax);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <int MODE>
 static hipError_t launch_mode(int cls, const LekfJob* djobs, int njobs, long npt, int kmax,
-                              hipStream_t st)
+                              int nvmax, void* ws, size_t ws_bytes, hipStream_t st)
 {
     switch (cls) {
-    case 4 * 8 + 1: return launch_k<1, 4, MODE>(djobs, njobs, npt, kmax, st);
-    case 8 * 8 + 1: return launch_k<1, 8, MODE>(djobs, njobs, npt, kmax, st);
-    case 12 * 8 + 1: return launch_k<1, 12, MODE>(djobs, njobs, npt, kmax, st);
-    case 16 * 8 + 2: return launch_k<2, 16, MODE>(djobs, njobs, npt, kmax, st);
-    case 20 * 8 + 2: return launch_k<2, 20, MODE>(djobs, njobs, npt, kmax, st);
-    case 24 * 8 + 2: return launch_k<2, 24, MODE>(djobs, njobs, npt, kmax, st);
-    case 28 * 8 + 2: return launch_k<2, 28, MODE>(djobs, njobs, npt, kmax, st);
-    case 32 * 8 + 3: return launch_k<3, 32, MODE>(djobs, njobs, npt, kmax, st);
-    case 3: return launch_k<3, 0, MODE>(djobs, njobs, npt, kmax, st);
-    case 4: return launch_k<4, 0, MODE>(djobs, njobs, npt, kmax, st);
-    case 5: return launch_k<5, 0, MODE>(djobs, njobs, npt, kmax, st);
-    case 6: return launch_k<6, 0, MODE>(djobs, njobs, npt, kmax, st);
+    case 4 * 8 + 1: return launch_fused<1, 4, MODE>(djobs, njobs, npt, kmax, st);
+    case 8 * 8 + 1: return launch_fused<1, 8, MODE>(djobs, njobs, npt, kmax, st);
+    case 12 * 8 + 1: return launch_fused<1, 12, MODE>(djobs, njobs, npt, kmax, st);
+    case 16 * 8 + 2: return launch_fused<2, 16, MODE>(djobs, njobs, npt, kmax, st);
+    case 20 * 8 + 2: return launch_fused<2, 20, MODE>(djobs, njobs, npt, kmax, st);
+    case 24 * 8 + 2: return launch_fused<2, 24, MODE>(djobs, njobs, npt, kmax, st);
+    case 28 * 8 + 2: return launch_fused<2, 28, MODE>(djobs, njobs, npt, kmax, st);
+    case 32 * 8 + 3: return launch_fused<3, 32, MODE>(djobs, njobs, npt, kmax, st);
+    case 3: return launch_split<3, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    case 4: return launch_split<4, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    case 5: return launch_split<5, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    case 6: return launch_split<6, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 // Launch the solve over npt flat points of njobs jobs (device array djobs).
-// Every job of a launch has the same kernel class; kmax (the largest k) sets
-// the LDS size.
-hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int mode,
-                             hipStream_t st)
+// Every job of a launch has the same kernel class; kmax (the largest k) and
+// nvmax (the most state rows a point applies to) size the LDS; ws is the
+// Gram workspace of lekf_solve_ws_bytes(npt, kmax) bytes (k > 32).
+hipError_t lekf_solve_launch(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
+                             int mode, void* ws, size_t ws_bytes, hipStream_t st)
 {
     if (npt <= 0) return hipSuccess;
+    if (nvmax > LEKF_NVMAX) return hipErrorInvalidValue;
     const int cls = lekf_solve_class(kmax);
-    if (mode == 0) return launch_mode<0>(cls, djobs, njobs, npt, kmax, st);
-    if (mode == 1) return launch_mode<1>(cls, djobs, njobs, npt, kmax, st);
-    return launch_mode<2>(cls, djobs, njobs, npt, kmax, st);
+    if (mode == 0) return launch_mode<0>(cls, djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    if (mode == 1) return launch_mode<1>(cls, djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    return launch_mode<2>(cls, djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
 }

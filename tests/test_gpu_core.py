@@ -27,7 +27,9 @@ def _rel(a, b):
 
 
 CASES = [(0, 20), (1, 20), (4, 20), (21, 20), (200, 20), (57, 3), (13, 5), (100, 16), (100, 31),
-         (100, 32), (200, 40), (1000, 50), (300, 63), (200, 64), (150, 80), (120, 95)]
+         (100, 32), (200, 40), (1000, 50), (300, 63), (200, 64), (150, 80), (120, 95),
+         # p < k: C has k - p equal eigenvalues (k-1)/alpha (clustered spectrum)
+         (0, 40), (1, 33), (5, 50), (30, 64), (10, 95)]
 
 
 @pytest.mark.parametrize("p,k", CASES)
@@ -79,10 +81,10 @@ def test_bad_sizes_raise(gpu):
     assert e.value.code == 99
 
 
-def test_batch_matches_single_and_reference(gpu):
+@pytest.mark.parametrize("k", [20, 50])
+def test_batch_matches_single_and_reference(gpu, k):
     """Many uneven local problems in one launch; each equals its lone solve bit for bit."""
     from letkf_amd.device import DeviceArray, sync
-    k = 20
     ps = [0, 1, 3, 17, 64, 150, 333, 5, 200, 90]
     nobs = 2000
     rng = np.random.default_rng(3)
