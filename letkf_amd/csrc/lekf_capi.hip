@@ -466,7 +466,11 @@ struct lekf_handle {
     int nbkt = 1;
     double dlon_zero = 0;  // regional rectangle half-width in grid units (dlat_zero above)
     DevBuf d_zlev, cellcum, cap_n, cap_off, o_ri, o_rj, o_z, o_bkt, r_dep, r_bkt, r_z;
-    DevBuf gws;  // Gram workspace of the k > 32 solve
+    DevBuf gws;                      // Gram workspace of the k > 32 solve
+    DevBuf sel_key, sel_ob, sel_rho;  // per-slot scratch of the capped selection
+    DevBuf maxcand;
+    long slot_cap = 0;
+    int sel_slots = 0;
     std::vector<long> chunk_pts;  // chunk boundaries (points)
     std::vector<long> chunk_off;  // list offsets at the boundaries
     long list_budget = 512L * 1024 * 1024;
@@ -488,7 +492,8 @@ long handle_bytes(const lekf_handle* h)
                           &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx,
                           &h->loc_rho, &h->status, &h->djob, &h->d_zlev, &h->cellcum,
                           &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
-                          &h->r_dep, &h->r_bkt, &h->r_z, &h->gws};
+                          &h->r_dep, &h->r_bkt, &h->r_z, &h->gws, &h->sel_key,
+                          &h->sel_ob, &h->sel_rho, &h->maxcand};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -506,7 +511,7 @@ void free_all(lekf_handle* h)
                     &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
                     &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
                     &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
-                    &h->gws};
+                    &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->maxcand};
     for (auto* b : bs) b->release();
 }
 
@@ -636,6 +641,10 @@ RegionalArgs make_reg_args(const lekf_handle* h)
     R.loc_cnt = h->loc_cnt.as<int>();
     R.loc_idx = h->loc_idx.as<int>();
     R.loc_rho = h->loc_rho.as<double>();
+    R.s_key = h->sel_key.as<unsigned long long>();
+    R.s_ob = h->sel_ob.as<int>();
+    R.s_rho = h->sel_rho.as<double>();
+    R.slot_cap = h->slot_cap;
     return R;
 }
 
@@ -644,7 +653,7 @@ int reg_select_chunk(lekf_handle* h, int c)
     RegionalArgs R = make_reg_args(h);
     R.pt_begin = h->chunk_pts[c];
     R.list_base = h->chunk_off[c];
-    HIPCHK(lekf_reg_select_launch(R, h->chunk_pts[c + 1] - h->chunk_pts[c], h->st));
+    HIPCHK(lekf_reg_select_launch(R, h->chunk_pts[c + 1] - h->chunk_pts[c], h->sel_slots, h->st));
     h->held_chunk = c;
     return 0;
 }
@@ -1116,6 +1125,7 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     hipStream_t st = h->st;
     const int ne = h->ne, nx = h->r.nx, ny = h->r.ny;
     int nb = 1;
+    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;  // obs index and bucket share 32 bits
     for (int n = 0; n < nobs; ++n) {
         if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
         nb = std::max(nb, bucket[n] + 1);
@@ -1193,7 +1203,12 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     HIPCHK(h->cap_n.ensure(sizeof(int) * npt));
     HIPCHK(h->cap_off.ensure(sizeof(long) * (npt + 1)));
     RegionalArgs R = make_reg_args(h);
-    HIPCHK(lekf_reg_capacity_launch(R, h->cap_n.as<int>(), h->cap_off.as<long>(), st));
+    HIPCHK(h->maxcand.ensure(sizeof(int)));
+    HIPCHK(hipMemsetAsync(h->maxcand.p, 0, sizeof(int), st));
+    HIPCHK(lekf_reg_capacity_launch(R, h->cap_n.as<int>(), h->cap_off.as<long>(),
+                                    h->maxcand.as<int>(), st));
+    int maxcand = 0;
+    HIPCHK(hipMemcpyAsync(&maxcand, h->maxcand.p, sizeof(int), hipMemcpyDeviceToHost, st));
     std::vector<long> off(npt + 1);
     HIPCHK(hipMemcpyAsync(off.data(), h->cap_off.p, sizeof(long) * (npt + 1), hipMemcpyDeviceToHost,
                           st));
@@ -1213,6 +1228,16 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     HIPCHK(h->loc_idx.ensure(sizeof(int) * maxlist));
     HIPCHK(h->loc_rho.ensure(sizeof(double) * maxlist));
     h->loc_cap = maxlist;
+    // capped selection: one scratch list per workgroup slot, as long as the
+    // largest rectangle's candidate count
+    h->sel_slots = lekf_reg_select_slots();
+    h->slot_cap = h->r.nobs_max > 0 ? std::max(maxcand, 1) : 0;
+    if (h->slot_cap > 0) {
+        const size_t ne_ = (size_t)h->sel_slots * (size_t)h->slot_cap;
+        HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * ne_));
+        HIPCHK(h->sel_ob.ensure(sizeof(int) * ne_));
+        HIPCHK(h->sel_rho.ensure(sizeof(double) * ne_));
+    }
     h->have_obs = true;
     return upload_jobs(h);
 }

@@ -12,9 +12,10 @@
 // (the reference's (element, obs type) pair) keeps its nobs_max smallest
 // (rdiag, scan position) pairs -- exactly the set the reference's insertion
 // sort keeps (strict < keeps earlier obs on ties).  Instead of inserting
-// serially, one wave per point finds each bucket's threshold pair with
-// radix-digit histogram passes over the 64-bit key (10-bit digits in LDS),
-// resolves the last digit bin by sorting at most LEKF_REG_LIST candidates,
+// serially, a workgroup per point evaluates every candidate once into a
+// scratch list in scan order, finds each bucket's threshold pair with
+// radix-digit histogram passes over the list's 64-bit keys (10-bit digits in
+// LDS), resolves the last digit bin by sorting at most LEKF_REG_LIST entries,
 // and emits the kept obs bucket by bucket in scan order.  Without a cap the
 // accepted obs are emitted in scan order.
 #include "lekf_dev.hpp"
@@ -29,23 +30,29 @@ __device__ __forceinline__ unsigned long long lane_lt(int lane)
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-struct RegLds {
+constexpr int kSelT = 256;  // threads per workgroup of the selection (4 waves)
+constexpr int kSelW = kSelT / 64;
+constexpr int kSelU = 1;    // candidates / list entries per thread per round
+
+struct SelLds {
     int hist[LEKF_REG_NB][1024];
+    unsigned long long lbits[LEKF_REG_NB][LEKF_REG_LIST];
+    int lpos[LEKF_REG_NB][LEKF_REG_LIST];
+    int seg_beg[LEKF_REG_SEG];
+    int seg_pre[LEKF_REG_SEG + 1];
     int cnt[LEKF_REG_NB];
     int need[LEKF_REG_NB];
     int bincnt[LEKF_REG_NB];
-    int mode[LEKF_REG_NB];  // 0 keep all, 1 refine, 2 collect, 3 tie scan, 4 resolved
+    int mode[LEKF_REG_NB];  // 0 keep all, 1 refine, 2 collect, 3 tie scan
     int shift[LEKF_REG_NB];
+    int lcnt[LEKF_REG_NB];
+    int thr_pos[LEKF_REG_NB];
+    int run[LEKF_REG_NB];
+    int base[LEKF_REG_NB + 1];
     unsigned long long pref[LEKF_REG_NB];
     unsigned long long thr_bits[LEKF_REG_NB];
-    int thr_q[LEKF_REG_NB];
-    int lcnt[LEKF_REG_NB];
-    unsigned long long lbits[LEKF_REG_NB][LEKF_REG_LIST];
-    int lq[LEKF_REG_NB][LEKF_REG_LIST];
-    int seg_beg[LEKF_REG_SEG];
-    int seg_pre[LEKF_REG_SEG + 1];
-    int outc[LEKF_REG_NB];
-    int base[LEKF_REG_NB + 1];
+    int wcnt[kSelW][LEKF_REG_NB];
+    int ecnt[kSelW][kSelU][LEKF_REG_NB];
     int any;
 };
 
@@ -56,33 +63,51 @@ struct Cand {
     unsigned long long bits;
 };
 
-__device__ __forceinline__ Cand eval_cand(const RegionalArgs& A, int o, double ri, double rj,
-                                          double rz)
+struct ObsRec {
+    double ri, rj, z, err;
+    int b;
+};
+
+__device__ __forceinline__ ObsRec load_obs(const RegionalArgs& A, int o)
+{
+    ObsRec r;
+    r.ri = A.ori[o];
+    r.rj = A.orj[o];
+    r.z = A.oz[o];
+    r.err = A.oerr[o];
+    r.b = A.obkt[o];
+    return r;
+}
+
+// The reference's test and weights (scale/letkf/letkf_tools.f90:978-1037),
+// each operation rounded as written (no contraction), so the cutoff decision
+// and rdiag are bit-identical to the reference's.
+__device__ __forceinline__ Cand eval_rec(const RegionalArgs& A, const ObsRec& r, int o, double ri,
+                                         double rj, double rz)
 {
     Cand c;
-    const double rdx = __dmul_rn(__dsub_rn(ri, A.ori[o]), A.dx);
-    const double rdy = __dmul_rn(__dsub_rn(rj, A.orj[o]), A.dy);
+    const double rdx = __dmul_rn(__dsub_rn(ri, r.ri), A.dx);
+    const double rdy = __dmul_rn(__dsub_rn(rj, r.rj), A.dy);
     const double nd_h = __ddiv_rn(__dsqrt_rn(__dadd_rn(__dmul_rn(rdx, rdx), __dmul_rn(rdy, rdy))),
                                   A.sigma_h);
-    const double nd_v = __ddiv_rn(fabs(__dsub_rn(A.oz[o], rz)), A.sigma_v);
+    const double nd_v = __ddiv_rn(fabs(__dsub_rn(r.z, rz)), A.sigma_v);
     const double nd = __dadd_rn(__dmul_rn(nd_h, nd_h), __dmul_rn(nd_v, nd_v));
     c.o = o;
     c.ok = !(nd > kNdMax);
-    c.b = A.obkt[o];
+    c.b = r.b;
     c.rho = 0.0;
     c.rd = 0.0;
     c.bits = 0;
     if (c.ok) {
         c.rho = exp(-0.5 * nd);
-        const double e = A.oerr[o];
-        c.rd = __ddiv_rn(__dmul_rn(e, e), c.rho);
+        c.rd = __ddiv_rn(__dmul_rn(r.err, r.err), c.rho);
         c.bits = (unsigned long long)__double_as_longlong(c.rd);
     }
     return c;
 }
 
 // candidate q of the point's scan (row segments in LDS)
-__device__ __forceinline__ int cand_obs(const RegLds& S, int nseg, int q, int& seg)
+__device__ __forceinline__ int cand_obs(const SelLds& S, int nseg, int q, int& seg)
 {
     while (seg + 1 < nseg && S.seg_pre[seg + 1] <= q) ++seg;
     return S.seg_beg[seg] + (q - S.seg_pre[seg]);
@@ -114,248 +139,336 @@ __device__ __forceinline__ int next_shift(int shift)
     return shift == 3 ? 0 : (shift == 0 ? -1 : shift - 10);
 }
 
-__global__ __launch_bounds__(64) void reg_select(RegionalArgs A)
+// Block-ordered append: every thread with flag set gets a distinct position
+// in block (thread) order after the S.run[slot] entries already placed; the
+// per-slot running count advances by the block's total.  Uniform call.
+__device__ __forceinline__ int block_append(SelLds& S, bool flag, int slot, int tid)
 {
-    __shared__ RegLds S;
-    const long b = A.pt_begin + blockIdx.x;  // global point = lev * nij + ij
-    const int lane = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const unsigned long long m = __ballot(flag);
+    if (lane == 0) S.wcnt[wv][slot] = __popcll(m);
+    __syncthreads();
+    int off = S.run[slot];
+    for (int w = 0; w < wv; ++w) off += S.wcnt[w][slot];
+    const int pos = off + __popcll(m & lane_lt(lane));
+    __syncthreads();
+    if (tid == 0) {
+        int t = 0;
+        for (int w = 0; w < kSelW; ++w) t += S.wcnt[w][slot];
+        S.run[slot] += t;
+    }
+    __syncthreads();
+    return pos;
+}
+
+// Persistent selection: workgroup g handles points g, g + gridDim.x, ...
+// Each candidate of a point is evaluated once.  Without a cap the accepted
+// candidates are written in scan order.  With a cap they go to the
+// workgroup's scratch slot in scan order (rdiag bits, obs | bucket << 30,
+// rho); each over-full bucket's threshold pair (key, list position) is
+// found by digit histograms over the list, and the kept entries are written
+// bucket by bucket in scan order.
+__global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
+{
+    __shared__ SelLds S;
+    const int tid = threadIdx.x;
     const int nij = A.nx * A.ny;
-    const int lev = (int)(b / nij), ij = (int)(b % nij);
-    const double ri = (double)(ij % A.nx + 1), rj = (double)(ij / A.nx + 1), rz = A.zlev[lev];
     const int nb = A.nbkt;
     const int cap = A.nobs_max;
-    int imin = (int)floor(ri - A.dlon_zero), imax = (int)ceil(ri + A.dlon_zero);
-    int jmin = (int)floor(rj - A.dlat_zero), jmax = (int)ceil(rj + A.dlat_zero);
-    if (imin < 1) imin = 1;
-    if (imax > A.nx) imax = A.nx;
-    if (jmin < 1) jmin = 1;
-    if (jmax > A.ny) jmax = A.ny;
-    int nseg = (imin > imax || jmin > jmax) ? 0 : jmax - jmin + 1;
-    if (lane == 0) {
-        int acc = 0;
-        for (int s = 0; s < nseg; ++s) {
-            const int j = jmin + s;  // 1-based row
-            // nobsgrd(i, j) = cumulative count through cell (i, j); (0, j) = end of row j-1
-            const long rowbase = (long)(j - 1) * A.nx;
-            const int beg = (imin > 1) ? A.cellcum[rowbase + imin - 2] : (rowbase > 0 ? A.cellcum[rowbase - 1] : 0);
-            const int end = A.cellcum[rowbase + imax - 1];
-            S.seg_beg[s] = beg;
-            S.seg_pre[s] = acc;
-            acc += end - beg;
-        }
-        S.seg_pre[nseg] = acc;
-        S.any = 0;
-    }
-    for (int t = lane; t < LEKF_REG_NB; t += 64) {
-        S.cnt[t] = 0;
-        S.lcnt[t] = 0;
-        S.outc[t] = 0;
-        S.mode[t] = 0;
-    }
-    for (int t = lane; t < LEKF_REG_NB * 1024; t += 64) (&S.hist[0][0])[t] = 0;
-    __syncthreads();
-    const int ncand = nseg ? S.seg_pre[nseg] : 0;
-    const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
+    unsigned long long* skey = A.s_key + (size_t)blockIdx.x * A.slot_cap;
+    int* sob = A.s_ob + (size_t)blockIdx.x * A.slot_cap;
+    double* srho = A.s_rho + (size_t)blockIdx.x * A.slot_cap;
 
-    if (cap <= 0) {
-        // no cap: accepted candidates in scan order
-        int seg = 0, n = 0;
-        for (int q0 = 0; q0 < ncand; q0 += 64) {
-            const int q = q0 + lane;
-            Cand c;
-            c.ok = false;
-            if (q < ncand) c = eval_cand(A, cand_obs(S, nseg, q, seg), ri, rj, rz);
-            const unsigned long long m = __ballot(c.ok);
-            if (c.ok) {
-                const long w = obase + n + __popcll(m & lane_lt(lane));
-                A.loc_idx[w] = c.o;
-                A.loc_rho[w] = c.rho;
+    for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
+        const long b = A.pt_begin + pi;  // global point = lev * nij + ij
+        const int lev = (int)(b / nij), ij = (int)(b % nij);
+        const double ri = (double)(ij % A.nx + 1), rj = (double)(ij / A.nx + 1), rz = A.zlev[lev];
+        int imin = (int)floor(ri - A.dlon_zero), imax = (int)ceil(ri + A.dlon_zero);
+        int jmin = (int)floor(rj - A.dlat_zero), jmax = (int)ceil(rj + A.dlat_zero);
+        if (imin < 1) imin = 1;
+        if (imax > A.nx) imax = A.nx;
+        if (jmin < 1) jmin = 1;
+        if (jmax > A.ny) jmax = A.ny;
+        const int nseg = (imin > imax || jmin > jmax) ? 0 : jmax - jmin + 1;
+        if (tid == 0) {
+            int acc = 0;
+            for (int s = 0; s < nseg; ++s) {
+                const int j = jmin + s;  // 1-based row
+                // cellcum = inclusive cumulative count through cell (i, j), row-major
+                const long rowbase = (long)(j - 1) * A.nx;
+                const int beg = (imin > 1) ? A.cellcum[rowbase + imin - 2]
+                                           : (rowbase > 0 ? A.cellcum[rowbase - 1] : 0);
+                const int end = A.cellcum[rowbase + imax - 1];
+                S.seg_beg[s] = beg;
+                S.seg_pre[s] = acc;
+                acc += end - beg;
             }
-            n += __popcll(m);
+            S.seg_pre[nseg] = acc;
         }
-        if (lane == 0) {
-            A.loc_off[b] = obase;
-            A.loc_cnt[b] = n;
+        for (int t = tid; t < LEKF_REG_NB; t += kSelT) {
+            S.cnt[t] = 0;
+            S.lcnt[t] = 0;
+            S.mode[t] = 0;
+            S.run[t] = 0;
         }
-        return;
-    }
+        for (int t = tid; t < LEKF_REG_NB * 1024; t += kSelT) (&S.hist[0][0])[t] = 0;
+        __syncthreads();
+        const int ncand = S.seg_pre[nseg];
+        const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
 
-    // pass 0: counts and top-digit histograms per bucket
-    {
-        int seg = 0;
-        for (int q0 = 0; q0 < ncand; q0 += 64) {
-            const int q = q0 + lane;
-            if (q < ncand) {
-                const Cand c = eval_cand(A, cand_obs(S, nseg, q, seg), ri, rj, rz);
-                if (c.ok) {
-                    atomicAdd(&S.cnt[c.b], 1);
-                    atomicAdd(&S.hist[c.b][digit(c.bits, 53)], 1);
+        // evaluate every candidate once: kSelU per thread per round, all loads
+        // of a round issued first; scan position q = q0 + u * kSelT + tid
+        int nacc = 0;
+        {
+            const int lane = tid & 63, wv = tid >> 6;
+            int seg = 0;
+            for (int q0 = 0; q0 < ncand; q0 += kSelT * kSelU) {
+                int o[kSelU];
+                ObsRec r[kSelU];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    const int q = q0 + u * kSelT + tid;
+                    o[u] = q < ncand ? cand_obs(S, nseg, q, seg) : -1;
+                    if (o[u] >= 0) r[u] = load_obs(A, o[u]);
+                }
+                Cand c[kSelU];
+                unsigned long long m[kSelU];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    c[u].ok = false;
+                    c[u].b = 0;
+                    if (o[u] >= 0) c[u] = eval_rec(A, r[u], o[u], ri, rj, rz);
+                    m[u] = __ballot(c[u].ok);
+                }
+                if (lane == 0)
+#pragma unroll
+                    for (int u = 0; u < kSelU; ++u) S.wcnt[wv][u] = __popcll(m[u]);
+                __syncthreads();
+                int pos[kSelU];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    int before = 0, tot = 0;
+                    for (int w = 0; w < kSelW; ++w) {
+                        const int cw = S.wcnt[w][u];
+                        before += w < wv ? cw : 0;
+                        tot += cw;
+                    }
+                    pos[u] = nacc + before + __popcll(m[u] & lane_lt(lane));
+                    nacc += tot;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    if (!c[u].ok) continue;
+                    if (cap <= 0) {
+                        A.loc_idx[obase + pos[u]] = c[u].o;
+                        A.loc_rho[obase + pos[u]] = c[u].rho;
+                    } else {
+                        skey[pos[u]] = c[u].bits;
+                        sob[pos[u]] = c[u].o | (c[u].b << 30);
+                        srho[pos[u]] = c[u].rho;
+                        atomicAdd(&S.cnt[c[u].b], 1);
+                        atomicAdd(&S.hist[c[u].b][digit(c[u].bits, 53)], 1);
+                    }
                 }
             }
         }
-    }
-    __syncthreads();
-    // choose the boundary digit of every bucket over its cap
-    if (lane < nb) {
-        const int t = lane;
-        if (S.cnt[t] > cap) {
-            int need = cap, d = 0, acc = 0;
-            for (; d < 1024; ++d) {
-                if (acc + S.hist[t][d] >= need) break;
-                acc += S.hist[t][d];
+        __syncthreads();
+        if (cap <= 0) {
+            if (tid == 0) {
+                A.loc_off[b] = obase;
+                A.loc_cnt[b] = nacc;
             }
-            S.need[t] = need - acc;
-            S.pref[t] = (unsigned long long)d << 53;
-            S.bincnt[t] = S.hist[t][d];
-            S.shift[t] = 43;
-            S.mode[t] = S.bincnt[t] <= LEKF_REG_LIST ? 2 : 1;
+            __syncthreads();
+            continue;
         }
-    }
-    __syncthreads();
-    // refine passes while a boundary bin is too large to sort
-    for (int pass = 0; pass < 8; ++pass) {
-        if (lane == 0) {
-            int a = 0;
-            for (int t = 0; t < nb; ++t) a |= (S.mode[t] == 1);
-            S.any = a;
-        }
-        __syncthreads();
-        if (!S.any) break;
-        for (int t = 0; t < nb; ++t)
-            if (S.mode[t] == 1)
-                for (int u = lane; u < 1024; u += 64) S.hist[t][u] = 0;
-        __syncthreads();
-        int seg = 0;
-        for (int q0 = 0; q0 < ncand; q0 += 64) {
-            const int q = q0 + lane;
-            if (q < ncand) {
-                const Cand c = eval_cand(A, cand_obs(S, nseg, q, seg), ri, rj, rz);
-                if (c.ok && S.mode[c.b] == 1 && prefix_match(c.bits, S.pref[c.b], S.shift[c.b]))
-                    atomicAdd(&S.hist[c.b][digit(c.bits, S.shift[c.b])], 1);
-            }
-        }
-        __syncthreads();
-        if (lane < nb && S.mode[lane] == 1) {
-            const int t = lane;
-            const int sh = S.shift[t];
-            int need = S.need[t], d = 0, acc = 0;
-            for (; d < 1024; ++d) {
-                if (acc + S.hist[t][d] >= need) break;
-                acc += S.hist[t][d];
-            }
-            S.need[t] = need - acc;
-            S.pref[t] |= (unsigned long long)d << sh;
-            S.bincnt[t] = S.hist[t][d];
-            const int ns = next_shift(sh);
-            S.shift[t] = ns;
-            if (S.bincnt[t] <= LEKF_REG_LIST)
-                S.mode[t] = 2;
-            else if (ns < 0)
-                S.mode[t] = 3;  // every key bit fixed: equal keys, break ties by scan order
-        }
-        __syncthreads();
-    }
-    // collect the boundary bins (sortable) / resolve exact ties in scan order
-    {
-        int seg = 0, seen[LEKF_REG_NB];
-        for (int t = 0; t < LEKF_REG_NB; ++t) seen[t] = 0;
-        for (int q0 = 0; q0 < ncand; q0 += 64) {
-            const int q = q0 + lane;
-            Cand c;
-            c.ok = false;
-            if (q < ncand) c = eval_cand(A, cand_obs(S, nseg, q, seg), ri, rj, rz);
-            bool coll = c.ok && S.mode[c.b] == 2 && prefix_match(c.bits, S.pref[c.b], S.shift[c.b]);
-            if (coll) {
-                const int pos = atomicAdd(&S.lcnt[c.b], 1);
-                if (pos < LEKF_REG_LIST) {
-                    S.lbits[c.b][pos] = c.bits;
-                    S.lq[c.b][pos] = q;
+        // boundary digit of every bucket over its cap
+        if (tid < nb) {
+            const int t = tid;
+            if (S.cnt[t] > cap) {
+                int need = cap, d = 0, acc = 0;
+                for (; d < 1024; ++d) {
+                    if (acc + S.hist[t][d] >= need) break;
+                    acc += S.hist[t][d];
                 }
-            }
-            for (int t = 0; t < nb; ++t) {
-                if (S.mode[t] != 3) continue;
-                const bool tie = c.ok && c.b == t && c.bits == S.pref[t];
-                const unsigned long long m = __ballot(tie);
-                const int before = seen[t];
-                const int rank = before + __popcll(m & lane_lt(lane));
-                if (tie && rank == S.need[t] - 1) {
-                    S.thr_bits[t] = c.bits;
-                    S.thr_q[t] = q;
-                }
-                seen[t] = before + __popcll(m);
+                S.need[t] = need - acc;
+                S.pref[t] = (unsigned long long)d << 53;
+                S.bincnt[t] = S.hist[t][d];
+                S.shift[t] = 43;
+                S.mode[t] = S.bincnt[t] <= LEKF_REG_LIST ? 2 : 1;
             }
         }
-    }
-    __syncthreads();
-    // sort each collected list by (bits, q) and take the need-th element
-    for (int t = 0; t < nb; ++t) {
-        if (S.mode[t] != 2) continue;
-        const int n = S.lcnt[t] < LEKF_REG_LIST ? S.lcnt[t] : LEKF_REG_LIST;
-        // odd-even transposition sort, n <= LEKF_REG_LIST
-        for (int ph = 0; ph < n; ++ph) {
-            for (int i = 2 * lane + (ph & 1); i + 1 < n; i += 128) {
-                const unsigned long long a = S.lbits[t][i], c = S.lbits[t][i + 1];
-                const int qa = S.lq[t][i], qc = S.lq[t][i + 1];
-                if (a > c || (a == c && qa > qc)) {
-                    S.lbits[t][i] = c;
-                    S.lbits[t][i + 1] = a;
-                    S.lq[t][i] = qc;
-                    S.lq[t][i + 1] = qa;
+        __syncthreads();
+        // refine while a boundary bin is too large to sort
+        for (int pass = 0; pass < 8; ++pass) {
+            if (tid == 0) {
+                int a = 0;
+                for (int t = 0; t < nb; ++t) a |= (S.mode[t] == 1);
+                S.any = a;
+            }
+            __syncthreads();
+            if (!S.any) break;
+            for (int t = 0; t < nb; ++t)
+                if (S.mode[t] == 1)
+                    for (int u = tid; u < 1024; u += kSelT) S.hist[t][u] = 0;
+            __syncthreads();
+            for (int i = tid; i < nacc; i += kSelT) {
+                const unsigned long long bits = skey[i];
+                const int t = sob[i] >> 30;
+                if (S.mode[t] == 1 && prefix_match(bits, S.pref[t], S.shift[t]))
+                    atomicAdd(&S.hist[t][digit(bits, S.shift[t])], 1);
+            }
+            __syncthreads();
+            if (tid < nb && S.mode[tid] == 1) {
+                const int t = tid;
+                const int sh = S.shift[t];
+                int need = S.need[t], d = 0, acc = 0;
+                for (; d < 1024; ++d) {
+                    if (acc + S.hist[t][d] >= need) break;
+                    acc += S.hist[t][d];
                 }
+                S.need[t] = need - acc;
+                S.pref[t] |= (unsigned long long)d << sh;
+                S.bincnt[t] = S.hist[t][d];
+                const int ns = next_shift(sh);
+                S.shift[t] = ns;
+                if (S.bincnt[t] <= LEKF_REG_LIST)
+                    S.mode[t] = 2;
+                else if (ns < 0)
+                    S.mode[t] = 3;  // every key bit fixed: equal keys, break ties by position
             }
             __syncthreads();
         }
-        if (lane == 0) {
-            S.thr_bits[t] = S.lbits[t][S.need[t] - 1];
-            S.thr_q[t] = S.lq[t][S.need[t] - 1];
+        // collect the boundary bins; resolve exact ties in list (scan) order
+        for (int t = tid; t < LEKF_REG_NB; t += kSelT) S.run[t] = 0;
+        __syncthreads();
+        for (int i0 = 0; i0 < nacc; i0 += kSelT) {
+            const int i = i0 + tid;
+            bool in = i < nacc;
+            const unsigned long long bits = in ? skey[i] : 0ull;
+            const int t = in ? (sob[i] >> 30) : 0;
+            if (in && S.mode[t] == 2 && prefix_match(bits, S.pref[t], S.shift[t])) {
+                const int pos = atomicAdd(&S.lcnt[t], 1);
+                if (pos < LEKF_REG_LIST) {
+                    S.lbits[t][pos] = bits;
+                    S.lpos[t][pos] = i;
+                }
+            }
+            for (int u = 0; u < nb; ++u) {
+                if (S.mode[u] != 3) continue;
+                const bool tie = in && t == u && bits == S.pref[u];
+                const int rank = block_append(S, tie, u, tid);
+                if (tie && rank == S.need[u] - 1) {
+                    S.thr_bits[u] = bits;
+                    S.thr_pos[u] = i;
+                }
+            }
         }
         __syncthreads();
-    }
-    if (lane == 0) {
-        int acc = 0;
+        // sort each collected bin by (bits, position); the need-th is the threshold
         for (int t = 0; t < nb; ++t) {
-            S.base[t] = acc;
-            acc += S.cnt[t] < cap ? S.cnt[t] : cap;
-        }
-        S.base[nb] = acc;
-    }
-    __syncthreads();
-    // emit bucket by bucket in scan order
-    {
-        int seg = 0;
-        int run[LEKF_REG_NB];
-        for (int t = 0; t < LEKF_REG_NB; ++t) run[t] = 0;
-        for (int q0 = 0; q0 < ncand; q0 += 64) {
-            const int q = q0 + lane;
-            Cand c;
-            c.ok = false;
-            if (q < ncand) c = eval_cand(A, cand_obs(S, nseg, q, seg), ri, rj, rz);
-            bool keep = false;
-            if (c.ok) {
-                const int t = c.b;
-                keep = (S.mode[t] == 0) || c.bits < S.thr_bits[t] ||
-                       (c.bits == S.thr_bits[t] && q <= S.thr_q[t]);
-            }
-            for (int t = 0; t < nb; ++t) {
-                const bool mine = keep && c.b == t;
-                const unsigned long long m = __ballot(mine);
-                if (mine) {
-                    const long w = obase + S.base[t] + run[t] + __popcll(m & lane_lt(lane));
-                    A.loc_idx[w] = c.o;
-                    A.loc_rho[w] = c.rho;
+            if (S.mode[t] != 2) continue;
+            const int n = S.lcnt[t] < LEKF_REG_LIST ? S.lcnt[t] : LEKF_REG_LIST;
+            for (int ph = 0; ph < n; ++ph) {
+                const int i = 2 * tid + (ph & 1);
+                if (i + 1 < n) {
+                    const unsigned long long x = S.lbits[t][i], y = S.lbits[t][i + 1];
+                    const int px = S.lpos[t][i], py = S.lpos[t][i + 1];
+                    if (x > y || (x == y && px > py)) {
+                        S.lbits[t][i] = y;
+                        S.lbits[t][i + 1] = x;
+                        S.lpos[t][i] = py;
+                        S.lpos[t][i + 1] = px;
+                    }
                 }
-                run[t] += __popcll(m);
+                __syncthreads();
+            }
+            if (tid == 0) {
+                S.thr_bits[t] = S.lbits[t][S.need[t] - 1];
+                S.thr_pos[t] = S.lpos[t][S.need[t] - 1];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            int acc = 0;
+            for (int t = 0; t < nb; ++t) {
+                S.base[t] = acc;
+                acc += S.cnt[t] < cap ? S.cnt[t] : cap;
+                S.run[t] = 0;
+            }
+            S.base[nb] = acc;
+        }
+        __syncthreads();
+        // emit bucket by bucket in scan order (list position i = i0 + u * kSelT + tid)
+        {
+            const int lane = tid & 63, wv = tid >> 6;
+            int runb[LEKF_REG_NB];
+#pragma unroll
+            for (int t = 0; t < LEKF_REG_NB; ++t) runb[t] = 0;
+            for (int i0 = 0; i0 < nacc; i0 += kSelT * kSelU) {
+                unsigned long long bits[kSelU];
+                int ob[kSelU];
+                bool keep[kSelU];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    const int i = i0 + u * kSelT + tid;
+                    const bool in = i < nacc;
+                    bits[u] = in ? skey[i] : 0ull;
+                    ob[u] = in ? sob[i] : 0;
+                    const int t = ob[u] >> 30;
+                    keep[u] = in && (S.mode[t] == 0 || bits[u] < S.thr_bits[t] ||
+                                     (bits[u] == S.thr_bits[t] && i <= S.thr_pos[t]));
+                }
+                unsigned long long m[kSelU][LEKF_REG_NB];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                    for (int t = 0; t < LEKF_REG_NB; ++t) m[u][t] = __ballot(keep[u] && (ob[u] >> 30) == t);
+                if (lane == 0)
+#pragma unroll
+                    for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                        for (int t = 0; t < LEKF_REG_NB; ++t) S.ecnt[wv][u][t] = __popcll(m[u][t]);
+                __syncthreads();
+                int pos[kSelU];
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) pos[u] = -1;
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                    for (int t = 0; t < LEKF_REG_NB; ++t) {
+                        int before = 0, tot = 0;
+                        for (int w = 0; w < kSelW; ++w) {
+                            const int cw = S.ecnt[w][u][t];
+                            before += w < wv ? cw : 0;
+                            tot += cw;
+                        }
+                        if (keep[u] && (ob[u] >> 30) == t)
+                            pos[u] = runb[t] + before + __popcll(m[u][t] & lane_lt(lane));
+                        runb[t] += tot;
+                    }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u) {
+                    const int t = ob[u] >> 30;
+                    if (keep[u] && pos[u] < S.base[t + 1] - S.base[t]) {
+                        const long w = obase + S.base[t] + pos[u];
+                        A.loc_idx[w] = ob[u] & 0x3fffffff;
+                        A.loc_rho[w] = srho[i0 + u * kSelT + tid];
+                    }
+                }
             }
         }
-    }
-    if (lane == 0) {
-        A.loc_off[b] = obase;
-        A.loc_cnt[b] = S.base[nb];
+        if (tid == 0) {
+            A.loc_off[b] = obase;
+            A.loc_cnt[b] = S.base[nb];
+        }
+        __syncthreads();
     }
 }
 
 // per point: candidate count of the rectangle -> list capacity
-__global__ void reg_capacity(RegionalArgs A, int* capn)
+__global__ void reg_capacity(RegionalArgs A, int* capn, int* maxcand)
 {
     const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long npt = (long)A.nx * A.ny * A.nlev;
@@ -375,6 +488,7 @@ __global__ void reg_capacity(RegionalArgs A, int* capn)
         const int beg = (imin > 1) ? A.cellcum[rowbase + imin - 2] : (rowbase > 0 ? A.cellcum[rowbase - 1] : 0);
         n += A.cellcum[rowbase + imax - 1] - beg;
     }
+    atomicMax(maxcand, (int)n);
     if (A.nobs_max > 0) {
         const long c = (long)A.nobs_max * A.nbkt;
         n = n < c ? n : c;
@@ -482,17 +596,29 @@ hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, 
     return hipGetLastError();
 }
 
-hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_off, hipStream_t st)
+hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_off, int* maxcand,
+                                    hipStream_t st)
 {
     const long npt = (long)A.nx * A.ny * A.nlev;
-    hipLaunchKernelGGL(reg_capacity, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st, A, capn);
+    hipLaunchKernelGGL(reg_capacity, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st, A, capn,
+                       maxcand);
     hipLaunchKernelGGL(scan_exclusive_long, dim3(1), dim3(1024), 0, st, capn, cap_off, npt);
     return hipGetLastError();
 }
 
-hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, hipStream_t st)
+// workgroup slots of the persistent selection (LDS allows four per CU)
+int lekf_reg_select_slots()
+{
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    return 4 * (ncu > 0 ? ncu : 256);
+}
+
+hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st)
 {
     if (npt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(reg_select, dim3((unsigned)npt), dim3(64), 0, st, A);
+    const long g = npt < nslots ? npt : nslots;
+    hipLaunchKernelGGL(reg_select, dim3((unsigned)g), dim3(kSelT), 0, st, A, npt);
     return hipGetLastError();
 }

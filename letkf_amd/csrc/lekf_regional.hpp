@@ -38,11 +38,21 @@ struct RegionalArgs {
     int* loc_cnt;
     int* loc_idx;
     double* loc_rho;
+    // per-workgroup scratch of the capped selection: the point's accepted
+    // candidates in scan order (key = rdiag bits, obs | bucket << 30, rho)
+    unsigned long long* s_key;
+    int* s_ob;
+    double* s_rho;
+    long slot_cap;                          // entries per workgroup slot
 };
 
 hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
 hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
                                    hipStream_t st);
-hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_off,
+// per point list capacity (rectangle count, capped) and its exclusive scan;
+// maxcand <- the largest uncapped rectangle count (zeroed before)
+hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_off, int* maxcand,
                                     hipStream_t st);
-hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, hipStream_t st);
+// persistent selection over npt points with nslots workgroups (scratch slots)
+hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st);
+int lekf_reg_select_slots();
