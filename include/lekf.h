@@ -210,6 +210,13 @@ int lekf_das_solve(lekf_handle *h);
 int lekf_set_stream(lekf_handle *h, void *stream);
 /* batch the local solves of several handles into shared launches */
 int lekf_das_run_many(lekf_handle **hs, int n);
+/* the same batching, enqueued on one caller stream (hipStream_t) without
+ * waiting: every handle's selection and the shared solve launches go to
+ * stream; the handles keep running on it afterwards (lekf_set_stream moves
+ * them).  Completion: record an event on stream (lekf_event_record) and wait
+ * or poll (lekf_event_query).  Calls that overlap must use distinct streams
+ * and distinct handles. */
+int lekf_das_run_batch(lekf_handle **hs, int n, void *stream);
 int lekf_sync(lekf_handle *h);
 int lekf_get_analysis(lekf_handle *h, double *anal3d, double *anal2d);
 int lekf_get_infl(lekf_handle *h, double *infl3d, double *infl2d);
@@ -249,9 +256,14 @@ void *lekf_event_create(void);
 int lekf_event_destroy(void *ev);
 int lekf_event_record(void *ev, void *stream);
 int lekf_event_elapsed_ms(void *ev0, void *ev1, float *ms);
+/* 1 if all work before the event has completed, 0 if not, negative on error */
+int lekf_event_query(void *ev);
+int lekf_event_sync(void *ev);
 
 /* library info */
 int lekf_device_count(void);
+/* free and total device memory of the current GPU, bytes */
+int lekf_mem_info(size_t *free_bytes, size_t *total_bytes);
 const char *lekf_version(void);
 const char *lekf_last_error(void);
 

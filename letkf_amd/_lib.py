@@ -182,6 +182,14 @@ def lib():
     L.lekf_event_record.argtypes = [c_void_p, c_void_p]
     L.lekf_event_elapsed_ms.restype = c_int
     L.lekf_event_elapsed_ms.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
+    L.lekf_event_query.restype = c_int
+    L.lekf_event_query.argtypes = [c_void_p]
+    L.lekf_event_sync.restype = c_int
+    L.lekf_event_sync.argtypes = [c_void_p]
+    L.lekf_mem_info.restype = c_int
+    L.lekf_mem_info.argtypes = [POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t)]
+    L.lekf_das_run_batch.restype = c_int
+    L.lekf_das_run_batch.argtypes = [POINTER(c_void_p), c_int, c_void_p]
     _lib = L
     return L
 
@@ -197,6 +205,7 @@ EXPORTED = [
     "lekf_device_count", "lekf_version", "lekf_last_error", "lekf_dev_malloc", "lekf_dev_free",
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
+    "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
 ]
 
 

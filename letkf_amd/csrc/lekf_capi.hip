@@ -298,6 +298,21 @@ int lekf_event_elapsed_ms(void* ev0, void* ev1, float* ms)
     return LEKF_OK;
 }
 
+int lekf_event_query(void* ev)
+{
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    set_err(e, "hipEventQuery");
+    return -LEKF_ERR_HIP;
+}
+
+int lekf_event_sync(void* ev)
+{
+    HIPCHK(hipEventSynchronize((hipEvent_t)ev));
+    return LEKF_OK;
+}
+
 const char* lekf_last_error(void) { return g_errmsg; }
 
 int lekf_device_count(void)
@@ -305,6 +320,15 @@ int lekf_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int lekf_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return LEKF_OK;
 }
 
 int lekf_core_f64(int ne, int nobs, int nobsl, const double* hdxb, const double* rdiag,
@@ -1252,38 +1276,38 @@ int lekf_set_stream(lekf_handle* h, void* stream)
     return LEKF_OK;
 }
 
-int lekf_das_run_many(lekf_handle** hs, int n)
+namespace {
+struct BatchBufs {
+    DevBuf jobs[3], ws;
+};
+std::mutex g_batch_mu;
+std::map<hipStream_t, BatchBufs> g_batch_bufs;  // per stream: batches on other streams overlap
+
+// Enqueue the selection of every handle and the grouped solves on st.
+int run_batch_on(lekf_handle** hs, int n, hipStream_t st)
 {
-    if (!hs || n <= 0) return LEKF_ERR_ARG;
-    struct Bufs {
-        DevBuf jobs[3], ws;
-    };
-    static std::mutex mu;
-    static std::map<hipStream_t, Bufs> bufs;  // per leading stream: calls on other streams overlap
-    std::lock_guard<std::mutex> lk(mu);
-    hipStream_t st = hs[0]->st;
-    Bufs& B = bufs[st];
+    for (int i = 0; i < n; ++i)
+        if (!hs[i] || !hs[i]->have_ens || !hs[i]->have_obs) return LEKF_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    BatchBufs& B = g_batch_bufs[st];
     std::vector<lekf_handle*> batched;
     for (int i = 0; i < n; ++i) {
-        if (!hs[i] || !hs[i]->have_ens || !hs[i]->have_obs) return LEKF_ERR_ARG;
-        if (hs[i]->kind == 2 && hs[i]->chunk_pts.size() != 2) {
-            // lists built chunk by chunk: runs on its own (select and solve per chunk)
-            const int rc = lekf_das_run(hs[i]);
+        lekf_handle* h = hs[i];
+        if (h->st != st) {
+            HIPCHK(hipStreamSynchronize(h->st));
+            if (h->own_stream) HIPCHK(hipStreamDestroy(h->st));
+            h->own_stream = false;
+            h->st = st;
+        }
+        if (h->kind == 2 && h->chunk_pts.size() != 2) {
+            // lists built chunk by chunk: select and solve per chunk, on its own
+            const int rc = lekf_das_run(h);
             if (rc) return rc;
             continue;
         }
-        batched.push_back(hs[i]);
-        if (hs[i]->st != st) {
-            // selection runs on the leading stream so the shared solve sees it
-            hipStream_t keep = hs[i]->st;
-            hs[i]->st = st;
-            int rc = run_select(hs[i]);
-            hs[i]->st = keep;
-            if (rc) return rc;
-        } else {
-            int rc = run_select(hs[i]);
-            if (rc) return rc;
-        }
+        batched.push_back(h);
+        const int rc = run_select(h);
+        if (rc) return rc;
     }
     // one grouped launch set per arithmetic mode
     for (int mode = 0; mode < 3; ++mode) {
@@ -1294,8 +1318,41 @@ int lekf_das_run_many(lekf_handle** hs, int n)
         const int rc = launch_grouped(J, mode, B.jobs[mode], B.ws, st);
         if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(st));
-    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(hs[i]->st));
+    return LEKF_OK;
+}
+}  // namespace
+
+int lekf_das_run_batch(lekf_handle** hs, int n, void* stream)
+{
+    if (!hs || n <= 0) return LEKF_ERR_ARG;
+    return run_batch_on(hs, n, (hipStream_t)stream);
+}
+
+int lekf_das_run_many(lekf_handle** hs, int n)
+{
+    if (!hs || n <= 0 || !hs[0]) return LEKF_ERR_ARG;
+    // selection and solves on the leading handle's stream; the others keep theirs
+    hipStream_t st = hs[0]->st;
+    std::vector<hipStream_t> keep(n);
+    std::vector<bool> own(n);
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i]) return LEKF_ERR_ARG;
+        keep[i] = hs[i]->st;
+        own[i] = hs[i]->own_stream;
+        if (i > 0 && hs[i]->st != st) {
+            HIPCHK(hipStreamSynchronize(hs[i]->st));
+            hs[i]->own_stream = false;  // borrowed for the batch, restored below
+            hs[i]->st = st;
+        }
+    }
+    const int rc = run_batch_on(hs, n, st);
+    const hipError_t e = hipStreamSynchronize(st);
+    for (int i = 0; i < n; ++i) {
+        hs[i]->st = keep[i];
+        hs[i]->own_stream = own[i];
+    }
+    if (rc) return rc;
+    HIPCHK(e);
     return LEKF_OK;
 }
 

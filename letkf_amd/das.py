@@ -224,6 +224,13 @@ class RegionalLetkf(_Handle):
 
 
 def run_many(handles):
-    """Batch the local solves of several handles into shared launches."""
+    """Batch the local solves of several handles into shared launches (waits)."""
     arr = (ctypes.c_void_p * len(handles))(*[h.h for h in handles])
     check(_lib.lib().lekf_das_run_many(arr, len(handles)), "das_run_many")
+
+
+def run_batch(handles, stream):
+    """Enqueue the batched cycle of several handles on one stream without waiting;
+    the handles run on that stream afterwards."""
+    arr = (ctypes.c_void_p * len(handles))(*[h.h for h in handles])
+    check(_lib.lib().lekf_das_run_batch(arr, len(handles), stream), "das_run_batch")

@@ -105,6 +105,16 @@ class Event:
         _lib.check(_lib.lib().lekf_event_elapsed_ms(a.ptr, b.ptr, ctypes.byref(ms)), "elapsed")
         return float(ms.value)
 
+    def query(self):
+        """True once all work recorded before the event has completed."""
+        r = _lib.lib().lekf_event_query(self.ptr)
+        if r < 0:
+            raise _lib.LetkfError(-r, "event_query")
+        return r == 1
+
+    def sync(self):
+        _lib.check(_lib.lib().lekf_event_sync(self.ptr), "event_sync")
+
     def __del__(self):
         try:
             if self.ptr:
@@ -116,3 +126,11 @@ class Event:
 
 def sync(stream=None):
     _lib.check(_lib.lib().lekf_stream_sync(stream), "stream_sync")
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f = ctypes.c_size_t(0)
+    t = ctypes.c_size_t(0)
+    _lib.check(_lib.lib().lekf_mem_info(ctypes.byref(f), ctypes.byref(t)), "mem_info")
+    return int(f.value), int(t.value)
