@@ -153,6 +153,148 @@ def cpu_baseline(p, ref, max_threads=16):
                        f"(oracle restatement) not timed")}
 
 
+def radar_bench(sub, steps, precs=("f64", "f32", "bf16x3"), cpu_sample=0):
+    """Regional radar config on a grid of sub = (nx, ny, nlev) columns:
+    k=50, u,v,w,T,qv,qr, ~60% of columns observed at every level, sigma_h = sigma_v = 2 km,
+    nobs_max = 1000 per bucket.  Each form's analysis is compared with the FP64 GPU
+    analysis (itself held to 1e-10 of the reference by the tests)."""
+    from letkf_amd import das, synth
+    from letkf_amd.device import Event
+    t0 = time.perf_counter()
+    p = synth.radar(sub=sub)
+    gen_s = time.perf_counter() - t0
+    npt = p["nx"] * p["ny"] * p["nlev"]
+    spread = p["gues3d"].std(axis=2, ddof=1)
+    res, ref = {}, None
+    for prec in precs:
+        h = das.RegionalLetkf(nx=p["nx"], ny=p["ny"], zlev=p["zlev"], dx=p["dx"], dy=p["dy"],
+                              ne=p["ne"], nv3d=p["nv3d"], sigma_h=p["sigma_h"],
+                              sigma_v=p["sigma_v"], nobs_max=p["nobs_max"], infl_value=1.0,
+                              prec=prec)
+        h.set_ensemble(p["gues3d"])
+        t1 = time.perf_counter()
+        h.set_obs(p["ri"], p["rj"], p["z"], p["err"], p["dep"], p["bucket"], p["hdxf"])
+        h.sync()
+        obs_ms = (time.perf_counter() - t1) * 1e3
+        st = h.stream()
+        h.run()
+        h.sync()
+        lat = []
+        e0, e1 = Event(), Event()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            e0.record(st)
+            h.run()
+            e1.record(st)
+            h.sync()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        a3, _ = h.analysis()
+        nl = h.nobsl()
+        h.close()
+        r = {"local_analyses_per_s": npt / (np.mean(lat) / 1e3), "cycle_ms": float(np.mean(lat)),
+             "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+             "obs_setup_ms": obs_ms, "mean_nobsl": float(nl.mean())}
+        if ref is None:
+            ref = a3
+        else:
+            r["err_vs_f64_over_bg_spread"] = float((np.abs(a3 - ref).max(axis=2) / spread).max())
+        res[prec] = r
+        print(f"[bench] radar {sub} {prec}: {r['cycle_ms']:.0f} ms/cycle", file=sys.stderr,
+              flush=True)
+    if "f64" in res:
+        for prec in res:
+            res[prec]["speedup_vs_f64"] = res[prec]["local_analyses_per_s"] / res["f64"]["local_analyses_per_s"]
+    out = {"grid": list(sub), "k": p["ne"], "local_analyses_per_cycle": npt,
+           "nobs": int(p["ri"].size), "nobs_max_per_bucket": p["nobs_max"], "gen_s": gen_s,
+           "forms": res}
+    if cpu_sample:
+        out["cpu_baseline"] = radar_cpu_sample(p, cpu_sample)
+    return out
+
+
+def radar_cpu_sample(p, npts):
+    """Reference letkf_core + apply on a seeded sample of radar points (their capped
+    local lists from the oracle's SCALE selection restatement), OpenMP over points."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    nx, ny, nlev, nij = p["nx"], p["ny"], p["nlev"], p["nx"] * p["ny"]
+    perm, grd = oracle.obs_bins_regional(p["ri"], p["rj"], nx, ny)
+    so = {k: np.asarray(p[k])[perm] for k in ("ri", "rj", "z", "err", "bucket")}
+    rng = np.random.default_rng(7)
+    pts = np.sort(rng.choice(nij * nlev, npts, replace=False))
+    idx_all, rho_all, off = [], [], [0]
+    for b in pts:
+        lev, ij = divmod(int(b), nij)
+        idx, rl, _ = oracle.obs_local_regional(float(ij % nx + 1), float(ij // nx + 1),
+                                               p["zlev"][lev], nx, ny, grd, so, p["dx"], p["dy"],
+                                               p["sigma_h"], p["sigma_v"], p["nobs_max"], 2)
+        idx_all.append(idx)
+        rho_all.append(rl)
+        off.append(off[-1] + idx.size)
+    g3 = p["gues3d"]
+    xb = np.ascontiguousarray(g3.transpose(1, 0, 3, 2)).reshape(nlev * nij, p["nv3d"], p["ne"])[pts]
+    hd = np.asarray(p["hdxf"])[perm]
+    t0 = time.perf_counter()
+    oracle.ref_solve_apply(np.array(off), np.concatenate(idx_all), np.concatenate(rho_all), hd,
+                           so["err"] ** 2, np.asarray(p["dep"])[perm], xb, np.ones(npts))
+    t = time.perf_counter() - t0
+    return {"value": npts / t, "unit": "local analyses/s", "cores": oracle.ref_threads(),
+            "kind": "reference",
+            "sample": (f"{npts} seeded radar points of the {nx}x{ny}x{nlev} grid, their capped "
+                       f"local lists (mean {np.mean(np.diff(off)):.0f} obs): reference Fortran "
+                       f"letkf_core + EISPACK + reference DGEMM + apply, {t:.2f} s; selection "
+                       f"not timed")}
+
+
+def serve_bench(duration_s=20.0, rate=2.0, precs=("f64", "bf16x3"), seed=5):
+    """Served analysis stream: Poisson arrivals of LETKF cycles of mixed sizes,
+    each with a 30-s deadline.  Mix (by arrival): 50% Lorenz-96 (N in 40..960,
+    k in 20..80), 35% SPEEDY-sized (k=20), 15% regional radar on 64x64x60 columns (k=50).
+    Every job is a fresh instance with its own background and obs."""
+    from letkf_amd import serve, synth
+    rng = np.random.default_rng(seed)
+    tmpl = {"l96_40_20": serve.l96_job, "l96_240_40": serve.l96_job, "l96_960_80": serve.l96_job,
+            "speedy_20": serve.global_job, "radar_64_50": serve.regional_job}
+    probs = {"l96_40_20": synth.l96(nx=40, ne=20), "l96_240_40": synth.l96(nx=240, ne=40, seed=2),
+             "l96_960_80": synth.l96(nx=960, ne=80, seed=3), "speedy_20": synth.speedy(),
+             "radar_64_50": synth.radar(sub=(64, 64, 60), seed=4)}
+    names = list(tmpl)
+    weights = np.array([0.2, 0.15, 0.15, 0.35, 0.15])
+    arr = serve.poisson_arrivals(rate, duration_s, seed)
+    kinds = rng.choice(len(names), size=arr.size, p=weights / weights.sum())
+    out = {}
+    for prec in precs:
+        srv = serve.Server(streams=2)
+        # warm-up: one job of each kind (kernel loading, first allocations)
+        for n in names:
+            srv.submit(tmpl[n](f"warm-{n}", probs[n], prec)).result(timeout=300)
+        srv.completed.clear()
+        t0 = time.perf_counter()
+        futs = []
+        for i, (ta, kk) in enumerate(zip(arr, kinds)):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            n = names[kk]
+            futs.append(srv.submit(tmpl[n](f"{n}-{i}", probs[n], prec)))
+        srv.drain(timeout=600)
+        wall = time.perf_counter() - t0
+        res = [f.result() for f in futs]
+        stt = srv.stats()
+        srv.close()
+        lat = np.array([r.latency_s for r in res])
+        out[prec] = {"jobs": len(res), "wall_s": wall, "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max()),
+                     "deadline_met": float(np.mean(lat <= 30.0)),
+                     "local_analyses_per_s": stt["local_analyses"] / wall,
+                     "max_jobs_per_batch": int(max(r.batch_jobs for r in res))}
+        print(f"[bench] serve {prec}: {len(res)} jobs p50 {out[prec]['p50_s']:.3f}s "
+              f"p99 {out[prec]['p99_s']:.3f}s", file=sys.stderr, flush=True)
+    return {"arrivals_per_s": rate, "duration_s": duration_s, "mix": dict(zip(names, weights.tolist())),
+            "forms": out}
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
     give the scale factor between the counters and HBM bytes."""
@@ -177,6 +319,11 @@ def main():
                     help="only a 1 GiB device-to-device copy, 5 times (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the radar and served-stream measurements")
+    ap.add_argument("--radar-full", action="store_true",
+                    help="radar on the full 256x256x60 grid instead of 128x128x60")
+    ap.add_argument("--serve-s", type=float, default=20.0, help="served-stream duration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the LETKF local analyses of a job run on a single "
@@ -269,6 +416,10 @@ def main():
               flush=True)
         out["cpu_baseline"] = cb
     h.close()
+    if not args.no_extra:
+        sub = (256, 256, 60) if args.radar_full else (128, 128, 60)
+        out["radar"] = radar_bench(sub, steps=3, cpu_sample=0 if args.no_cpu else 1500)
+        out["serve"] = serve_bench(duration_s=args.serve_s)
     print(json.dumps(out), flush=True)
 
 
