@@ -190,6 +190,20 @@ int lekf_set_infl(lekf_handle *h, const double *infl3d, const double *infl2d);
 int lekf_set_obs_global(lekf_handle *h, int nobs, const double *elm, const double *lon,
                         const double *lat, const double *lev, const double *dat,
                         const double *err, const int *qc, const double *hxf);
+/* The same with H(x) computed on the GPU from the resident background by the
+ * reference obs operator (obsope + Trans_XtoY, speedy/letkf/obsope.f90:35-111,
+ * speedy/common/common_obs_speedy.f90:30-282): u, v, t, q (3-D variables
+ * 0, 1, 2, iq) and ps (2-D variable ips); other elements are rejected. */
+int lekf_set_obs_global_hx(lekf_handle *h, int nobs, const double *elm, const double *lon,
+                           const double *lat, const double *lev, const double *dat,
+                           const double *err);
+/* The obs operator alone (obsope, speedy/letkf/obsope.f90:54-105): H(x) of
+ * every member hxf(nobs,ne) column-major (0 where rejected), qc(nobs) = 1 when
+ * every member accepts the obs, and optionally the grid coordinates ri, rj
+ * (nobs) and rk(nobs,ne) of phys2ijk (common_obs_speedy.f90:137-211). */
+int lekf_obsope_global(lekf_handle *h, int nobs, const double *elm, const double *lon,
+                       const double *lat, const double *lev, const double *dat, double *hxf,
+                       int *qc, double *ri, double *rj, double *rk);
 /* Regional obs after the obs operator (SCALE's obsda): grid-unit position
  * (ri, rj), height z (m), error, departure dep = y - mean H(x), bucket
  * (the (element, obs type) index, 0..3), qc (NULL: all good) and the obs-space

@@ -126,6 +126,10 @@ def lib():
     L.lekf_set_infl.argtypes = [c_void_p, _dp, _dp]
     L.lekf_set_obs_global.restype = c_int
     L.lekf_set_obs_global.argtypes = [c_void_p, c_int] + [_dp] * 6 + [_ip, _dp]
+    L.lekf_set_obs_global_hx.restype = c_int
+    L.lekf_set_obs_global_hx.argtypes = [c_void_p, c_int] + [_dp] * 6
+    L.lekf_obsope_global.restype = c_int
+    L.lekf_obsope_global.argtypes = [c_void_p, c_int] + [_dp] * 6 + [_ip, _dp, _dp, _dp]
     L.lekf_set_obs_l96.restype = c_int
     L.lekf_set_obs_l96.argtypes = [c_void_p, c_int, _ip, _dp, _dp, _dp]
     L.lekf_das_run.restype = c_int
@@ -206,6 +210,7 @@ EXPORTED = [
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
+    "lekf_set_obs_global_hx", "lekf_obsope_global",
 ]
 
 

@@ -15,6 +15,7 @@
 #include "lekf_cull.hpp"
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
+#include "lekf_obsope.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
 
@@ -896,6 +897,63 @@ int lekf_set_infl(lekf_handle* h, const double* infl3d, const double* infl2d)
     return LEKF_OK;
 }
 
+int upload_obs_records(lekf_handle* h, int nobs, const double* elm, const double* lon,
+                       const double* lat, const double* lev, const double* dat, const double* err)
+{
+    hipStream_t st = h->st;
+    int rc;
+    if ((rc = upload(h->r_elm, elm, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lon, lon, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lat, lat, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lev, lev, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_dat, dat, sizeof(double) * nobs, st)))
+        return rc;
+    if (err && (rc = upload(h->r_err, err, sizeof(double) * nobs, st))) return rc;
+    return 0;
+}
+
+// H(x) of the resident background into r_hxf (row = obs, k members) and the
+// member-combined QC flag into r_qc
+int run_obsope_global(lekf_handle* h, int nobs, double* ri, double* rj, double* rk)
+{
+    hipStream_t st = h->st;
+    const size_t n = (size_t)(nobs > 0 ? nobs : 1);
+    HIPCHK(h->r_hxf.ensure(sizeof(double) * n * h->ne));
+    HIPCHK(h->r_qc.ensure(sizeof(int) * n));
+    ObsOpArgs O;
+    memset(&O, 0, sizeof(O));
+    O.nobs = nobs;
+    O.ne = h->ne;
+    O.nlon = h->g.nlon;
+    O.nlat = h->g.nlat;
+    O.nlev = h->nlev;
+    O.nv3d = h->nv3d;
+    O.nv2d = h->nv2d;
+    O.iu = 0;
+    O.iv = 1;
+    O.it = 2;
+    O.iq = h->g.iq >= 0 ? h->g.iq : 3;
+    O.ips = h->g.ips;
+    O.lat = h->d_lat.as<double>();
+    O.sig = h->d_sig.as<double>();
+    O.x3 = h->x3b.as<double>();
+    O.x2 = h->x2b.as<double>();
+    O.elm = h->r_elm.as<double>();
+    O.olon = h->r_lon.as<double>();
+    O.olat = h->r_lat.as<double>();
+    O.olev = h->r_lev.as<double>();
+    O.odat = h->r_dat.as<double>();
+    O.hxf = h->r_hxf.as<double>();
+    O.qc = h->r_qc.as<int>();
+    O.ri = ri;
+    O.rj = rj;
+    O.rk = rk;
+    HIPCHK(lekf_obsope_global_launch(O, st));
+    return 0;
+}
+
+int set_obs_global_dev(lekf_handle* h, int nobs, bool have_qc, long h_so, long h_sm);
+
 int lekf_set_obs_global(lekf_handle* h, int nobs, const double* elm, const double* lon,
                         const double* lat, const double* lev, const double* dat,
                         const double* err, const int* qc, const double* hxf)
@@ -903,18 +961,71 @@ int lekf_set_obs_global(lekf_handle* h, int nobs, const double* elm, const doubl
     if (!h || h->kind != 0 || nobs < 0) return LEKF_ERR_ARG;
     if (nobs > 0 && (!elm || !lon || !lat || !lev || !dat || !err || !hxf)) return LEKF_ERR_ARG;
     hipStream_t st = h->st;
+    int rc = upload_obs_records(h, nobs, elm, lon, lat, lev, dat, err);
+    if (!rc) rc = upload(h->r_hxf, hxf, sizeof(double) * nobs * (size_t)h->ne, st);
+    if (rc) return rc;
+    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
+    return set_obs_global_dev(h, nobs, qc != nullptr, 1, nobs);
+}
+
+int lekf_set_obs_global_hx(lekf_handle* h, int nobs, const double* elm, const double* lon,
+                           const double* lat, const double* lev, const double* dat,
+                           const double* err)
+{
+    if (!h || h->kind != 0 || nobs < 0 || !h->have_ens) return LEKF_ERR_ARG;
+    if (nobs > 0 && (!elm || !lon || !lat || !lev || !dat || !err)) return LEKF_ERR_ARG;
+    int rc = upload_obs_records(h, nobs, elm, lon, lat, lev, dat, err);
+    if (!rc) rc = run_obsope_global(h, nobs, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    return set_obs_global_dev(h, nobs, true, h->ne, 1);
+}
+
+int lekf_obsope_global(lekf_handle* h, int nobs, const double* elm, const double* lon,
+                       const double* lat, const double* lev, const double* dat, double* hxf,
+                       int* qc, double* ri, double* rj, double* rk)
+{
+    if (!h || h->kind != 0 || nobs < 0 || !h->have_ens || !hxf || !qc) return LEKF_ERR_ARG;
+    if (nobs == 0) return LEKF_OK;
+    if (!elm || !lon || !lat || !lev || !dat) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    int rc = upload_obs_records(h, nobs, elm, lon, lat, lev, dat, nullptr);
+    if (rc) return rc;
+    DevBuf dri, drj, drk;
+    HIPCHK(dri.ensure(sizeof(double) * nobs));
+    HIPCHK(drj.ensure(sizeof(double) * nobs));
+    HIPCHK(drk.ensure(sizeof(double) * nobs * (size_t)h->ne));
+    rc = run_obsope_global(h, nobs, dri.as<double>(), drj.as<double>(), drk.as<double>());
+    if (rc) return rc;
+    const size_t ne = (size_t)h->ne;
+    std::vector<double> t((size_t)nobs * ne);
+    HIPCHK(hipMemcpyAsync(t.data(), h->r_hxf.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipMemcpyAsync(qc, h->r_qc.p, sizeof(int) * nobs, hipMemcpyDeviceToHost, st));
+    if (ri) HIPCHK(hipMemcpyAsync(ri, dri.p, sizeof(double) * nobs, hipMemcpyDeviceToHost, st));
+    if (rj) HIPCHK(hipMemcpyAsync(rj, drj.p, sizeof(double) * nobs, hipMemcpyDeviceToHost, st));
+    std::vector<double> tk(rk ? t.size() : 0);
+    if (rk) HIPCHK(hipMemcpyAsync(tk.data(), drk.p, sizeof(double) * tk.size(),
+                                  hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // device rows (obs, members) -> reference column-major (nobs, ne)
+    for (size_t n = 0; n < (size_t)nobs; ++n)
+        for (size_t m = 0; m < ne; ++m) {
+            hxf[n + m * nobs] = t[n * ne + m];
+            if (rk) rk[n + m * nobs] = tk[n * ne + m];
+        }
+    dri.release();
+    drj.release();
+    drk.release();
+    return LEKF_OK;
+}
+
+// departures, QC and binning of obs records and H(x) already on the device
+// (r_elm .. r_err, r_hxf with member stride h_sm and obs stride h_so, r_qc)
+int set_obs_global_dev(lekf_handle* h, int nobs, bool have_qc, long h_so, long h_sm)
+{
+    hipStream_t st = h->st;
     const int ne = h->ne;
     const size_t n = (size_t)(nobs > 0 ? nobs : 1);
-    int rc;
-    if ((rc = upload(h->r_elm, elm, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_lon, lon, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_lat, lat, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_lev, lev, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_dat, dat, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_err, err, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_hxf, hxf, sizeof(double) * nobs * (size_t)ne, st)))
-        return rc;
-    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
     long n2 = 2048;
     while (n2 < nobs) n2 <<= 1;
     const int ncell = h->g.nlon * h->g.nlat;
@@ -951,10 +1062,10 @@ int lekf_set_obs_global(lekf_handle* h, int nobs, const double* elm, const doubl
     A.olev = h->r_lev.as<double>();
     A.dat = h->r_dat.as<double>();
     A.err = h->r_err.as<double>();
-    A.qc = qc ? h->r_qc.as<int>() : nullptr;
+    A.qc = have_qc ? h->r_qc.as<int>() : nullptr;
     A.hxf = h->r_hxf.as<double>();
-    A.h_so = 1;
-    A.h_sm = nobs;
+    A.h_so = h_so;
+    A.h_sm = h_sm;
     A.gross_error = h->g.gross_error;
     A.tmp_mean = h->tmp_mean.as<double>();
     A.tmp_dep = h->tmp_dep.as<double>();

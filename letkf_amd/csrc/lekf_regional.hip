@@ -161,6 +161,211 @@ __device__ __forceinline__ int block_append(SelLds& S, bool flag, int slot, int 
     return pos;
 }
 
+// Keep, per bucket, the cap entries with the smallest (key, list position) of
+// a list in scan order (key = rdiag bits, ob = obs | bucket << 30, rho) and
+// write them bucket by bucket in list order to out_idx / out_rho; returns the
+// count.  Each over-full bucket's threshold pair is found by digit histograms
+// over the list.  Uniform call by all kSelT threads.
+__device__ int cap_select(SelLds& S, int tid, const unsigned long long* skey, const int* sob,
+                          const double* srho, int nacc, int nb, int cap, int* out_idx,
+                          double* out_rho)
+{
+    for (int t = tid; t < LEKF_REG_NB; t += kSelT) {
+        S.cnt[t] = 0;
+        S.lcnt[t] = 0;
+        S.mode[t] = 0;
+        S.run[t] = 0;
+    }
+    for (int t = tid; t < LEKF_REG_NB * 1024; t += kSelT) (&S.hist[0][0])[t] = 0;
+    __syncthreads();
+    for (int i = tid; i < nacc; i += kSelT) {
+        const int t = sob[i] >> 30;
+        atomicAdd(&S.cnt[t], 1);
+        atomicAdd(&S.hist[t][digit(skey[i], 53)], 1);
+    }
+    __syncthreads();
+    // boundary digit of every bucket over its cap
+    if (tid < nb) {
+        const int t = tid;
+        if (S.cnt[t] > cap) {
+            int need = cap, d = 0, acc = 0;
+            for (; d < 1024; ++d) {
+                if (acc + S.hist[t][d] >= need) break;
+                acc += S.hist[t][d];
+            }
+            S.need[t] = need - acc;
+            S.pref[t] = (unsigned long long)d << 53;
+            S.bincnt[t] = S.hist[t][d];
+            S.shift[t] = 43;
+            S.mode[t] = S.bincnt[t] <= LEKF_REG_LIST ? 2 : 1;
+        }
+    }
+    __syncthreads();
+    // refine while a boundary bin is too large to sort
+    for (int pass = 0; pass < 8; ++pass) {
+        if (tid == 0) {
+            int a = 0;
+            for (int t = 0; t < nb; ++t) a |= (S.mode[t] == 1);
+            S.any = a;
+        }
+        __syncthreads();
+        if (!S.any) break;
+        for (int t = 0; t < nb; ++t)
+            if (S.mode[t] == 1)
+                for (int u = tid; u < 1024; u += kSelT) S.hist[t][u] = 0;
+        __syncthreads();
+        for (int i = tid; i < nacc; i += kSelT) {
+            const unsigned long long bits = skey[i];
+            const int t = sob[i] >> 30;
+            if (S.mode[t] == 1 && prefix_match(bits, S.pref[t], S.shift[t]))
+                atomicAdd(&S.hist[t][digit(bits, S.shift[t])], 1);
+        }
+        __syncthreads();
+        if (tid < nb && S.mode[tid] == 1) {
+            const int t = tid;
+            const int sh = S.shift[t];
+            int need = S.need[t], d = 0, acc = 0;
+            for (; d < 1024; ++d) {
+                if (acc + S.hist[t][d] >= need) break;
+                acc += S.hist[t][d];
+            }
+            S.need[t] = need - acc;
+            S.pref[t] |= (unsigned long long)d << sh;
+            S.bincnt[t] = S.hist[t][d];
+            const int ns = next_shift(sh);
+            S.shift[t] = ns;
+            if (S.bincnt[t] <= LEKF_REG_LIST)
+                S.mode[t] = 2;
+            else if (ns < 0)
+                S.mode[t] = 3;  // every key bit fixed: equal keys, break ties by position
+        }
+        __syncthreads();
+    }
+    // collect the boundary bins; resolve exact ties in list (scan) order
+    for (int t = tid; t < LEKF_REG_NB; t += kSelT) S.run[t] = 0;
+    __syncthreads();
+    for (int i0 = 0; i0 < nacc; i0 += kSelT) {
+        const int i = i0 + tid;
+        bool in = i < nacc;
+        const unsigned long long bits = in ? skey[i] : 0ull;
+        const int t = in ? (sob[i] >> 30) : 0;
+        if (in && S.mode[t] == 2 && prefix_match(bits, S.pref[t], S.shift[t])) {
+            const int pos = atomicAdd(&S.lcnt[t], 1);
+            if (pos < LEKF_REG_LIST) {
+                S.lbits[t][pos] = bits;
+                S.lpos[t][pos] = i;
+            }
+        }
+        for (int u = 0; u < nb; ++u) {
+            if (S.mode[u] != 3) continue;
+            const bool tie = in && t == u && bits == S.pref[u];
+            const int rank = block_append(S, tie, u, tid);
+            if (tie && rank == S.need[u] - 1) {
+                S.thr_bits[u] = bits;
+                S.thr_pos[u] = i;
+            }
+        }
+    }
+    __syncthreads();
+    // sort each collected bin by (bits, position); the need-th is the threshold
+    for (int t = 0; t < nb; ++t) {
+        if (S.mode[t] != 2) continue;
+        const int n = S.lcnt[t] < LEKF_REG_LIST ? S.lcnt[t] : LEKF_REG_LIST;
+        for (int ph = 0; ph < n; ++ph) {
+            const int i = 2 * tid + (ph & 1);
+            if (i + 1 < n) {
+                const unsigned long long x = S.lbits[t][i], y = S.lbits[t][i + 1];
+                const int px = S.lpos[t][i], py = S.lpos[t][i + 1];
+                if (x > y || (x == y && px > py)) {
+                    S.lbits[t][i] = y;
+                    S.lbits[t][i + 1] = x;
+                    S.lpos[t][i] = py;
+                    S.lpos[t][i + 1] = px;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            S.thr_bits[t] = S.lbits[t][S.need[t] - 1];
+            S.thr_pos[t] = S.lpos[t][S.need[t] - 1];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int acc = 0;
+        for (int t = 0; t < nb; ++t) {
+            S.base[t] = acc;
+            acc += S.cnt[t] < cap ? S.cnt[t] : cap;
+            S.run[t] = 0;
+        }
+        S.base[nb] = acc;
+    }
+    __syncthreads();
+    // emit bucket by bucket in scan order (list position i = i0 + u * kSelT + tid)
+    {
+        const int lane = tid & 63, wv = tid >> 6;
+        int runb[LEKF_REG_NB];
+#pragma unroll
+        for (int t = 0; t < LEKF_REG_NB; ++t) runb[t] = 0;
+        for (int i0 = 0; i0 < nacc; i0 += kSelT * kSelU) {
+            unsigned long long bits[kSelU];
+            int ob[kSelU];
+            bool keep[kSelU];
+#pragma unroll
+            for (int u = 0; u < kSelU; ++u) {
+                const int i = i0 + u * kSelT + tid;
+                const bool in = i < nacc;
+                bits[u] = in ? skey[i] : 0ull;
+                ob[u] = in ? sob[i] : 0;
+                const int t = ob[u] >> 30;
+                keep[u] = in && (S.mode[t] == 0 || bits[u] < S.thr_bits[t] ||
+                                 (bits[u] == S.thr_bits[t] && i <= S.thr_pos[t]));
+            }
+            unsigned long long m[kSelU][LEKF_REG_NB];
+#pragma unroll
+            for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                for (int t = 0; t < LEKF_REG_NB; ++t) m[u][t] = __ballot(keep[u] && (ob[u] >> 30) == t);
+            if (lane == 0)
+#pragma unroll
+                for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                    for (int t = 0; t < LEKF_REG_NB; ++t) S.ecnt[wv][u][t] = __popcll(m[u][t]);
+            __syncthreads();
+            int pos[kSelU];
+#pragma unroll
+            for (int u = 0; u < kSelU; ++u) pos[u] = -1;
+#pragma unroll
+            for (int u = 0; u < kSelU; ++u)
+#pragma unroll
+                for (int t = 0; t < LEKF_REG_NB; ++t) {
+                    int before = 0, tot = 0;
+                    for (int w = 0; w < kSelW; ++w) {
+                        const int cw = S.ecnt[w][u][t];
+                        before += w < wv ? cw : 0;
+                        tot += cw;
+                    }
+                    if (keep[u] && (ob[u] >> 30) == t)
+                        pos[u] = runb[t] + before + __popcll(m[u][t] & lane_lt(lane));
+                    runb[t] += tot;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kSelU; ++u) {
+                const int t = ob[u] >> 30;
+                if (keep[u] && pos[u] < S.base[t + 1] - S.base[t]) {
+                    const long w = S.base[t] + pos[u];
+                    out_idx[w] = ob[u] & 0x3fffffff;
+                    out_rho[w] = srho[i0 + u * kSelT + tid];
+                }
+            }
+        }
+    }
+    const int kept = S.base[nb];
+    __syncthreads();
+    return kept;
+}
+
 // Persistent selection: workgroup g handles points g, g + gridDim.x, ...
 // Each candidate of a point is evaluated once.  Without a cap the accepted
 // candidates are written in scan order.  With a cap they go to the
@@ -205,13 +410,6 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
             }
             S.seg_pre[nseg] = acc;
         }
-        for (int t = tid; t < LEKF_REG_NB; t += kSelT) {
-            S.cnt[t] = 0;
-            S.lcnt[t] = 0;
-            S.mode[t] = 0;
-            S.run[t] = 0;
-        }
-        for (int t = tid; t < LEKF_REG_NB * 1024; t += kSelT) (&S.hist[0][0])[t] = 0;
         __syncthreads();
         const int ncand = S.seg_pre[nseg];
         const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
@@ -267,8 +465,6 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
                         skey[pos[u]] = c[u].bits;
                         sob[pos[u]] = c[u].o | (c[u].b << 30);
                         srho[pos[u]] = c[u].rho;
-                        atomicAdd(&S.cnt[c[u].b], 1);
-                        atomicAdd(&S.hist[c[u].b][digit(c[u].bits, 53)], 1);
                     }
                 }
             }
@@ -282,186 +478,11 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
             __syncthreads();
             continue;
         }
-        // boundary digit of every bucket over its cap
-        if (tid < nb) {
-            const int t = tid;
-            if (S.cnt[t] > cap) {
-                int need = cap, d = 0, acc = 0;
-                for (; d < 1024; ++d) {
-                    if (acc + S.hist[t][d] >= need) break;
-                    acc += S.hist[t][d];
-                }
-                S.need[t] = need - acc;
-                S.pref[t] = (unsigned long long)d << 53;
-                S.bincnt[t] = S.hist[t][d];
-                S.shift[t] = 43;
-                S.mode[t] = S.bincnt[t] <= LEKF_REG_LIST ? 2 : 1;
-            }
-        }
-        __syncthreads();
-        // refine while a boundary bin is too large to sort
-        for (int pass = 0; pass < 8; ++pass) {
-            if (tid == 0) {
-                int a = 0;
-                for (int t = 0; t < nb; ++t) a |= (S.mode[t] == 1);
-                S.any = a;
-            }
-            __syncthreads();
-            if (!S.any) break;
-            for (int t = 0; t < nb; ++t)
-                if (S.mode[t] == 1)
-                    for (int u = tid; u < 1024; u += kSelT) S.hist[t][u] = 0;
-            __syncthreads();
-            for (int i = tid; i < nacc; i += kSelT) {
-                const unsigned long long bits = skey[i];
-                const int t = sob[i] >> 30;
-                if (S.mode[t] == 1 && prefix_match(bits, S.pref[t], S.shift[t]))
-                    atomicAdd(&S.hist[t][digit(bits, S.shift[t])], 1);
-            }
-            __syncthreads();
-            if (tid < nb && S.mode[tid] == 1) {
-                const int t = tid;
-                const int sh = S.shift[t];
-                int need = S.need[t], d = 0, acc = 0;
-                for (; d < 1024; ++d) {
-                    if (acc + S.hist[t][d] >= need) break;
-                    acc += S.hist[t][d];
-                }
-                S.need[t] = need - acc;
-                S.pref[t] |= (unsigned long long)d << sh;
-                S.bincnt[t] = S.hist[t][d];
-                const int ns = next_shift(sh);
-                S.shift[t] = ns;
-                if (S.bincnt[t] <= LEKF_REG_LIST)
-                    S.mode[t] = 2;
-                else if (ns < 0)
-                    S.mode[t] = 3;  // every key bit fixed: equal keys, break ties by position
-            }
-            __syncthreads();
-        }
-        // collect the boundary bins; resolve exact ties in list (scan) order
-        for (int t = tid; t < LEKF_REG_NB; t += kSelT) S.run[t] = 0;
-        __syncthreads();
-        for (int i0 = 0; i0 < nacc; i0 += kSelT) {
-            const int i = i0 + tid;
-            bool in = i < nacc;
-            const unsigned long long bits = in ? skey[i] : 0ull;
-            const int t = in ? (sob[i] >> 30) : 0;
-            if (in && S.mode[t] == 2 && prefix_match(bits, S.pref[t], S.shift[t])) {
-                const int pos = atomicAdd(&S.lcnt[t], 1);
-                if (pos < LEKF_REG_LIST) {
-                    S.lbits[t][pos] = bits;
-                    S.lpos[t][pos] = i;
-                }
-            }
-            for (int u = 0; u < nb; ++u) {
-                if (S.mode[u] != 3) continue;
-                const bool tie = in && t == u && bits == S.pref[u];
-                const int rank = block_append(S, tie, u, tid);
-                if (tie && rank == S.need[u] - 1) {
-                    S.thr_bits[u] = bits;
-                    S.thr_pos[u] = i;
-                }
-            }
-        }
-        __syncthreads();
-        // sort each collected bin by (bits, position); the need-th is the threshold
-        for (int t = 0; t < nb; ++t) {
-            if (S.mode[t] != 2) continue;
-            const int n = S.lcnt[t] < LEKF_REG_LIST ? S.lcnt[t] : LEKF_REG_LIST;
-            for (int ph = 0; ph < n; ++ph) {
-                const int i = 2 * tid + (ph & 1);
-                if (i + 1 < n) {
-                    const unsigned long long x = S.lbits[t][i], y = S.lbits[t][i + 1];
-                    const int px = S.lpos[t][i], py = S.lpos[t][i + 1];
-                    if (x > y || (x == y && px > py)) {
-                        S.lbits[t][i] = y;
-                        S.lbits[t][i + 1] = x;
-                        S.lpos[t][i] = py;
-                        S.lpos[t][i + 1] = px;
-                    }
-                }
-                __syncthreads();
-            }
-            if (tid == 0) {
-                S.thr_bits[t] = S.lbits[t][S.need[t] - 1];
-                S.thr_pos[t] = S.lpos[t][S.need[t] - 1];
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            int acc = 0;
-            for (int t = 0; t < nb; ++t) {
-                S.base[t] = acc;
-                acc += S.cnt[t] < cap ? S.cnt[t] : cap;
-                S.run[t] = 0;
-            }
-            S.base[nb] = acc;
-        }
-        __syncthreads();
-        // emit bucket by bucket in scan order (list position i = i0 + u * kSelT + tid)
-        {
-            const int lane = tid & 63, wv = tid >> 6;
-            int runb[LEKF_REG_NB];
-#pragma unroll
-            for (int t = 0; t < LEKF_REG_NB; ++t) runb[t] = 0;
-            for (int i0 = 0; i0 < nacc; i0 += kSelT * kSelU) {
-                unsigned long long bits[kSelU];
-                int ob[kSelU];
-                bool keep[kSelU];
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
-                    const int i = i0 + u * kSelT + tid;
-                    const bool in = i < nacc;
-                    bits[u] = in ? skey[i] : 0ull;
-                    ob[u] = in ? sob[i] : 0;
-                    const int t = ob[u] >> 30;
-                    keep[u] = in && (S.mode[t] == 0 || bits[u] < S.thr_bits[t] ||
-                                     (bits[u] == S.thr_bits[t] && i <= S.thr_pos[t]));
-                }
-                unsigned long long m[kSelU][LEKF_REG_NB];
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u)
-#pragma unroll
-                    for (int t = 0; t < LEKF_REG_NB; ++t) m[u][t] = __ballot(keep[u] && (ob[u] >> 30) == t);
-                if (lane == 0)
-#pragma unroll
-                    for (int u = 0; u < kSelU; ++u)
-#pragma unroll
-                        for (int t = 0; t < LEKF_REG_NB; ++t) S.ecnt[wv][u][t] = __popcll(m[u][t]);
-                __syncthreads();
-                int pos[kSelU];
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u) pos[u] = -1;
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u)
-#pragma unroll
-                    for (int t = 0; t < LEKF_REG_NB; ++t) {
-                        int before = 0, tot = 0;
-                        for (int w = 0; w < kSelW; ++w) {
-                            const int cw = S.ecnt[w][u][t];
-                            before += w < wv ? cw : 0;
-                            tot += cw;
-                        }
-                        if (keep[u] && (ob[u] >> 30) == t)
-                            pos[u] = runb[t] + before + __popcll(m[u][t] & lane_lt(lane));
-                        runb[t] += tot;
-                    }
-                __syncthreads();
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
-                    const int t = ob[u] >> 30;
-                    if (keep[u] && pos[u] < S.base[t + 1] - S.base[t]) {
-                        const long w = obase + S.base[t] + pos[u];
-                        A.loc_idx[w] = ob[u] & 0x3fffffff;
-                        A.loc_rho[w] = srho[i0 + u * kSelT + tid];
-                    }
-                }
-            }
-        }
+        const int kept = cap_select(S, tid, skey, sob, srho, nacc, nb, cap, A.loc_idx + obase,
+                                    A.loc_rho + obase);
         if (tid == 0) {
             A.loc_off[b] = obase;
-            A.loc_cnt[b] = S.base[nb];
+            A.loc_cnt[b] = kept;
         }
         __syncthreads();
     }

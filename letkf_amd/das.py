@@ -148,6 +148,29 @@ class GlobalLetkf(_Handle):
                                              iptr(q) if q is not None else None,
                                              hx.ctypes.data_as(_lib._dp)), "set_obs_global")
 
+    def set_obs_hx(self, elm, lon, lat, lev, dat, err):
+        """set_letkf_obs with H(x) computed on the GPU from the resident background
+        (the reference obs operator; the obs go through the same QC and binning)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (elm, lon, lat, lev, dat, err)]
+        check(_lib.lib().lekf_set_obs_global_hx(self.h, arrs[0].size, *[dptr(a) for a in arrs]),
+              "set_obs_global_hx")
+
+    def obs_operator(self, elm, lon, lat, lev, dat):
+        """H(x) of every member of the resident background: (hxf (nobs, ne), qc (nobs),
+        ri (nobs), rj (nobs), rk (nobs, ne))."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (elm, lon, lat, lev, dat)]
+        n = arrs[0].size
+        hxf = np.zeros((n, self.ne), order="F")
+        rk = np.zeros((n, self.ne), order="F")
+        qc = np.zeros(n, dtype=np.int32)
+        ri = np.zeros(n)
+        rj = np.zeros(n)
+        check(_lib.lib().lekf_obsope_global(self.h, n, *[dptr(a) for a in arrs],
+                                            hxf.ctypes.data_as(_lib._dp), iptr(qc), dptr(ri),
+                                            dptr(rj), rk.ctypes.data_as(_lib._dp)),
+              "obsope_global")
+        return hxf, qc, ri, rj, rk
+
     def obs_index(self):
         nb = ctypes.c_int(0)
         grd = np.zeros((self.nlon, self.nlat), dtype=np.int32, order="F")

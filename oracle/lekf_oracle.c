@@ -20,7 +20,10 @@
  *   orc_obs_local_l96 -> closed form (p = 21 at every point for the L96 config).
  *   orc_obs_local_regional -> brute force + insertion-sort cap restated independently
  *          in the test.
- *   orc_phys2ijk / orc_itpl_* -> exact reproduction of linear fields, CEILING tables.
+ *   orc_phys2ij / orc_itpl_* / orc_trans_xtoy -> reproduction of fields linear in
+ *          the grid indices, the longitude wrap to column 1, latitude-table
+ *          points mapping to integer rj, the obsope vertical and type rules
+ *          (tests/test_oracle_obsope.py).
  */
 #include <float.h>
 #include <math.h>

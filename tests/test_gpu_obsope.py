@@ -1,0 +1,103 @@
+"""GPU obs operator (phys2ijk + obsope rules + Trans_XtoY on the resident ensemble)
+against the oracle's restatement of speedy/common/common_obs_speedy.f90:30-282.
+
+Bit-exact: the QC flag, ri, rj and the CEILING corners (i, j, k).  rk and H(x)
+pass through log (device vs host libm): rk within 1e-12, H(x) within 1e-12 of
+the field scale.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from letkf_amd import das, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prob():
+    return synth.speedy()
+
+
+def _handle(p):
+    h = das.GlobalLetkf(lon=p["lon"], lat=p["lat"], sig=p["sig"], ne=p["ne"], nv3d=4,
+                        nv2d=p["gues2d"].shape[2], sigma_obs=p["sigma_obs"],
+                        sigma_obsv=p["sigma_obsv"], gross_error=p["gross_error"],
+                        infl_value=p["infl"])
+    h.set_ensemble(p["gues3d"], p["gues2d"])
+    return h
+
+
+def _obs_sample(p, n=500, seed=1):
+    o = p["obs"]
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(o["elm"].size, n, replace=False)
+    cols = {k: np.asarray(o[k])[sel].astype(float) for k in ("elm", "lon", "lat", "lev", "dat")}
+    # edge cases: longitude 0 / 360 / wrap cell, poles, out-of-range levels, bad ps, rh
+    lat0, latn = p["lat"][0], p["lat"][-1]
+    extra = [(oracle.ID_U, 0.0, 10.0, 5.0e4, 1.0), (oracle.ID_V, 360.0, -20.0, 3.0e4, 1.0),
+             (oracle.ID_T, 359.5, 30.0, 7.0e4, 1.0), (oracle.ID_T, 100.0, lat0 - 0.5, 5.0e4, 1.0),
+             (oracle.ID_Q, 100.0, latn + 0.5, 5.0e4, 1.0), (oracle.ID_U, 50.0, 0.0, 1.2e5, 1.0),
+             (oracle.ID_T, 50.0, 0.0, 1.2e5, 1.0), (oracle.ID_T, 50.0, 0.0, 1.0e3, 1.0),
+             (oracle.ID_PS, 50.0, 0.0, 0.0, -500.0), (oracle.ID_PS, 3.0, 40.0, 0.0, 1.0e5),
+             (oracle.ID_RH, 50.0, 0.0, 5.0e4, 0.5), (oracle.ID_U, -2.0, 0.0, 5.0e4, 1.0)]
+    for e in extra:
+        for k, v in zip(("elm", "lon", "lat", "lev", "dat"), e):
+            cols[k] = np.append(cols[k], v)
+    return cols
+
+
+def _member_fields(p, m):
+    nlon, nlat, nlev = p["lon"].size, p["lat"].size, p["sig"].size
+    v3d = np.ascontiguousarray(p["gues3d"][:, :, m, :].transpose(2, 1, 0)).reshape(4, nlev, nlat, nlon)
+    ps = p["gues2d"][:, m, 0].reshape(nlat, nlon)
+    return v3d, ps
+
+
+def test_obs_operator_matches_oracle(gpu, prob):
+    p = prob
+    h = _handle(p)
+    o = _obs_sample(p)
+    hxf, qc, ri, rj, rk = h.obs_operator(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"])
+    n, ne = hxf.shape
+    ref_qc = np.ones(n, dtype=np.int32)
+    scale = np.abs(p["gues3d"]).max(axis=(0, 1, 2))
+    for m in range(ne):
+        v3d, ps = _member_fields(p, m)
+        for i in range(n):
+            st, y, ri0, rj0, rk0 = oracle.trans_xtoy(o["elm"][i], o["lon"][i], o["lat"][i],
+                                                     o["lev"][i], o["dat"][i], p["lat"], p["sig"],
+                                                     v3d, ps)
+            if st:
+                ref_qc[i] = 0
+                assert hxf[i, m] == 0.0  # a member that rejects the obs reports 0
+                continue
+            if qc[i]:
+                assert ri[i] == ri0 and rj[i] == rj0, i
+                assert np.ceil(rk[i, m]) == np.ceil(rk0), (i, m)
+                assert abs(rk[i, m] - rk0) <= 1e-12, (i, m)
+                el = int(o["elm"][i])
+                sc = 1.0e5 if el == oracle.ID_PS else scale[[oracle.ID_U, oracle.ID_V, oracle.ID_T,
+                                                             oracle.ID_Q].index(el)]
+                assert abs(hxf[i, m] - y) <= 1e-12 * sc, (i, m, hxf[i, m], y)
+    np.testing.assert_array_equal(qc, ref_qc)
+    assert ref_qc.sum() > 400 and ref_qc[-12:].sum() < 12  # the edge rows include rejects
+    h.close()
+
+
+def test_set_obs_with_device_hx_equals_host_hx(gpu, prob):
+    p = prob
+    o = p["obs"]
+    h1 = _handle(p)
+    hxf, qc, _, _, _ = h1.obs_operator(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"])
+    h1.set_obs(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"], o["err"], hxf, qc=qc)
+    h1.run()
+    a1 = h1.analysis()
+    h2 = _handle(p)
+    h2.set_obs_hx(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"], o["err"])
+    h2.run()
+    a2 = h2.analysis()
+    assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
+    np.testing.assert_array_equal(h1.nobsl(), h2.nobsl())
+    h1.close()
+    h2.close()
