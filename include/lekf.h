@@ -146,6 +146,8 @@ typedef struct lekf_global_cfg {
     int prec;                 /* LEKF_PREC_* */
     int infl_mode;            /* LEKF_INFL_* */
     double infl_value;        /* fixed value, or the initial adaptive field value */
+    int nobs_max;             /* > 0: keep at most nobs_max local obs per point, nearest
+                               * first by err^2/rho (ties: scan order); 0: no cap */
 } lekf_global_cfg;
 
 typedef struct lekf_l96_cfg {

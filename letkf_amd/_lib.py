@@ -67,7 +67,7 @@ class GlobalCfg(Structure):
         ("sig", _dp), ("ne", c_int), ("nv3d", c_int), ("nv2d", c_int), ("ips", c_int),
         ("iq", c_int), ("q_top_lev", c_int), ("sigma_obs", c_double), ("sigma_obsv", c_double),
         ("gross_error", c_double), ("var_local", c_double * 7), ("prec", c_int),
-        ("infl_mode", c_int), ("infl_value", c_double),
+        ("infl_mode", c_int), ("infl_value", c_double), ("nobs_max", c_int),
     ]
 
 

@@ -496,6 +496,10 @@ struct lekf_handle {
     DevBuf maxcand;
     long slot_cap = 0;
     int sel_slots = 0;
+    // global grid with nobs_max: raw lists per column chunk, capped lists at b * nobs_max
+    DevBuf capl_off, capl_cnt, capl_idx, capl_rho;
+    std::vector<int> gchunk_cols;    // column chunk boundaries
+    std::vector<long> gchunk_base;   // cand_off * nlev at the boundaries
     std::vector<long> chunk_pts;  // chunk boundaries (points)
     std::vector<long> chunk_off;  // list offsets at the boundaries
     long list_budget = 512L * 1024 * 1024;
@@ -518,7 +522,8 @@ long handle_bytes(const lekf_handle* h)
                           &h->loc_rho, &h->status, &h->djob, &h->d_zlev, &h->cellcum,
                           &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
                           &h->r_dep, &h->r_bkt, &h->r_z, &h->gws, &h->sel_key,
-                          &h->sel_ob, &h->sel_rho, &h->maxcand};
+                          &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off, &h->capl_cnt,
+                          &h->capl_idx, &h->capl_rho};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -536,7 +541,8 @@ void free_all(lekf_handle* h)
                     &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
                     &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
                     &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
-                    &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->maxcand};
+                    &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off,
+                    &h->capl_cnt, &h->capl_idx, &h->capl_rho};
     for (auto* b : bs) b->release();
 }
 
@@ -599,10 +605,11 @@ void fill_job(lekf_handle* h)
     J.hdxf32 = h->prec != LEKF_PREC_F64 ? h->o_hdxf32.as<float>() : nullptr;
     J.rdiag = h->o_rdiag.as<double>();
     J.dep = h->o_dep.as<double>();
-    J.loc_off = h->loc_off.as<long>();
-    J.loc_cnt = h->loc_cnt.as<int>();
-    J.loc_idx = h->loc_idx.as<int>();
-    J.loc_rho = h->loc_rho.as<double>();
+    const bool capped = h->kind == 0 && h->g.nobs_max > 0;  // the solve reads the capped lists
+    J.loc_off = (capped ? h->capl_off : h->loc_off).as<long>();
+    J.loc_cnt = (capped ? h->capl_cnt : h->loc_cnt).as<int>();
+    J.loc_idx = (capped ? h->capl_idx : h->loc_idx).as<int>();
+    J.loc_rho = (capped ? h->capl_rho : h->loc_rho).as<double>();
     J.infl_in = h->infl_in.as<double>();
     J.infl_out = h->infl_out.as<double>();
     J.status = h->status.as<int>();
@@ -724,7 +731,38 @@ int run_select(lekf_handle* h)
         A.loc_cnt = h->loc_cnt.as<int>();
         A.loc_idx = h->loc_idx.as<int>();
         A.loc_rho = h->loc_rho.as<double>();
-        HIPCHK(lekf_cull_global_fill_launch(A, st));
+        if (h->g.nobs_max <= 0) {
+            HIPCHK(lekf_cull_global_fill_launch(A, h->nij, st));
+        } else {
+            // column chunks: raw lists of the chunk, then the nearest-first cap
+            ListCapArgs K;
+            memset(&K, 0, sizeof(K));
+            K.nij = h->nij;
+            K.nlev = h->nlev;
+            K.cap = h->g.nobs_max;
+            K.in_off = h->loc_off.as<long>();
+            K.in_cnt = h->loc_cnt.as<int>();
+            K.in_idx = h->loc_idx.as<int>();
+            K.in_rho = h->loc_rho.as<double>();
+            K.rdiag = h->o_rdiag.as<double>();
+            K.out_off = h->capl_off.as<long>();
+            K.out_cnt = h->capl_cnt.as<int>();
+            K.out_idx = h->capl_idx.as<int>();
+            K.out_rho = h->capl_rho.as<double>();
+            K.s_key = h->sel_key.as<unsigned long long>();
+            K.s_ob = h->sel_ob.as<int>();
+            K.s_rho = h->sel_rho.as<double>();
+            K.slot_cap = h->slot_cap;
+            for (size_t c = 0; c + 1 < h->gchunk_cols.size(); ++c) {
+                A.col_begin = h->gchunk_cols[c];
+                A.list_base = h->gchunk_base[c];
+                const int ncols = h->gchunk_cols[c + 1] - h->gchunk_cols[c];
+                HIPCHK(lekf_cull_global_fill_launch(A, ncols, st));
+                K.col_begin = A.col_begin;
+                K.ncols = ncols;
+                HIPCHK(lekf_list_cap_launch(K, h->sel_slots, st));
+            }
+        }
     } else {
         L96CullArgs A;
         A.nx = h->nij;
@@ -1116,7 +1154,38 @@ int set_obs_global_dev(lekf_handle* h, int nobs, bool have_qc, long h_so, long h
     HIPCHK(hipMemcpyAsync(&tot, h->cand_off.as<long>() + h->nij, sizeof(long), hipMemcpyDeviceToHost,
                           st));
     HIPCHK(hipStreamSynchronize(st));
-    h->loc_cap = tot * h->nlev;
+    if (h->g.nobs_max <= 0) {
+        h->loc_cap = tot * h->nlev;
+    } else {
+        // raw lists in column chunks that fit the list budget; capped lists for all points
+        std::vector<long> co(h->nij + 1);
+        HIPCHK(hipMemcpy(co.data(), h->cand_off.p, sizeof(long) * (h->nij + 1),
+                         hipMemcpyDeviceToHost));
+        h->gchunk_cols.assign(1, 0);
+        h->gchunk_base.assign(1, 0);
+        long maxraw = 1, maxcol = 1;
+        for (int c = 0; c < h->nij;) {
+            int d = c + 1;
+            while (d < h->nij && (co[d + 1] - co[c]) * h->nlev <= h->list_budget) ++d;
+            h->gchunk_cols.push_back(d);
+            h->gchunk_base.push_back(co[d] * h->nlev);
+            maxraw = std::max(maxraw, (co[d] - co[c]) * h->nlev);
+            c = d;
+        }
+        for (int c = 0; c < h->nij; ++c) maxcol = std::max(maxcol, co[c + 1] - co[c]);
+        h->loc_cap = maxraw;
+        const size_t npt = (size_t)h->nij * h->nlev, cap = (size_t)h->g.nobs_max;
+        HIPCHK(h->capl_off.ensure(sizeof(long) * npt));
+        HIPCHK(h->capl_cnt.ensure(sizeof(int) * npt));
+        HIPCHK(h->capl_idx.ensure(sizeof(int) * npt * cap));
+        HIPCHK(h->capl_rho.ensure(sizeof(double) * npt * cap));
+        h->sel_slots = lekf_reg_select_slots();
+        h->slot_cap = maxcol;
+        const size_t se = (size_t)h->sel_slots * (size_t)h->slot_cap;
+        HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * se));
+        HIPCHK(h->sel_ob.ensure(sizeof(int) * se));
+        HIPCHK(h->sel_rho.ensure(sizeof(double) * se));
+    }
     HIPCHK(h->loc_idx.ensure(sizeof(int) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
     HIPCHK(h->loc_rho.ensure(sizeof(double) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
     h->have_obs = true;
@@ -1512,7 +1581,7 @@ int lekf_get_nobsl(lekf_handle* h, int* nobsl)
 {
     if (!h || !nobsl) return LEKF_ERR_ARG;
     const size_t npt = (size_t)h->nij * h->nlev;
-    HIPCHK(hipMemcpyAsync(nobsl, h->loc_cnt.p, sizeof(int) * npt, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(nobsl, h->job.loc_cnt, sizeof(int) * npt, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return LEKF_OK;
 }
@@ -1531,15 +1600,19 @@ int lekf_get_local(lekf_handle* h, long b, int cap, int* idx, double* rho)
         }
         if (hipStreamSynchronize(h->st) != hipSuccess) return -LEKF_ERR_HIP;
     }
+    if (!h->have_obs) return -LEKF_ERR_ARG;
+    if (hipStreamSynchronize(h->st) != hipSuccess) return -LEKF_ERR_HIP;
+    // the lists the solve reads (capped ones for a global grid with nobs_max)
+    const lekf_job& J = h->job;
     long off = 0;
     int cnt = 0;
-    if (hipMemcpy(&off, h->loc_off.as<long>() + b, sizeof(long), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&cnt, h->loc_cnt.as<int>() + b, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(&off, J.loc_off + b, sizeof(long), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&cnt, J.loc_cnt + b, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return -LEKF_ERR_HIP;
     const int m = cnt < cap ? cnt : cap;
     if (m > 0) {
-        if (hipMemcpy(idx, h->loc_idx.as<int>() + off, sizeof(int) * m, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(rho, h->loc_rho.as<double>() + off, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(idx, J.loc_idx + off, sizeof(int) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(rho, J.loc_rho + off, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
             return -LEKF_ERR_HIP;
     }
     return cnt;

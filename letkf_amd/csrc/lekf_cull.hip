@@ -159,7 +159,7 @@ __device__ __forceinline__ double dist_gc(double sin_o, double cos_o, double lon
 
 __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
 {
-    const int ij = blockIdx.x;
+    const int ij = A.col_begin + blockIdx.x;
     const int lane = threadIdx.x;
     __shared__ int s_cnt[LEKF_MAX_LEV];
     __shared__ int s_seg_beg[2 * LEKF_MAX_BANDS];
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
     __syncthreads();
     const int ncand = s_seg_pre[nsegs];
     const int cap = A.cand_cnt[ij];
-    const long base = A.cand_off[ij] * (long)nlev;
+    const long base = A.cand_off[ij] * (long)nlev - A.list_base;
     const double lat1 = A.col_lat[ij];
     const double r180 = 1.0 / 180.0;
     const double lonr_g = __dmul_rn(__dmul_rn(A.col_lon[ij], LEKF_PI_REF), r180);
@@ -528,9 +528,10 @@ hipError_t lekf_cull_global_count_launch(const GlobalCullArgs& A, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, hipStream_t st)
+hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, int ncols, hipStream_t st)
 {
-    hipLaunchKernelGGL(cull_global_fill, dim3(A.nij), dim3(64), 0, st, A);
+    if (ncols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cull_global_fill, dim3(ncols), dim3(64), 0, st, A);
     return hipGetLastError();
 }
 

@@ -25,10 +25,12 @@ struct GlobalCullArgs {
     // outputs
     int* cand_cnt;               // nij
     long* cand_off;              // nij + 1
-    long* loc_off;               // nlev * nij
+    long* loc_off;               // nlev * nij (relative to the list buffer of the launch)
     int* loc_cnt;                // nlev * nij
-    int* loc_idx;                // nlev * sum(cand_cnt)
+    int* loc_idx;                // nlev * sum(cand_cnt) over the launch's columns
     double* loc_rho;
+    int col_begin;               // first column of a fill launch
+    long list_base;              // cand_off[col_begin] * nlev
 };
 
 struct L96CullArgs {
@@ -66,7 +68,7 @@ struct ObsPrepArgs {
 
 hipError_t lekf_cull_global_launch(const GlobalCullArgs& A, hipStream_t st);
 hipError_t lekf_cull_global_count_launch(const GlobalCullArgs& A, hipStream_t st);
-hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, hipStream_t st);
+hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, int ncols, hipStream_t st);
 hipError_t lekf_cull_l96_launch(const L96CullArgs& A, hipStream_t st);
 hipError_t lekf_obs_prep_launch(const ObsPrepArgs& A, hipStream_t st);
 hipError_t lekf_sort_keys_launch(unsigned long long* keys, long n2, hipStream_t st);

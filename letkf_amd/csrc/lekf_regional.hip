@@ -488,6 +488,45 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
     }
 }
 
+// Nearest-first cap of per-point lists (persistent, one point per workgroup).
+__global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
+{
+    __shared__ SelLds S;
+    const int tid = threadIdx.x;
+    unsigned long long* skey = A.s_key + (size_t)blockIdx.x * A.slot_cap;
+    int* sob = A.s_ob + (size_t)blockIdx.x * A.slot_cap;
+    double* srho = A.s_rho + (size_t)blockIdx.x * A.slot_cap;
+    const long npt = (long)A.ncols * A.nlev;
+    for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
+        const long b = (pi / A.ncols) * (long)A.nij + A.col_begin + pi % A.ncols;
+        const long off = A.in_off[b];
+        const int n = A.in_cnt[b];
+        const long ob = b * (long)A.cap;
+        int kept = n;
+        if (n <= A.cap) {
+            for (int i = tid; i < n; i += kSelT) {
+                A.out_idx[ob + i] = A.in_idx[off + i];
+                A.out_rho[ob + i] = A.in_rho[off + i];
+            }
+        } else {
+            for (int i = tid; i < n; i += kSelT) {
+                const int o = A.in_idx[off + i];
+                const double rho = A.in_rho[off + i];
+                skey[i] = (unsigned long long)__double_as_longlong(__ddiv_rn(A.rdiag[o], rho));
+                sob[i] = o;
+                srho[i] = rho;
+            }
+            __syncthreads();
+            kept = cap_select(S, tid, skey, sob, srho, n, 1, A.cap, A.out_idx + ob, A.out_rho + ob);
+        }
+        if (tid == 0) {
+            A.out_off[b] = ob;
+            A.out_cnt[b] = kept;
+        }
+        __syncthreads();
+    }
+}
+
 // per point: candidate count of the rectangle -> list capacity
 __global__ void reg_capacity(RegionalArgs A, int* capn, int* maxcand)
 {
@@ -634,6 +673,15 @@ int lekf_reg_select_slots()
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     return 4 * (ncu > 0 ? ncu : 256);
+}
+
+hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st)
+{
+    const long npt = (long)A.ncols * A.nlev;
+    if (npt <= 0) return hipSuccess;
+    const long g = npt < nslots ? npt : nslots;
+    hipLaunchKernelGGL(list_cap, dim3((unsigned)g), dim3(kSelT), 0, st, A);
+    return hipGetLastError();
 }
 
 hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st)

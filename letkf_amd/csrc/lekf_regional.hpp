@@ -46,6 +46,30 @@ struct RegionalArgs {
     long slot_cap;                          // entries per workgroup slot
 };
 
+// Nearest-first cap of per-point local lists (the global grid's nobs_max):
+// keeps the cap entries with the smallest (rdiag[o]/rho, list position) of
+// each input list and writes them in list order at out_*[b * cap].
+struct ListCapArgs {
+    int nij, nlev;                          // points b = lev * nij + ij
+    int col_begin, ncols;                   // columns [col_begin, col_begin + ncols), all levels
+    int cap;
+    const long* in_off;
+    const int* in_cnt;
+    const int* in_idx;
+    const double* in_rho;
+    const double* rdiag;                    // per sorted obs (err^2)
+    long* out_off;
+    int* out_cnt;
+    int* out_idx;
+    double* out_rho;
+    unsigned long long* s_key;              // per-workgroup scratch, slot_cap entries each
+    int* s_ob;
+    double* s_rho;
+    long slot_cap;                          // >= the longest input list
+};
+
+hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st);
+
 hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
 hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
                                    hipStream_t st);

@@ -115,7 +115,7 @@ class _Handle:
 class GlobalLetkf(_Handle):
     def __init__(self, *, lon, lat, sig, ne, nv3d=4, nv2d=1, ips=0, iq=3, q_top_lev=5,
                  sigma_obs=500.0e3, sigma_obsv=0.4, gross_error=10.0, var_local=None,
-                 prec="f64", infl_mode="adaptive", infl_value=1.01):
+                 prec="f64", infl_mode="adaptive", infl_value=1.01, nobs_max=0):
         self._lon = np.ascontiguousarray(lon, dtype=np.float64)
         self._lat = np.ascontiguousarray(lat, dtype=np.float64)
         self._sig = np.ascontiguousarray(sig, dtype=np.float64)
@@ -130,6 +130,7 @@ class GlobalLetkf(_Handle):
         c.prec = _lib.PREC[prec]
         c.infl_mode = _lib.LEKF_INFL_FIXED if infl_mode == "fixed" else _lib.LEKF_INFL_ADAPTIVE
         c.infl_value = infl_value
+        c.nobs_max = nobs_max
         st = ctypes.c_int(0)
         self.h = _lib.lib().lekf_global_create(ctypes.byref(c), ctypes.byref(st))
         if not self.h:

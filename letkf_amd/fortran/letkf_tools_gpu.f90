@@ -26,6 +26,7 @@ MODULE letkf_tools_gpu
     REAL(C_DOUBLE) :: var_local(7)
     INTEGER(C_INT) :: prec, infl_mode
     REAL(C_DOUBLE) :: infl_value
+    INTEGER(C_INT) :: nobs_max
   END TYPE
 
   TYPE, BIND(C) :: lekf_l96_cfg_t
@@ -162,6 +163,7 @@ SUBROUTINE das_letkf_global(nlon, nlat, nlev, lon, lat, sig, ne, nv3d, nv2d, gue
     IF (PRESENT(prec)) cfg%prec = prec
     cfg%infl_mode = 0
     cfg%infl_value = 1.01d0
+    cfg%nobs_max = 0
     h_global = lekf_global_create(cfg, st)
     CALL stop_on(st, 'lekf_global_create')
     g_shape = shp
