@@ -247,6 +247,62 @@ def radar_cpu_sample(p, npts):
                        f"not timed")}
 
 
+def gfs_bench(steps=2, precs=("f64", "f32", "bf16x3")):
+    """GFS-sized global config (configs[2]): 192x94x64, u,v,T,q + ps, k=40, ~600k
+    clustered obs, sigma_h 700 km, sigma_v 0.4 ln p, nobs_max = 1000 nearest-first.
+    Each form's analysis is compared with the FP64 GPU analysis."""
+    from letkf_amd import das, synth
+    from letkf_amd.device import Event
+    t0 = time.perf_counter()
+    p = synth.gfs()
+    gen_s = time.perf_counter() - t0
+    npt = p["nij"] * p["nlev"]
+    spread = p["gues3d"].std(axis=2, ddof=1)
+    res, ref = {}, None
+    for prec in precs:
+        h = das.GlobalLetkf(lon=p["lon"], lat=p["lat"], sig=p["sig"], ne=p["ne"], nv3d=4, nv2d=1,
+                            iq=p["iq"], q_top_lev=p["q_top_lev"], sigma_obs=p["sigma_obs"],
+                            sigma_obsv=p["sigma_obsv"], gross_error=p["gross_error"], prec=prec,
+                            infl_value=p["infl"], nobs_max=1000)
+        h.set_ensemble(p["gues3d"], p["gues2d"])
+        o = p["obs"]
+        h.set_obs(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"], o["err"], o["hxf"])
+        h.run()
+        h.sync()
+        st = h.stream()
+        lat, sel, sol = [], [], []
+        e = [Event() for _ in range(3)]
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            e[0].record(st)
+            h.select()
+            e[1].record(st)
+            h.solve()
+            e[2].record(st)
+            h.sync()
+            lat.append((time.perf_counter() - t1) * 1e3)
+            sel.append(Event.elapsed_ms(e[0], e[1]))
+            sol.append(Event.elapsed_ms(e[1], e[2]))
+        a3, _ = h.analysis()
+        nl = h.nobsl()
+        h.close()
+        r = {"local_analyses_per_s": npt / (np.mean(lat) / 1e3), "cycle_ms": float(np.mean(lat)),
+             "select_ms": float(np.mean(sel)), "solve_apply_ms": float(np.mean(sol)),
+             "mean_nobsl": float(nl.mean())}
+        if ref is None:
+            ref = a3
+        else:
+            r["err_vs_f64_over_bg_spread"] = float((np.abs(a3 - ref).max(axis=2) /
+                                                   np.maximum(spread, 1e-300)).max())
+        res[prec] = r
+        print(f"[bench] gfs {prec}: {r['cycle_ms']:.0f} ms/cycle", file=sys.stderr, flush=True)
+    for prec in res:
+        res[prec]["speedup_vs_f64"] = res[prec]["local_analyses_per_s"] / res["f64"]["local_analyses_per_s"]
+    return {"grid": [p["lon"].size, p["lat"].size, p["sig"].size], "k": p["ne"],
+            "local_analyses_per_cycle": npt, "nobs": int(p["obs"]["elm"].size),
+            "nobs_max": 1000, "gen_s": gen_s, "forms": res}
+
+
 def serve_bench(duration_s=20.0, rate=2.0, precs=("f64", "bf16x3"), seed=5):
     """Served analysis stream: Poisson arrivals of LETKF cycles of mixed sizes,
     each with a 30-s deadline.  Mix (by arrival): 50% Lorenz-96 (N in 40..960,
@@ -419,6 +475,7 @@ def main():
     if not args.no_extra:
         sub = (256, 256, 60) if args.radar_full else (128, 128, 60)
         out["radar"] = radar_bench(sub, steps=3, cpu_sample=0 if args.no_cpu else 1500)
+        out["gfs"] = gfs_bench()
         out["serve"] = serve_bench(duration_s=args.serve_s)
     print(json.dumps(out), flush=True)
 
