@@ -219,6 +219,11 @@ int lekf_set_obs_l96(lekf_handle *h, int ny, const int *gidx, const double *y, c
 /* one analysis (selection + local solves + apply) on the handle's stream;
  * asynchronous.  The analysis stays on the device. */
 int lekf_das_run(lekf_handle *h);
+/* enable != 0: lekf_das_run replays the cycle's kernel launches as one HIP
+ * graph.  The first run after enabling or after any lekf_set_* /
+ * lekf_cycle_swap runs eagerly and captures the graph; later runs launch it.
+ * The graph holds no memset or allocation. */
+int lekf_set_graph(lekf_handle *h, int enable);
 /* the two halves of lekf_das_run: selection kernels, then the local solves + apply */
 int lekf_das_select(lekf_handle *h);
 int lekf_das_solve(lekf_handle *h);

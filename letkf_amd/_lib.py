@@ -134,6 +134,8 @@ def lib():
     L.lekf_set_obs_l96.argtypes = [c_void_p, c_int, _ip, _dp, _dp, _dp]
     L.lekf_das_run.restype = c_int
     L.lekf_das_run.argtypes = [c_void_p]
+    L.lekf_set_graph.restype = c_int
+    L.lekf_set_graph.argtypes = [c_void_p, c_int]
     L.lekf_das_select.restype = c_int
     L.lekf_das_select.argtypes = [c_void_p]
     L.lekf_das_solve.restype = c_int
@@ -210,7 +212,7 @@ EXPORTED = [
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
-    "lekf_set_obs_global_hx", "lekf_obsope_global",
+    "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph",
 ]
 
 

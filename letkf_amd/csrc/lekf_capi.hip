@@ -478,6 +478,9 @@ struct lekf_handle {
     DevBuf tmp_mean, tmp_dep, keys, cell_cnt, perm, o_hdxf, o_hdxf32, o_dep, o_rdiag, o_err,
         o_typ, o_sin, o_cos, o_lonr, o_lnlev, o_lndat, nobsgrd;
     bool have_obs = false;
+    // replay of the cycle's launches as one HIP graph (lekf_set_graph)
+    bool use_graph = false, graph_ok = false;
+    hipGraphExec_t gexec = nullptr;
     // local lists
     DevBuf cand_cnt, cand_off, loc_off, loc_cnt, loc_idx, loc_rho, status;
     long loc_cap = 0;
@@ -886,10 +889,18 @@ lekf_handle* lekf_l96_create(const lekf_l96_cfg* cfg, int* status)
     return h;
 }
 
+void graph_drop(lekf_handle* h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->graph_ok = false;
+}
+
 void lekf_destroy(lekf_handle* h)
 {
     if (!h) return;
     (void)hipStreamSynchronize(h->st);
+    graph_drop(h);
     free_all(h);
     if (h->own_stream) (void)hipStreamDestroy(h->st);
     delete h;
@@ -902,6 +913,7 @@ long lekf_device_bytes(lekf_handle* h) { return h ? handle_bytes(h) : 0; }
 int lekf_set_ensemble(lekf_handle* h, const double* gues3d, const double* gues2d)
 {
     if (!h || !gues3d || (h->nv2d > 0 && !gues2d)) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     hipStream_t st = h->st;
     const size_t n3 = (size_t)h->nij * h->nlev * h->ne * h->nv3d;
     const size_t n2 = (size_t)h->nij * h->ne * h->nv2d;
@@ -925,6 +937,7 @@ int lekf_set_ensemble(lekf_handle* h, const double* gues3d, const double* gues2d
 int lekf_set_infl(lekf_handle* h, const double* infl3d, const double* infl2d)
 {
     if (!h || !infl3d) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     const size_t npt = (size_t)h->nij * h->nlev;
     // reference infl3d(nij,nlev) column-major == device point order lev*nij + ij
     HIPCHK(hipMemcpyAsync(h->infl_in.p, infl3d, sizeof(double) * npt, hipMemcpyHostToDevice, h->st));
@@ -997,6 +1010,7 @@ int lekf_set_obs_global(lekf_handle* h, int nobs, const double* elm, const doubl
                         const double* err, const int* qc, const double* hxf)
 {
     if (!h || h->kind != 0 || nobs < 0) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     if (nobs > 0 && (!elm || !lon || !lat || !lev || !dat || !err || !hxf)) return LEKF_ERR_ARG;
     hipStream_t st = h->st;
     int rc = upload_obs_records(h, nobs, elm, lon, lat, lev, dat, err);
@@ -1011,6 +1025,7 @@ int lekf_set_obs_global_hx(lekf_handle* h, int nobs, const double* elm, const do
                            const double* err)
 {
     if (!h || h->kind != 0 || nobs < 0 || !h->have_ens) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     if (nobs > 0 && (!elm || !lon || !lat || !lev || !dat || !err)) return LEKF_ERR_ARG;
     int rc = upload_obs_records(h, nobs, elm, lon, lat, lev, dat, err);
     if (!rc) rc = run_obsope_global(h, nobs, nullptr, nullptr, nullptr);
@@ -1200,6 +1215,7 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
                      const double* hxf)
 {
     if (!h || h->kind != 1 || ny < 0) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     hipStream_t st = h->st;
     const int ne = h->ne;
     const size_t n = (size_t)(ny > 0 ? ny : 1);
@@ -1252,7 +1268,7 @@ int lekf_das_solve(lekf_handle* h)
                      h->gws, h->st);
 }
 
-int lekf_das_run(lekf_handle* h)
+int das_run_eager(lekf_handle* h)
 {
     if (h && h->kind == 2 && h->have_ens && h->have_obs) {
         // chunks of points whose local lists fit the list buffer: select, then solve
@@ -1268,6 +1284,43 @@ int lekf_das_run(lekf_handle* h)
     int rc = lekf_das_select(h);
     if (rc) return rc;
     return lekf_das_solve(h);
+}
+
+int lekf_das_run(lekf_handle* h)
+{
+    if (!h) return LEKF_ERR_ARG;
+    if (!h->use_graph) return das_run_eager(h);
+    if (h->graph_ok) {
+        HIPCHK(hipGraphLaunch(h->gexec, h->st));
+        return LEKF_OK;
+    }
+    // first cycle after a change: run eagerly (this also sizes every work
+    // buffer), then capture the same launch sequence for the next cycles
+    int rc = das_run_eager(h);
+    if (rc) return rc;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    rc = das_run_eager(h);
+    const hipError_t e = hipStreamEndCapture(h->st, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) set_err(e, "hipStreamEndCapture");
+        return rc ? rc : LEKF_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ei);
+    h->graph_ok = true;
+    return LEKF_OK;
+}
+
+int lekf_set_graph(lekf_handle* h, int enable)
+{
+    if (!h) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    graph_drop(h);
+    h->use_graph = enable != 0;
+    return LEKF_OK;
 }
 
 lekf_handle* lekf_regional_create(const lekf_regional_cfg* cfg, int* status)
@@ -1325,6 +1378,7 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
                           const int* qc, const double* hdxf)
 {
     if (!h || h->kind != 2 || nobs < 0) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
     if (nobs > 0 && (!ri || !rj || !z || !err || !dep || !bucket || !hdxf)) return LEKF_ERR_ARG;
     hipStream_t st = h->st;
     const int ne = h->ne, nx = h->r.nx, ny = h->r.ny;
@@ -1654,6 +1708,7 @@ int lekf_cycle_swap(lekf_handle* h)
 {
     if (!h) return LEKF_ERR_ARG;
     HIPCHK(hipStreamSynchronize(h->st));
+    graph_drop(h);
     std::swap(h->x3b, h->x3a);
     std::swap(h->x2b, h->x2a);
     std::swap(h->infl_in, h->infl_out);

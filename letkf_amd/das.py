@@ -56,6 +56,11 @@ class _Handle:
     def run(self):
         check(_lib.lib().lekf_das_run(self.h), "das_run")
 
+    def set_graph(self, enable=True):
+        """Replay each cycle's launches as one HIP graph (captured on the first run
+        after any change of inputs)."""
+        check(_lib.lib().lekf_set_graph(self.h, int(bool(enable))), "set_graph")
+
     def select(self):
         check(_lib.lib().lekf_das_select(self.h), "das_select")
 
