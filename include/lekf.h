@@ -281,6 +281,17 @@ int lekf_event_elapsed_ms(void *ev0, void *ev1, float *ms);
 int lekf_event_query(void *ev);
 int lekf_event_sync(void *ev);
 
+/* launch tuning (process-wide; read when a launch is issued).  The values
+ * change kernel shapes only: the workspace size and the selection slot count
+ * leave every result bit-identical, the Gram wave count changes the order of
+ * the partial-sum reduction only.  Handles recapture their graphs after a
+ * change.  lekf_tune_get returns the value, or a negative error code. */
+#define LEKF_TUNE_GRAM_WAVES 0 /* waves per point of the k > 32 Gram kernel: 2, 4 or 8 */
+#define LEKF_TUNE_GRAM_WS_MB 1 /* Gram workspace per launch, MiB, 16 .. 65536 */
+#define LEKF_TUNE_SEL_PER_CU 2 /* capped-selection workgroups per CU: 1 .. 4 */
+int lekf_tune_set(int param, long value);
+long lekf_tune_get(int param);
+
 /* library info */
 int lekf_device_count(void);
 /* free and total device memory of the current GPU, bytes */

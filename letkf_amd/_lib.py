@@ -38,6 +38,10 @@ LEKF_F_MIXED = 32
 LEKF_F_BF16X3 = 64
 
 LEKF_PT_RANKDEF = 1
+
+LEKF_TUNE_GRAM_WAVES = 0
+LEKF_TUNE_GRAM_WS_MB = 1
+LEKF_TUNE_SEL_PER_CU = 2
 LEKF_PT_NOCONV = 2
 LEKF_NVMAX = 16
 LEKF_KMAX = 95
@@ -196,6 +200,10 @@ def lib():
     L.lekf_mem_info.argtypes = [POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t)]
     L.lekf_das_run_batch.restype = c_int
     L.lekf_das_run_batch.argtypes = [POINTER(c_void_p), c_int, c_void_p]
+    L.lekf_tune_set.restype = c_int
+    L.lekf_tune_set.argtypes = [c_int, c_long]
+    L.lekf_tune_get.restype = c_long
+    L.lekf_tune_get.argtypes = [c_int]
     _lib = L
     return L
 
@@ -212,7 +220,8 @@ EXPORTED = [
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
-    "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph",
+    "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
+    "lekf_tune_get",
 ]
 
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,12 +17,17 @@
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
 #include "lekf_obsope.hpp"
+#include "lekf_tune.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
+
+LekfTune g_lekf_tune;
 
 namespace {
 
 thread_local char g_errmsg[256];
+// launch tuning; the epoch counts changes so captured graphs can be retaken
+std::atomic<long> g_tune_epoch{0};
 
 void set_err(hipError_t e, const char* what)
 {
@@ -323,6 +329,37 @@ int lekf_device_count(void)
     return n;
 }
 
+int lekf_tune_set(int param, long value)
+{
+    switch (param) {
+    case LEKF_TUNE_GRAM_WAVES:
+        if (value != 2 && value != 4 && value != 8) return LEKF_ERR_ARG;
+        g_lekf_tune.gram_waves = (int)value;
+        break;
+    case LEKF_TUNE_GRAM_WS_MB:
+        if (value < 16 || value > 65536) return LEKF_ERR_ARG;
+        g_lekf_tune.gram_ws_bytes = (size_t)value << 20;
+        break;
+    case LEKF_TUNE_SEL_PER_CU:
+        if (value < 1 || value > 4) return LEKF_ERR_ARG;
+        g_lekf_tune.sel_per_cu = (int)value;
+        break;
+    default: return LEKF_ERR_ARG;
+    }
+    g_tune_epoch.fetch_add(1);
+    return LEKF_OK;
+}
+
+long lekf_tune_get(int param)
+{
+    switch (param) {
+    case LEKF_TUNE_GRAM_WAVES: return g_lekf_tune.gram_waves;
+    case LEKF_TUNE_GRAM_WS_MB: return (long)(g_lekf_tune.gram_ws_bytes >> 20);
+    case LEKF_TUNE_SEL_PER_CU: return g_lekf_tune.sel_per_cu;
+    default: return -LEKF_ERR_ARG;
+    }
+}
+
 int lekf_mem_info(size_t* free_bytes, size_t* total_bytes)
 {
     size_t f = 0, t = 0;
@@ -481,6 +518,7 @@ struct lekf_handle {
     // replay of the cycle's launches as one HIP graph (lekf_set_graph)
     bool use_graph = false, graph_ok = false;
     hipGraphExec_t gexec = nullptr;
+    long graph_epoch = 0;  // tuning epoch the graph was captured under
     // local lists
     DevBuf cand_cnt, cand_off, loc_off, loc_cnt, loc_idx, loc_rho, status;
     long loc_cap = 0;
@@ -688,7 +726,8 @@ int reg_select_chunk(lekf_handle* h, int c)
     RegionalArgs R = make_reg_args(h);
     R.pt_begin = h->chunk_pts[c];
     R.list_base = h->chunk_off[c];
-    HIPCHK(lekf_reg_select_launch(R, h->chunk_pts[c + 1] - h->chunk_pts[c], h->sel_slots, h->st));
+    HIPCHK(lekf_reg_select_launch(R, h->chunk_pts[c + 1] - h->chunk_pts[c],
+                                  std::min(h->sel_slots, lekf_reg_select_slots()), h->st));
     h->held_chunk = c;
     return 0;
 }
@@ -763,7 +802,7 @@ int run_select(lekf_handle* h)
                 HIPCHK(lekf_cull_global_fill_launch(A, ncols, st));
                 K.col_begin = A.col_begin;
                 K.ncols = ncols;
-                HIPCHK(lekf_list_cap_launch(K, h->sel_slots, st));
+                HIPCHK(lekf_list_cap_launch(K, std::min(h->sel_slots, lekf_reg_select_slots()), st));
             }
         }
     } else {
@@ -1194,7 +1233,7 @@ int set_obs_global_dev(lekf_handle* h, int nobs, bool have_qc, long h_so, long h
         HIPCHK(h->capl_cnt.ensure(sizeof(int) * npt));
         HIPCHK(h->capl_idx.ensure(sizeof(int) * npt * cap));
         HIPCHK(h->capl_rho.ensure(sizeof(double) * npt * cap));
-        h->sel_slots = lekf_reg_select_slots();
+        h->sel_slots = lekf_reg_select_slots_max();
         h->slot_cap = maxcol;
         const size_t se = (size_t)h->sel_slots * (size_t)h->slot_cap;
         HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * se));
@@ -1290,12 +1329,17 @@ int lekf_das_run(lekf_handle* h)
 {
     if (!h) return LEKF_ERR_ARG;
     if (!h->use_graph) return das_run_eager(h);
+    if (h->graph_ok && h->graph_epoch != g_tune_epoch.load()) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        graph_drop(h);
+    }
     if (h->graph_ok) {
         HIPCHK(hipGraphLaunch(h->gexec, h->st));
         return LEKF_OK;
     }
     // first cycle after a change: run eagerly (this also sizes every work
     // buffer), then capture the same launch sequence for the next cycles
+    const long epoch = g_tune_epoch.load();
     int rc = das_run_eager(h);
     if (rc) return rc;
     hipGraph_t g = nullptr;
@@ -1311,6 +1355,7 @@ int lekf_das_run(lekf_handle* h)
     (void)hipGraphDestroy(g);
     HIPCHK(ei);
     h->graph_ok = true;
+    h->graph_epoch = epoch;
     return LEKF_OK;
 }
 
@@ -1488,7 +1533,7 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     h->loc_cap = maxlist;
     // capped selection: one scratch list per workgroup slot, as long as the
     // largest rectangle's candidate count
-    h->sel_slots = lekf_reg_select_slots();
+    h->sel_slots = lekf_reg_select_slots_max();
     h->slot_cap = h->r.nobs_max > 0 ? std::max(maxcand, 1) : 0;
     if (h->slot_cap > 0) {
         const size_t ne_ = (size_t)h->sel_slots * (size_t)h->slot_cap;

@@ -20,6 +20,7 @@
 // accepted obs are emitted in scan order.
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
+#include "lekf_tune.hpp"
 
 namespace {
 
@@ -666,8 +667,17 @@ hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_
     return hipGetLastError();
 }
 
-// workgroup slots of the persistent selection (LDS allows four per CU)
+// workgroup slots of the persistent selection (LDS allows up to four per CU)
 int lekf_reg_select_slots()
+{
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per = g_lekf_tune.sel_per_cu < 1 ? 1 : (g_lekf_tune.sel_per_cu > 4 ? 4 : g_lekf_tune.sel_per_cu);
+    return per * (ncu > 0 ? ncu : 256);
+}
+
+int lekf_reg_select_slots_max()
 {
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess)

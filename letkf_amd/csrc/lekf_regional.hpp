@@ -79,4 +79,5 @@ hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_
                                     hipStream_t st);
 // persistent selection over npt points with nslots workgroups (scratch slots)
 hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st);
-int lekf_reg_select_slots();
+int lekf_reg_select_slots();      // tuned count
+int lekf_reg_select_slots_max();  // scratch is sized for this many

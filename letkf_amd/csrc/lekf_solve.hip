@@ -26,6 +26,7 @@
 // A point's arithmetic depends only on its own data, so its result is the
 // same whatever else shares the launch.
 #include "lekf_dev.hpp"
+#include "lekf_tune.hpp"
 
 namespace {
 
@@ -605,12 +606,10 @@ __host__ __device__ inline long gram_stride(int kmax)
     return ((long)kmax * kmax + kmax + 2 + 1) & ~1L;
 }
 
-constexpr int kGramWaves = 4;
-
-// Four waves per point; wave w takes every fourth group of obs, and the
-// partial Grams are summed in LDS in wave order (deterministic).
-template <int NT, int MODE>
-__global__ __launch_bounds__(64 * kGramWaves) void lekf_gram_kernel(
+// W waves per point; wave w takes every W-th group of obs, and the partial
+// Grams are summed in LDS in wave order (deterministic for a given W).
+template <int NT, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
     const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride)
 {
     extern __shared__ double smem[];
@@ -626,14 +625,14 @@ __global__ __launch_bounds__(64 * kGramWaves) void lekf_gram_kernel(
     if (p <= 0) return;
     double* A = smem;
     double* B = A + (size_t)k * lda;
-    double* sc = B + k;  // [0] sum w d^2, [4 + w] P3 of wave w
+    double* sc = B + k;  // [0] sum w d^2, [4 + w] P3 of wave w (4 + W slots)
     double p3 = 0.0;
     if (MODE == 0) {
         lekf_d4 acc[NT * (NT + 1) / 2];
 #pragma unroll
         for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
-        gram_f64<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
-        for (int w = 0; w < kGramWaves; ++w) {
+        gram_f64<NT>(J, off, p, k, lane, acc, p3, wv, W);
+        for (int w = 0; w < W; ++w) {
             if (wv == w) store_gram<NT, true>(acc, k, lda, lane, A, B, &sc[0], w > 0);
             __syncthreads();
         }
@@ -642,10 +641,10 @@ __global__ __launch_bounds__(64 * kGramWaves) void lekf_gram_kernel(
 #pragma unroll
         for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
         if (MODE == 1)
-            gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
+            gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, W);
         else
-            gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, kGramWaves);
-        for (int w = 0; w < kGramWaves; ++w) {
+            gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, W);
+        for (int w = 0; w < W; ++w) {
             if (wv == w) store_gram<NT, false>(acc, k, lda, lane, A, B, &sc[0], w > 0);
             __syncthreads();
         }
@@ -654,14 +653,14 @@ __global__ __launch_bounds__(64 * kGramWaves) void lekf_gram_kernel(
     if (lane == 0) sc[4 + wv] = p3;
     __syncthreads();
     double* out = ws + (gp - gp0) * gstride;
-    for (int t = tid; t < k * k; t += 64 * kGramWaves) {
+    for (int t = tid; t < k * k; t += 64 * W) {
         const int i = t / k, j = t - i * k;
         out[t] = A[i * lda + j];
     }
-    for (int i = tid; i < k; i += 64 * kGramWaves) out[(size_t)k * k + i] = B[i];
+    for (int i = tid; i < k; i += 64 * W) out[(size_t)k * k + i] = B[i];
     if (tid == 0) {
         double s = sc[4];
-        for (int w = 1; w < kGramWaves; ++w) s += sc[4 + w];
+        for (int w = 1; w < W; ++w) s += sc[4 + w];
         out[(size_t)k * k + k] = sc[0];
         out[(size_t)k * k + k + 1] = s;
     }
@@ -1311,13 +1310,11 @@ size_t lekf_solve_lds_bytes(int k, int nvmax)
     return eig_lds_doubles(k, nvmax) * sizeof(double);
 }
 
-static constexpr size_t kGramWsCap = 1ull << 30;
-
 size_t lekf_solve_ws_bytes(long npt, int kmax)
 {
     if (kmax <= 32 || npt <= 0) return 0;
     const size_t rec = (size_t)gram_stride(kmax) * sizeof(double);
-    size_t per = kGramWsCap / rec;
+    size_t per = g_lekf_tune.gram_ws_bytes / rec;
     if (per < 1) per = 1;
     if ((size_t)npt < per) per = (size_t)npt;
     return per * rec;
@@ -1367,12 +1364,20 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
     const long gs = gram_stride(kmax);
     const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
     if (per < 1 || !ws) return hipErrorInvalidValue;
-    const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 8) * sizeof(double);
+    const int W = g_lekf_tune.gram_waves;
+    const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 4 + W) * sizeof(double);
     const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
     for (long g0 = 0; g0 < npt; g0 += per) {
         const long nb = per < npt - g0 ? per : npt - g0;
-        hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE>), dim3((unsigned)nb), dim3(64 * kGramWaves),
-                           glds, st, djobs, njobs, g0, (double*)ws, gs);
+        if (W == 2)
+            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 2>), dim3((unsigned)nb), dim3(128), glds,
+                               st, djobs, njobs, g0, (double*)ws, gs);
+        else if (W == 8)
+            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 8>), dim3((unsigned)nb), dim3(512), glds,
+                               st, djobs, njobs, g0, (double*)ws, gs);
+        else
+            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 4>), dim3((unsigned)nb), dim3(256), glds,
+                               st, djobs, njobs, g0, (double*)ws, gs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (const double*)ws, gs, nvmax);

@@ -1,0 +1,12 @@
+// lekf_tune.hpp -- launch parameters chosen at run time (set by the autotuner
+// through lekf_tune_set; process-wide, read when a launch is issued).
+#pragma once
+#include <cstddef>
+
+struct LekfTune {
+    int gram_waves = 4;                 // waves per point of the k > 32 Gram kernel: 2, 4, 8
+    size_t gram_ws_bytes = 1ull << 30;  // Gram workspace per launch (points per sub-batch)
+    int sel_per_cu = 4;                 // selection workgroups (scratch slots) per CU: 1..4
+};
+
+extern LekfTune g_lekf_tune;

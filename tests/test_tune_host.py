@@ -1,0 +1,48 @@
+"""Host side of the launch tuning: range checks, get/set and the JSON cache
+(no kernel runs; the C ABI calls only touch the process-wide tuning table)."""
+import json
+
+import pytest
+
+from letkf_amd import _lib, tune
+
+
+@pytest.fixture(autouse=True)
+def _restore():
+    saved = tune.current()
+    yield
+    tune.apply(saved)
+
+
+def test_defaults_and_roundtrip():
+    cur = tune.current()
+    assert cur["gram_waves"] in (2, 4, 8)
+    assert cur["sel_per_cu"] in (1, 2, 3, 4)
+    tune.set("gram_waves", 8)
+    tune.set("gram_ws_mb", 64)
+    tune.set("sel_per_cu", 3)
+    assert tune.current() == {"gram_waves": 8, "gram_ws_mb": 64, "sel_per_cu": 3}
+
+
+@pytest.mark.parametrize("name,bad", [("gram_waves", 3), ("gram_waves", 16), ("gram_ws_mb", 8),
+                                      ("sel_per_cu", 0), ("sel_per_cu", 5)])
+def test_out_of_range_rejected(name, bad):
+    before = tune.get(name)
+    with pytest.raises(_lib.LetkfError):
+        tune.set(name, bad)
+    assert tune.get(name) == before
+
+
+def test_unknown_param():
+    L = _lib.lib()
+    assert L.lekf_tune_set(7, 1) == _lib.LEKF_ERR_ARG
+    assert L.lekf_tune_get(7) == -_lib.LEKF_ERR_ARG
+
+
+def test_cache_load(tmp_path):
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps({"k": {"params": {"gram_waves": 2, "gram_ws_mb": 256,
+                                                 "sel_per_cu": 1}, "ms": 1.0}}))
+    assert tune.load("missing", str(path)) is None
+    assert tune.load("k", str(path)) == {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
+    assert tune.current() == {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
