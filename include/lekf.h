@@ -61,7 +61,8 @@ extern "C" {
 #define LEKF_PT_NOCONV 2
 
 #define LEKF_NVMAX 16        /* state rows per point in the fused apply */
-#define LEKF_KMAX 95         /* largest ensemble size of the LDS-resident eigensolver */
+#define LEKF_KMAX 256        /* largest ensemble size */
+#define LEKF_KSPLIT 95       /* largest ensemble size of the LDS/register eigensolver */
 
 /*
  * One batched analysis job (device pointers).  The library fills pt_begin.

@@ -44,7 +44,8 @@ LEKF_TUNE_GRAM_WS_MB = 1
 LEKF_TUNE_SEL_PER_CU = 2
 LEKF_PT_NOCONV = 2
 LEKF_NVMAX = 16
-LEKF_KMAX = 95
+LEKF_KMAX = 256
+LEKF_KSPLIT = 95
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int)

@@ -18,11 +18,16 @@
 // Two kernel families, chosen by the ensemble size k:
 //   * k <= 32: one fused kernel, one wave per point; the eigendecomposition is
 //     a one-sided Jacobi with the matrix in registers.
-//   * 32 < k <= LEKF_KMAX: a Gram kernel (four waves per point, MFMA) writes C,
-//     b and the sums to a workspace; a second kernel (one wave per point)
+//   * 32 < k <= LEKF_KSPLIT: a Gram kernel (2-8 waves per point, MFMA) writes
+//     C, b and the sums to a workspace; a second kernel (one wave per point)
 //     reduces C to tridiagonal form by Householder reflections, diagonalises
 //     it by implicit QL with shifts (the method of EISPACK tred2/tql2, the
-//     reference's own eigensolver), and assembles W and the apply.
+//     reference's own eigensolver), and assembles W and the apply, with the
+//     matrix in registers (k <= 64) or LDS.
+//   * LEKF_KSPLIT < k <= LEKF_KMAX: a block Gram kernel (one wave per 64 x 64
+//     block of a point) writes the same workspace record, and the eigen
+//     kernel runs the same steps with the matrix left in that record in
+//     global memory (a k = 256 FP64 matrix is 512 KB, beyond the LDS).
 // A point's arithmetic depends only on its own data, so its result is the
 // same whatever else shares the launch.
 #include "lekf_dev.hpp"
@@ -666,6 +671,232 @@ __global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block Gram for LEKF_KSPLIT < k <= LEKF_KMAX: the augmented (k+1)^2 Gram is
+// cut into 64 x 64 blocks; one wave computes one upper block (bi <= bj) of one
+// point over all of its obs, as NB x NB MFMA tiles, and writes it to the
+// point's workspace record (both triangles of C, b and sum w d^2).
+// ---------------------------------------------------------------------------
+constexpr int kBlkNB = 4;  // 16 x 16 tiles per block side
+
+__host__ __device__ inline int gram_nblk(int kmax)
+{
+    const int nb = (kmax + 1 + 16 * kBlkNB - 1) / (16 * kBlkNB);
+    return nb * (nb + 1) / 2;
+}
+
+// block pair (bi, bj), bi <= bj, of upper-block index t (row-major over the
+// upper triangle)
+__device__ __forceinline__ void blk_pair(int t, int nb, int& bi, int& bj)
+{
+    int i = 0;
+    while (t >= nb - i) {
+        t -= nb - i;
+        ++i;
+    }
+    bi = i;
+    bj = i + t;
+}
+
+__device__ __forceinline__ void blk_store(double* out, int k, int grow, int gcol, double val)
+{
+    if (grow > gcol || gcol > k) return;
+    if (gcol < k) {
+        out[(size_t)grow * k + gcol] = val;
+        out[(size_t)gcol * k + grow] = val;
+    } else if (grow < k) {
+        out[(size_t)k * k + grow] = val;
+    } else {
+        out[(size_t)k * k + k] = val;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void lekf_gram_blk_kernel(const LekfJob* __restrict__ jobs,
+                                                           int njobs, long gp0,
+                                                           double* __restrict__ ws, long gstride)
+{
+    constexpr int NB = kBlkNB;
+    const long gp = gp0 + blockIdx.x;
+    const int lane = threadIdx.x;
+    const int ji = lekf_job_of(jobs, njobs, gp);
+    const LekfJob& J = jobs[ji];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int k = J.k;
+    const int nbk = (k + 1 + 16 * NB - 1) / (16 * NB);
+    if ((int)blockIdx.y >= nbk * (nbk + 1) / 2) return;  // block of a larger k of the launch
+    int bi, bj;
+    blk_pair((int)blockIdx.y, nbk, bi, bj);
+    const int cI = 16 * NB * bi, cJ = 16 * NB * bj;
+    const long off = J.loc_off[b];
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
+    if (p <= 0) return;
+    double* out = ws + (gp - gp0) * gstride;
+    const int lr = lane & 15, kq = lane >> 4;
+    const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
+    double p3 = 0.0;
+    if (MODE == 0) {
+        lekf_d4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_d4{0.0, 0.0, 0.0, 0.0};
+        for (int q0 = 0; q0 < p; q0 += 4) {
+            const int q = q0 + kq;
+            const bool valid = q < p;
+            double w = 0.0, dd = 0.0;
+            long o = 0;
+            if (valid) {
+                o = J.loc_idx[off + q];
+                const double rho = J.loc_rho[off + q];
+                const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
+                w = (wloc ? 1.0 : rho) / rd;
+                dd = J.dep[o];
+                if (lr == 0) p3 += rho;
+            }
+            double xr[NB], xc[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const int c1 = cI + 16 * t + lr, c2 = cJ + 16 * t + lr;
+                double a1 = 0.0, a2 = 0.0;
+                if (valid) {
+                    a1 = c1 < k ? J.hdxf[o * J.y_so + (long)c1 * J.y_sm] : (c1 == k ? dd : 0.0);
+                    a2 = c2 < k ? J.hdxf[o * J.y_so + (long)c2 * J.y_sm] : (c2 == k ? dd : 0.0);
+                }
+                xr[t] = w * a1;
+                xc[t] = a2;
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt)
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[I], xc[Jt], acc[I][Jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    blk_store(out, k, cI + 16 * I + kq + 4 * r, cJ + 16 * Jt + lr, acc[I][Jt][r]);
+    } else if (MODE == 1) {
+        lekf_f4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+        for (int q0 = 0; q0 < p; q0 += 4) {
+            const int q = q0 + kq;
+            const bool valid = q < p;
+            float w = 0.0f, dd = 0.0f;
+            long o = 0;
+            if (valid) {
+                o = J.loc_idx[off + q];
+                const double rho = J.loc_rho[off + q];
+                const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
+                w = (float)((wloc ? 1.0 : rho) / rd);
+                dd = (float)J.dep[o];
+                if (lr == 0) p3 += rho;
+            }
+            float xr[NB], xc[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const int c1 = cI + 16 * t + lr, c2 = cJ + 16 * t + lr;
+                float a1 = 0.0f, a2 = 0.0f;
+                if (valid) {
+                    a1 = c1 < k ? J.hdxf32[o * (long)k + c1] : (c1 == k ? dd : 0.0f);
+                    a2 = c2 < k ? J.hdxf32[o * (long)k + c2] : (c2 == k ? dd : 0.0f);
+                }
+                xr[t] = w * a1;
+                xc[t] = a2;
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt)
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[I], xc[Jt], acc[I][Jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    blk_store(out, k, cI + 16 * I + 4 * kq + r, cJ + 16 * Jt + lr,
+                              (double)acc[I][Jt][r]);
+    } else {
+        lekf_f4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+        for (int q0 = 0; q0 < p; q0 += 32) {
+            float wv8[8], dv[8];
+            long ov[8];
+            bool vv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int q = q0 + 8 * kq + e;
+                vv[e] = q < p;
+                wv8[e] = 0.0f;
+                dv[e] = 0.0f;
+                ov[e] = 0;
+                if (vv[e]) {
+                    const long o = J.loc_idx[off + q];
+                    const double rho = J.loc_rho[off + q];
+                    const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
+                    wv8[e] = (float)((wloc ? 1.0 : rho) / rd);
+                    dv[e] = (float)J.dep[o];
+                    ov[e] = o;
+                    if (lr == 0) p3 += rho;
+                }
+            }
+            lekf_bf8 ah[NB], al[NB], bh[NB], bl[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const int c1 = cI + 16 * t + lr, c2 = cJ + 16 * t + lr;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    float a1 = 0.0f, a2 = 0.0f;
+                    if (vv[e]) {
+                        a1 = c1 < k ? J.hdxf32[ov[e] * (long)k + c1] : (c1 == k ? dv[e] : 0.0f);
+                        a2 = c2 < k ? J.hdxf32[ov[e] * (long)k + c2] : (c2 == k ? dv[e] : 0.0f);
+                    }
+                    const float xa = wv8[e] * a1;
+                    const __bf16 ah1 = (__bf16)xa;
+                    const __bf16 al1 = (__bf16)(xa - (float)ah1);
+                    const __bf16 bh1 = (__bf16)a2;
+                    const __bf16 bl1 = (__bf16)(a2 - (float)bh1);
+                    ah[t][e] = ah1;
+                    al[t][e] = al1;
+                    bh[t][e] = bh1;
+                    bl[t][e] = bl1;
+                }
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt) {
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[I], bh[Jt], acc[I][Jt], 0, 0, 0);
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bl[Jt], acc[I][Jt], 0, 0, 0);
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bh[Jt], acc[I][Jt], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    blk_store(out, k, cI + 16 * I + 4 * kq + r, cJ + 16 * Jt + lr,
+                              (double)acc[I][Jt][r]);
+    }
+    if (blockIdx.y == 0) {
+        p3 = wave_sum(p3);
+        if (lane == 0) out[(size_t)k * k + k + 1] = p3;
+    }
+}
+
 // LDS of the eigen/assembly kernel (doubles)
 struct EigLds {
     double* Z;    // k x lda: C, then Householder vectors + Q, then eigenvectors (columns)
@@ -690,11 +921,22 @@ __host__ __device__ inline size_t eig_lds_doubles(int k, int nvmax)
     return (size_t)k * eig_lda(k) + 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 2 * (size_t)nvmax;
 }
 
-__device__ inline EigLds eig_carve(double* s, int k, int nvmax)
+// Z in LDS (zg == nullptr) or, for the large-k kernel, the point's workspace
+// record in global memory (zg)
+__host__ __device__ inline size_t eig_lds_doubles_big(int k, int nvmax)
+{
+    return 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 2 * (size_t)nvmax;
+}
+
+__device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nullptr)
 {
     EigLds L;
-    L.Z = s;
-    s += (size_t)k * eig_lda(k);
+    if (zg) {
+        L.Z = zg;
+    } else {
+        L.Z = s;
+        s += (size_t)k * eig_lda(k);
+    }
     L.d = s;
     s += k;
     L.e = s;
@@ -843,7 +1085,207 @@ __device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
     __syncthreads();
 }
 
+// The same reduction for n <= 64 * NR with the matrix in global memory
+// (row-major, leading dimension lda, exactly symmetric).  Lane r owns
+// columns r, r + 64, ... of the trailing block.  The reflector of step j is
+// read from and kept in row j (equal to column j by symmetry), so every
+// access is a contiguous row segment.  Q is accumulated as above and then
+// transposed in place: on return Z[r + c * lda] = Q[r][c], the layout the
+// lane-per-row QL sweeps read contiguously.
+template <int NR>
+__device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, double* tauv, int lane)
+{
+    for (int j = 0; j + 2 < n; ++j) {
+        const int o = j + 1, m = n - o;
+        double* rowj = A + (size_t)j * lda + o;
+        double x[NR];
+        double s2 = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = lane + 64 * t;
+            x[t] = r < m ? rowj[r] : 0.0;
+            if (r > 0) s2 = fma(x[t], x[t], s2);
+        }
+        const double sigma = wave_sum(s2);
+        const double alpha = lane_bcast(x[0], 0);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (sigma > 0.0) {
+            const double nrm = sqrt(fma(alpha, alpha, sigma));
+            beta = alpha <= 0.0 ? nrm : -nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        double v[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = lane + 64 * t;
+            v[t] = r == 0 ? 1.0 : (tau == 0.0 ? 0.0 : x[t] * scal);
+        }
+        if (lane == 0) {
+            d[j] = A[(size_t)j * lda + j];
+            e[j] = beta;
+            tauv[j] = tau;
+        }
+#pragma unroll
+        for (int t = 0; t < NR; ++t)
+            if (lane + 64 * t < m) rowj[lane + 64 * t] = v[t];
+        __syncthreads();
+        if (tau == 0.0) continue;
+        double pp[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) pp[t] = 0.0;
+#pragma unroll
+        for (int cb = 0; cb < NR; ++cb) {
+            const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
+            if (cn <= 0) break;
+#pragma unroll 4
+            for (int cc = 0; cc < cn; ++cc) {
+                const double vc = lane_bcast(v[cb], cc);
+                const double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+#pragma unroll
+                for (int t = 0; t < NR; ++t)
+                    if (lane + 64 * t < m) pp[t] = fma(row[lane + 64 * t], vc, pp[t]);
+            }
+        }
+        double pv = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            pp[t] *= tau;
+            pv = fma(pp[t], v[t], pv);
+        }
+        const double K = 0.5 * tau * wave_sum(pv);
+        double w[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) w[t] = fma(-K, v[t], pp[t]);
+#pragma unroll
+        for (int cb = 0; cb < NR; ++cb) {
+            const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
+            if (cn <= 0) break;
+#pragma unroll 4
+            for (int cc = 0; cc < cn; ++cc) {
+                const double vc = lane_bcast(v[cb], cc);
+                const double wc = lane_bcast(w[cb], cc);
+                double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+#pragma unroll
+                for (int t = 0; t < NR; ++t)
+                    if (lane + 64 * t < m) {
+                        const int r = lane + 64 * t;
+                        row[r] = __dsub_rn(row[r],
+                                           __dadd_rn(__dmul_rn(vc, w[t]), __dmul_rn(wc, v[t])));
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        d[n - 2] = A[(size_t)(n - 2) * lda + n - 2];
+        e[n - 2] = A[(size_t)(n - 1) * lda + n - 2];
+        d[n - 1] = A[(size_t)(n - 1) * lda + n - 1];
+        e[n - 1] = 0.0;
+    }
+    __syncthreads();
+    // Q = H_0 ... H_{n-3} from the right end, as in eig_tridiag; the reflector
+    // of step j is in row j, columns j+1..
+    for (int r = lane; r < 4; r += 64) {
+        const int i = r >> 1, c = r & 1;
+        A[(size_t)(n - 2 + i) * lda + n - 2 + c] = i == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int j = n - 3; j >= 0; --j) {
+        const int o = j + 1, m = n - o;
+        for (int r = lane; r < m; r += 64) {
+            A[(size_t)o * lda + o + r] = r == 0 ? 1.0 : 0.0;
+            A[(size_t)(o + r) * lda + o] = r == 0 ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        const double tau = tauv[j];
+        if (tau == 0.0) continue;
+        double v[NR], y[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = lane + 64 * t;
+            v[t] = r < m ? A[(size_t)j * lda + o + r] : 0.0;
+            y[t] = 0.0;
+        }
+#pragma unroll
+        for (int cb = 0; cb < NR; ++cb) {
+            const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
+            if (cn <= 0) break;
+#pragma unroll 4
+            for (int cc = 0; cc < cn; ++cc) {
+                const double vc = lane_bcast(v[cb], cc);
+                const double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+#pragma unroll
+                for (int t = 0; t < NR; ++t)
+                    if (lane + 64 * t < m) y[t] = fma(vc, row[lane + 64 * t], y[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NR; ++t) y[t] *= tau;
+#pragma unroll
+        for (int cb = 0; cb < NR; ++cb) {
+            const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
+            if (cn <= 0) break;
+#pragma unroll 4
+            for (int cc = 0; cc < cn; ++cc) {
+                const double vc = lane_bcast(v[cb], cc);
+                double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+#pragma unroll
+                for (int t = 0; t < NR; ++t)
+                    if (lane + 64 * t < m) row[lane + 64 * t] = fma(-vc, y[t], row[lane + 64 * t]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int r = lane; r < n; r += 64) {
+        A[r] = r == 0 ? 1.0 : 0.0;
+        A[(size_t)r * lda] = r == 0 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // in-place transpose: row i, columns i+1.. swap with column i
+    for (int i = 0; i + 1 < n; ++i) {
+        for (int c = i + 1 + lane; c < n; c += 64) {
+            const double a = A[(size_t)i * lda + c], bb = A[(size_t)c * lda + i];
+            A[(size_t)i * lda + c] = bb;
+            A[(size_t)c * lda + i] = a;
+        }
+    }
+    __syncthreads();
+}
+
 constexpr int kQlMaxIter = 60;
+
+// ql_apply for Z in global memory with Z[r + i * lda] = Z_ri (rows of a
+// column contiguous), NR rows per lane
+template <int NR>
+__device__ __forceinline__ void ql_apply_cm(EigLds& L, int n, int lda, int lane, int ilo, int m)
+{
+    double z1[NR];
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+        const int r = lane + 64 * t;
+        z1[t] = r < n ? L.Z[r + (size_t)m * lda] : 0.0;
+    }
+    for (int i = m - 1; i >= ilo; --i) {
+        const double c = L.rot[2 * i], s = L.rot[2 * i + 1];
+        const double* zi = L.Z + (size_t)i * lda;
+        double* zi1 = L.Z + (size_t)(i + 1) * lda;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = lane + 64 * t;
+            if (r < n) {
+                const double z0 = zi[r];
+                zi1[r] = fma(s, z0, c * z1[t]);
+                z1[t] = fma(c, z0, -s * z1[t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+        const int r = lane + 64 * t;
+        if (r < n) L.Z[r + (size_t)ilo * lda] = z1[t];
+    }
+}
 
 // Rotations (c_i, s_i), i = m-1 down to ilo, of one QL sweep applied to the
 // columns (i, i+1) of Z, row by row (lane = row): the new z_{i+1} is stored,
@@ -871,7 +1313,9 @@ __device__ __forceinline__ void ql_apply(EigLds& L, int n, int lda, int lane, in
 // estimate tst = max(|d_l| + |e_l|) does not change it.  Every lane runs
 // the scalar recurrence; the rotations of a sweep are kept in L.rot and
 // then applied to Z by all lanes.  Returns nonzero if an eigenvalue needed
-// more than kQlMaxIter sweeps.
+// more than kQlMaxIter sweeps.  CM > 0: Z in global memory, column-major
+// (ql_apply_cm with CM rows per lane).
+template <int CM = 0>
 __device__ int eig_ql(EigLds& L, int n, int lda, int lane)
 {
     double* d = L.d;
@@ -922,7 +1366,10 @@ __device__ int eig_ql(EigLds& L, int n, int lda, int lane)
                 di1 = di;
             }
             __syncthreads();
-            ql_apply(L, n, lda, lane, split ? i + 1 : l, m);
+            if constexpr (CM > 0)
+                ql_apply_cm<CM>(L, n, lda, lane, split ? i + 1 : l, m);
+            else
+                ql_apply(L, n, lda, lane, split ? i + 1 : l, m);
             if (!split) {
                 d[l] = d[l] - p;
                 e[l] = g;
@@ -1111,13 +1558,14 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
 }
 
 // One wave per point: C and sums from the Gram workspace, eigendecomposition
-// (KR > 0: in registers for k <= KR; KR = 0: in LDS), W assembly, inflation,
+// (KR > 0: in registers for k <= KR; KR = 0: in LDS; BIG > 0: in the point's
+// workspace record in global memory, BIG rows per lane), W assembly, inflation,
 // the optional trans/transm/pao outputs and the fused apply
 // Xa = x-mean + X' T evaluated as x-mean + (X' V) diag(sqrt((k-1)/L)) V^T
 // + (X' w-bar) 1^T.
-template <int KR>
+template <int KR, int BIG = 0>
 __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
-                                                      long gp0, const double* __restrict__ ws,
+                                                      long gp0, double* __restrict__ ws,
                                                       long gstride, int nvmax)
 {
     extern __shared__ double smem[];
@@ -1126,8 +1574,13 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
     const int ji = lekf_job_of(jobs, njobs, gp);
     const LekfJob& J = jobs[ji];
     const int b = (int)(gp - J.pt_begin + J.pt0);
-    const int k = J.k, lda = eig_lda(k);
-    EigLds L = eig_carve(smem, k, nvmax);
+    const int k = J.k, lda = BIG ? k : eig_lda(k);
+    double* const rec = ws + (gp - gp0) * gstride;
+    EigLds L = eig_carve(smem, k, nvmax, BIG ? rec : nullptr);
+    // Z_ij (component i of eigenvector j once the eigendecomposition is done)
+    auto Zv = [&](int i, int j) -> double {
+        return BIG ? L.Z[i + (size_t)j * lda] : L.Z[i * lda + j];
+    };
     const long off = J.loc_off[b];
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
     double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
@@ -1136,11 +1589,12 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
     const double sa = sqrt(alpha);  // p = 0 branch: T = sqrt(alpha) I
 
     if (p > 0) {
-        const double* g = ws + (gp - gp0) * gstride;
-        for (int t = lane; t < k * k; t += 64) {
-            const int i = t / k, j = t - i * k;
-            L.Z[i * lda + j] = g[t];
-        }
+        const double* g = rec;
+        if (!BIG)
+            for (int t = lane; t < k * k; t += 64) {
+                const int i = t / k, j = t - i * k;
+                L.Z[i * lda + j] = g[t];
+            }
         for (int i = lane; i < k; i += 64) L.B[i] = g[(size_t)k * k + i];
         const double p1 = g[(size_t)k * k + k], p3 = g[(size_t)k * k + k + 1];
         __syncthreads();
@@ -1153,7 +1607,10 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
         __syncthreads();
         for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += (double)(k - 1) * rho;
         __syncthreads();
-        if constexpr (KR > 0) {
+        if constexpr (BIG > 0) {
+            big_tridiag<(BIG > 0 ? BIG : 1)>(L.Z, k, lda, L.d, L.e, L.tau, lane);
+            if (eig_ql<BIG>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
+        } else if constexpr (KR > 0) {
             if (eig_reg<(KR > 0 ? KR : 4)>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
         } else {
             eig_tridiag(L, k, lda, lane);
@@ -1171,13 +1628,13 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
         // u = L^-1 V^T b ; w-bar = V u ; U <- sqrt((k-1)/lambda)
         for (int j = lane; j < k; j += 64) {
             double s = 0.0;
-            for (int i = 0; i < k; ++i) s = fma(L.Z[i * lda + j], L.B[i], s);
+            for (int i = 0; i < k; ++i) s = fma(Zv(i, j), L.B[i], s);
             L.U[j] = s / L.d[j];
         }
         __syncthreads();
         for (int i = lane; i < k; i += 64) {
             double s = 0.0;
-            for (int j = 0; j < k; ++j) s = fma(L.Z[i * lda + j], L.U[j], s);
+            for (int j = 0; j < k; ++j) s = fma(Zv(i, j), L.U[j], s);
             L.Wb[i] = s;
         }
         __syncthreads();
@@ -1188,7 +1645,7 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
             for (int w = lane; w < k * k; w += 64) {
                 const int m = w / k, i = w - m * k;  // column-major, coalesced over i
                 double s = 0.0;
-                for (int l = 0; l < k; ++l) s = fma(L.Z[i * lda + l] * L.U[l], L.Z[m * lda + l], s);
+                for (int l = 0; l < k; ++l) s = fma(Zv(i, l) * L.U[l], Zv(m, l), s);
                 if (!transm_mode) s += L.Wb[i];
                 tr[i + (size_t)m * k] = s;
             }
@@ -1198,7 +1655,7 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
             for (int w = lane; w < k * k; w += 64) {
                 const int m = w / k, i = w - m * k;
                 double s = 0.0;
-                for (int l = 0; l < k; ++l) s = fma(L.Z[i * lda + l] / L.d[l], L.Z[m * lda + l], s);
+                for (int l = 0; l < k; ++l) s = fma(Zv(i, l) / L.d[l], Zv(m, l), s);
                 pa[i + (size_t)m * k] = s;
             }
         }
@@ -1256,7 +1713,7 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
         for (int w = lane; w < nrow * k; w += 64) {
             const int r = w / k, l = w - r * k;
             double s = 0.0;
-            for (int j = 0; j < k; ++j) s = fma(L.X[r * k + j], L.Z[j * lda + l], s);
+            for (int j = 0; j < k; ++j) s = fma(L.X[r * k + j], Zv(j, l), s);
             L.R[r * k + l] = s * L.U[l];
         }
         for (int r = lane; r < nrow; r += 64) {
@@ -1274,7 +1731,7 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
             a = L.M[r] + L.X[r * k + m];
         } else if (p > 0) {
             double s = 0.0;
-            for (int l = 0; l < k; ++l) s = fma(L.R[r * k + l], L.Z[m * lda + l], s);
+            for (int l = 0; l < k; ++l) s = fma(L.R[r * k + l], Zv(m, l), s);
             a = L.M[r] + (L.Cw[r] + s);
         } else {
             a = L.M[r] + sa * L.X[r * k + m];
@@ -1293,7 +1750,10 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
 int lekf_solve_nt(int k) { return (k + 1 + 15) / 16; }
 
 // Kernel class of an ensemble size: KR * 8 + NT for k <= 32 (register
-// Jacobi, KR = k rounded up to a multiple of 4); NT for larger k.
+// Jacobi, KR = k rounded up to a multiple of 4); NT for k <= LEKF_KSPLIT;
+// kClassBig above (block Gram + eigensolver in global memory).
+constexpr int kClassBig = 1000;
+
 int lekf_solve_class(int k)
 {
     if (k <= 32) {
@@ -1301,12 +1761,14 @@ int lekf_solve_class(int k)
         const int nt = kr <= 12 ? 1 : (kr <= 28 ? 2 : 3);
         return kr * 8 + nt;
     }
+    if (k > LEKF_KSPLIT) return kClassBig;
     return lekf_solve_nt(k);
 }
 
 size_t lekf_solve_lds_bytes(int k, int nvmax)
 {
     if (k <= 32) return solve_lds_doubles(k) * sizeof(double);
+    if (k > LEKF_KSPLIT) return eig_lds_doubles_big(k, nvmax) * sizeof(double);
     return eig_lds_doubles(k, nvmax) * sizeof(double);
 }
 
@@ -1334,7 +1796,7 @@ static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int km
 static int eig_kr(int kmax) { return kmax <= 64 ? (kmax + 3) & ~3 : 0; }
 
 static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const LekfJob* djobs,
-                             int njobs, long g0, const double* ws, long gs, int nvmax)
+                             int njobs, long g0, double* ws, long gs, int nvmax)
 {
     switch (kr) {
 #define LEKF_EIG_CASE(K)                                                                       \
@@ -1380,7 +1842,33 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
                                st, djobs, njobs, g0, (double*)ws, gs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (const double*)ws, gs, nvmax);
+        e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs, nvmax);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// LEKF_KSPLIT < k <= LEKF_KMAX: block Gram (one wave per 64 x 64 block of a
+// point), then the eigen/assembly kernel working in place on each point's
+// workspace record
+template <int MODE>
+static hipError_t launch_big(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
+                             void* ws, size_t ws_bytes, hipStream_t st)
+{
+    const long gs = gram_stride(kmax);
+    const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
+    if (per < 1 || !ws) return hipErrorInvalidValue;
+    const size_t elds = eig_lds_doubles_big(kmax, nvmax) * sizeof(double);
+    const int nblk = gram_nblk(kmax);
+    for (long g0 = 0; g0 < npt; g0 += per) {
+        const long nb = per < npt - g0 ? per : npt - g0;
+        hipLaunchKernelGGL((lekf_gram_blk_kernel<MODE>), dim3((unsigned)nb, (unsigned)nblk),
+                           dim3(LEKF_WAVE), 0, st, djobs, njobs, g0, (double*)ws, gs);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((lekf_eig_kernel<0, 4>), dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,
+                           djobs, njobs, g0, (double*)ws, gs, nvmax);
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -1403,6 +1891,7 @@ static hipError_t launch_mode(int cls, const LekfJob* djobs, int njobs, long npt
     case 4: return launch_split<4, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     case 5: return launch_split<5, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     case 6: return launch_split<6, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
+    case kClassBig: return launch_big<MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     default: return hipErrorInvalidValue;
     }
 }

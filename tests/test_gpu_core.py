@@ -29,7 +29,9 @@ def _rel(a, b):
 CASES = [(0, 20), (1, 20), (4, 20), (21, 20), (200, 20), (57, 3), (13, 5), (100, 16), (100, 31),
          (100, 32), (200, 40), (1000, 50), (300, 63), (200, 64), (150, 80), (120, 95),
          # p < k: C has k - p equal eigenvalues (k-1)/alpha (clustered spectrum)
-         (0, 40), (1, 33), (5, 50), (30, 64), (10, 95)]
+         (0, 40), (1, 33), (5, 50), (30, 64), (10, 95),
+         # large k: block Gram + eigensolver in global memory
+         (200, 96), (300, 128), (150, 200), (400, 256), (1, 130), (0, 256), (60, 256)]
 
 
 @pytest.mark.parametrize("p,k", CASES)
@@ -45,7 +47,7 @@ def test_core_matches_reference(gpu, p, k):
 @pytest.mark.parametrize("opts", [dict(transm=True), dict(pao=True), dict(transm=True, pao=True),
                                   dict(rdiag_wloc=True), dict(minfl=1.2),
                                   dict(transm=True, pao=True, rdiag_wloc=True, minfl=1.1)])
-@pytest.mark.parametrize("p,k", [(0, 20), (37, 20), (250, 50)])
+@pytest.mark.parametrize("p,k", [(0, 20), (37, 20), (250, 50), (180, 140)])
 def test_core_options(gpu, opts, p, k):
     h, rd, rl, dp = synth.local_problem(p, k, seed=77 + p + k)
     if opts.get("rdiag_wloc"):
@@ -81,7 +83,7 @@ def test_bad_sizes_raise(gpu):
     assert e.value.code == 99
 
 
-@pytest.mark.parametrize("k", [20, 50])
+@pytest.mark.parametrize("k", [20, 50, 160])
 def test_batch_matches_single_and_reference(gpu, k):
     """Many uneven local problems in one launch; each equals its lone solve bit for bit."""
     from letkf_amd.device import DeviceArray, sync
