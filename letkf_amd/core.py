@@ -105,11 +105,18 @@ def make_job(*, k, npt, hdxf, rdiag, dep, loc_off, loc_idx, loc_rho, loc_cnt=Non
         j.x2a = _ptr(apply.get("x2a"))
         j.upd_mask = _ptr(apply.get("upd_mask"))
     j.flags = flags
+    # the descriptor holds raw device pointers: keep the buffers alive with it
+    j._keep = [hdxf, rdiag, dep, loc_off, loc_idx, loc_rho, loc_cnt, loc_rdiag, infl_in,
+               infl_out, trans, transm, pao, status, hdxf32, apply]
     return j
 
 
 def letkf_core_batch(jobs, prec="f64", stream=None):
-    """Launch the batched solve over a list of ``lekf_job`` (asynchronous)."""
+    """Launch the batched solve over a list of ``lekf_job`` (asynchronous).
+
+    The kernels read the jobs' buffers after this returns: keep the job
+    objects (which hold their buffers) alive until the stream is synchronised.
+    """
     L = _lib.lib()
     arr = (LekfJob * len(jobs))(*jobs)
     st = L.lekf_core_batch(arr, len(jobs), _lib.PREC[prec] if isinstance(prec, str) else prec,

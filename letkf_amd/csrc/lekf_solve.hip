@@ -909,7 +909,7 @@ struct EigLds {
     double* Wb;   // k  w-bar
     double* sc;   // 8
     double* X;    // nvmax * k  perturbation rows
-    double* R;    // nvmax * k
+    double* R;    // nvmax rows of k (k + 1 apart in the large-k vector path)
     double* M;    // nvmax  row means
     double* Cw;   // nvmax  X' w-bar
 };
@@ -918,14 +918,14 @@ __host__ __device__ inline int eig_lda(int k) { return k | 1; }
 
 __host__ __device__ inline size_t eig_lds_doubles(int k, int nvmax)
 {
-    return (size_t)k * eig_lda(k) + 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 2 * (size_t)nvmax;
+    return (size_t)k * eig_lda(k) + 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
 }
 
 // Z in LDS (zg == nullptr) or, for the large-k kernel, the point's workspace
 // record in global memory (zg)
 __host__ __device__ inline size_t eig_lds_doubles_big(int k, int nvmax)
 {
-    return 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 2 * (size_t)nvmax;
+    return 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
 }
 
 __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nullptr)
@@ -956,7 +956,7 @@ __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nul
     L.X = s;
     s += (size_t)nvmax * k;
     L.R = s;
-    s += (size_t)nvmax * k;
+    s += (size_t)nvmax * (k + 1);
     L.M = s;
     s += nvmax;
     L.Cw = s;
@@ -1092,7 +1092,7 @@ __device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
 // access is a contiguous row segment.  Q is accumulated as above and then
 // transposed in place: on return Z[r + c * lda] = Q[r][c], the layout the
 // lane-per-row QL sweeps read contiguously.
-template <int NR>
+template <int NR, bool FORMQ = true>
 __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, double* tauv, int lane)
 {
     for (int j = 0; j + 2 < n; ++j) {
@@ -1184,6 +1184,7 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
         e[n - 1] = 0.0;
     }
     __syncthreads();
+    if (!FORMQ) return;
     // Q = H_0 ... H_{n-3} from the right end, as in eig_tridiag; the reflector
     // of step j is in row j, columns j+1..
     for (int r = lane; r < 4; r += 64) {
@@ -1557,6 +1558,301 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
     return bad;
 }
 
+// ---------------------------------------------------------------------------
+// Large-k path without the trans/pao outputs: the eigenvectors V = Q S are
+// never formed.  The analysis needs V only through V^T y and V c for a few
+// vectors (b, the point's perturbation rows, w-bar), so those vectors are
+// carried through Q^T (the reflectors) and through each QL sweep's rotations
+// as the sweeps are computed (S^T), and the rotations are kept in a buffer
+// after the point's workspace record to be replayed in reverse (S) before Q is
+// applied.  Traffic per point drops from O(k^3) matrix sweeps to the
+// reduction itself plus O(k^2) per vector.
+// ---------------------------------------------------------------------------
+
+// rotation buffer: (c, s) pairs of up to 2k^2 + 4k rotations plus one
+// (ilo, m) trailer per sweep
+__host__ __device__ inline long big_rot_cap(int kmax) { return 2 * (2L * kmax * kmax + 4L * kmax); }
+__host__ __device__ inline long big_stride(int kmax) { return gram_stride(kmax) + big_rot_cap(kmax); }
+
+// the vectors carried along: nrow rows of R (k apart), then v1, then v2
+struct VecSet {
+    double* R;
+    int nrow, k;
+    double* v1;
+    double* v2;
+    __device__ double* at(int q) const
+    {
+        return q < nrow ? R + (size_t)q * k : (q == nrow ? v1 : v2);
+    }
+};
+
+// y <- Q^T y (transpose: H_0 first) or y <- Q y (H_{n-3} first) for nvec
+// vectors, H_j = I - tau_j v_j v_j^T with v_j in row j, columns j+1.., of A.
+// Lane r owns elements r, r + 64, ... of every vector.
+template <int NR>
+__device__ void big_refl_vecs(const double* A, int n, int lda, const double* tauv, const VecSet& V,
+                              int nvec, bool transpose, int lane)
+{
+    for (int s = 0; s + 2 < n; ++s) {
+        const int j = transpose ? s : n - 3 - s;
+        const double tau = tauv[j];
+        if (tau == 0.0) continue;
+        const int o = j + 1, m = n - o;
+        double v[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = lane + 64 * t;
+            v[t] = r < m ? A[(size_t)j * lda + o + r] : 0.0;
+        }
+        for (int q = 0; q < nvec; ++q) {
+            double* y = V.at(q) + o;
+            double sum = 0.0;
+#pragma unroll
+            for (int t = 0; t < NR; ++t)
+                if (lane + 64 * t < m) sum = fma(v[t], y[lane + 64 * t], sum);
+            sum = tau * wave_sum(sum);
+#pragma unroll
+            for (int t = 0; t < NR; ++t)
+                if (lane + 64 * t < m) y[lane + 64 * t] = fma(-sum, v[t], y[lane + 64 * t]);
+        }
+        // element o + r belongs to lane r, so ownership shifts by one lane per
+        // reflector: order this step's LDS stores before the next step's loads
+        __syncthreads();
+    }
+}
+
+// eig_ql on (d, e) without eigenvectors: each sweep's rotations are applied
+// to the nvec vectors (lane q owns vector q; the same update as a row of Z,
+// so u = Z^T y is maintained) and appended to rbuf.  Returns nonzero if an
+// eigenvalue needed more than kQlMaxIter sweeps; ovf is set if rbuf filled.
+__device__ int eig_ql_vec(EigLds& L, int n, int lane, const VecSet& V, int nvec, double* rbuf,
+                          long rcap, long& rpos, bool& ovf)
+{
+    double* d = L.d;
+    double* e = L.e;
+    double tst = 0.0;
+    for (int l = 0; l < n; ++l) {
+        tst = fmax(tst, fabs(d[l]) + fabs(e[l]));
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m < n - 1; ++m)
+                if (tst + fabs(e[m]) == tst) break;
+            if (m == l) break;
+            if (iter++ == kQlMaxIter) return 1;
+            const double dl = d[l], el = e[l];
+            double g = (d[l + 1] - dl) / (2.0 * el);
+            double r = fabs(g) > 1.0e100 ? fabs(g) : sqrt(fma(g, g, 1.0));
+            g = d[m] - dl + el / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = m - 1;
+            bool split = false;
+            double ei = e[i], di1 = d[i + 1];
+            for (; i >= l; --i) {
+                const double di = d[i];
+                const double eprev = i > l ? e[i - 1] : 0.0;
+                const double f = s * ei, bb = c * ei;
+                r = sqrt(fma(f, f, g * g));
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = di1 - p;
+                    e[m] = 0.0;
+                    split = true;
+                    break;
+                }
+                const double ri = rcp_nr(r);
+                s = f * ri;
+                c = g * ri;
+                g = di1 - p;
+                r = fma(di - g, s, 2.0 * c * bb);
+                p = s * r;
+                d[i + 1] = g + p;
+                g = fma(c, r, -bb);
+                if (lane == 0) {
+                    L.rot[2 * i] = c;
+                    L.rot[2 * i + 1] = s;
+                }
+                ei = eprev;
+                di1 = di;
+            }
+            __syncthreads();
+            const int ilo = split ? i + 1 : l;
+            const int cnt = m - ilo;
+            if (!ovf && rpos + 2L * cnt + 2 <= rcap) {
+                for (int q = lane; q < 2 * cnt; q += 64)
+                    rbuf[rpos + q] = L.rot[2 * (m - 1 - (q >> 1)) + (q & 1)];
+                if (lane == 0) {
+                    rbuf[rpos + 2L * cnt] = (double)ilo;
+                    rbuf[rpos + 2L * cnt + 1] = (double)m;
+                }
+                rpos += 2L * cnt + 2;
+            } else {
+                ovf = true;
+            }
+            if (lane < nvec) {
+                double* z = V.at(lane);
+                double z1 = z[m];
+                for (int ii = m - 1; ii >= ilo; --ii) {
+                    const double cc = L.rot[2 * ii], ss = L.rot[2 * ii + 1];
+                    const double z0 = z[ii];
+                    z[ii + 1] = fma(ss, z0, cc * z1);
+                    z1 = fma(cc, z0, -ss * z1);
+                }
+                z[ilo] = z1;
+            }
+            if (!split) {
+                d[l] = d[l] - p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+    return 0;
+}
+
+// y <- S y: the recorded sweeps in reverse order, each sweep's rotations in
+// increasing index (the inverse of the forward update)
+__device__ void ql_back_vec(EigLds& L, const VecSet& V, int nvec, const double* rbuf, long rpos,
+                            int lane)
+{
+    long pos = rpos;
+    while (pos > 0) {
+        const int ilo = (int)rbuf[pos - 2], m = (int)rbuf[pos - 1];
+        const int cnt = m - ilo;
+        const long base = pos - 2 - 2L * cnt;
+        for (int q = lane; q < 2 * cnt; q += 64)
+            L.rot[2 * (m - 1 - (q >> 1)) + (q & 1)] = rbuf[base + q];
+        __syncthreads();
+        if (lane < nvec) {
+            double* z = V.at(lane);
+            double zl = z[ilo];
+            for (int ii = ilo; ii < m; ++ii) {
+                const double cc = L.rot[2 * ii], ss = L.rot[2 * ii + 1];
+                const double zi1 = z[ii + 1];
+                z[ii] = fma(cc, zl, ss * zi1);
+                zl = fma(-ss, zl, cc * zi1);
+            }
+            z[m] = zl;
+        }
+        __syncthreads();
+        pos = base;
+    }
+}
+
+// one point of the large-k vector path (no trans/pao outputs)
+__device__ void big_vec_point(const LekfJob& J, int b, double* rec, double* rbuf, long rcap,
+                              EigLds& L, int lane)
+{
+    const int k = J.k, lda = k;
+    const long off = J.loc_off[b];
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
+    double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
+    int status = 0;
+    const double sa = sqrt(alpha);
+    const bool apply = (J.flags & LEKF_F_APPLY) != 0;
+    const int lev = apply ? b / J.nij : 0, ij = apply ? b - lev * J.nij : 0;
+    const int n3 = apply ? J.nv3d : 0, n2 = (apply && lev == 0) ? J.nv2d : 0;
+    const int nrow = n3 + n2;
+    const int ks = k + 1;  // row stride of R: odd, so lane-per-row LDS accesses spread over banks
+    if (nrow > 0) {
+        for (int w = lane; w < nrow * k; w += 64) {
+            const int r = w / k, m = w - r * k;
+            const double* src = (r < n3) ? J.x3b + ((size_t)b * n3 + r) * k
+                                         : J.x2b + ((size_t)ij * J.nv2d + (r - n3)) * k;
+            L.X[r * k + m] = src[m];
+        }
+        __syncthreads();
+        for (int r = lane; r < nrow; r += 64) {
+            double s = L.X[r * k];
+            for (int m = 1; m < k; ++m) s += L.X[r * k + m];
+            L.M[r] = s / (double)k;
+        }
+        __syncthreads();
+        for (int w = lane; w < nrow * k; w += 64) L.X[w] -= L.M[w / k];
+        __syncthreads();
+    }
+    if (p > 0) {
+        for (int i = lane; i < k; i += 64) L.B[i] = rec[(size_t)k * k + i];
+        const double p1 = rec[(size_t)k * k + k], p3 = rec[(size_t)k * k + k + 1];
+        __syncthreads();
+        double tr = 0.0;
+        for (int i = 0; i < k; ++i) tr += rec[(size_t)i * lda + i];
+        const double p2 = tr / (double)(k - 1);
+        if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
+        const double rho = 1.0 / alpha;
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) rec[(size_t)i * lda + i] += (double)(k - 1) * rho;
+        for (int w = lane; w < nrow * k; w += 64) L.R[(w / k) * ks + w % k] = L.X[w];
+        __syncthreads();
+        big_tridiag<4, false>(rec, k, lda, L.d, L.e, L.tau, lane);
+        const VecSet Vf{L.R, nrow, ks, L.B, nullptr};
+        big_refl_vecs<4>(rec, k, lda, L.tau, Vf, nrow + 1, true, lane);
+        __syncthreads();
+        long rpos = 0;
+        bool ovf = false;
+        if (eig_ql_vec(L, k, lane, Vf, nrow + 1, rbuf, rcap, rpos, ovf)) status |= LEKF_PT_NOCONV;
+        if (ovf) status |= LEKF_PT_NOCONV;
+        __syncthreads();
+        double lmax = 0.0, lmin = 1.0e308;
+        for (int j = 0; j < k; ++j) {
+            const double lj = L.d[j];
+            lmax = lj > lmax ? lj : lmax;
+            lmin = lj < lmin ? lj : lmin;
+        }
+        if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
+        // u = L^-1 V^T b;  X' w-bar = (V^T x')^T u;  rows <- sqrt((k-1)/L) V^T x'
+        for (int j = lane; j < k; j += 64) L.U[j] = L.B[j] / L.d[j];
+        __syncthreads();
+        for (int r = lane; r < nrow; r += 64) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s = fma(L.R[r * ks + j], L.U[j], s);
+            L.Cw[r] = s;
+        }
+        const bool want_wb = J.transm != nullptr;
+        if (want_wb)
+            for (int j = lane; j < k; j += 64) L.Wb[j] = L.U[j];
+        __syncthreads();
+        for (int w = lane; w < nrow * k; w += 64) {
+            const int j = w % k;
+            L.R[(w / k) * ks + j] *= sqrt((double)(k - 1) / L.d[j]);
+        }
+        __syncthreads();
+        const VecSet Vb{L.R, nrow, ks, L.Wb, nullptr};
+        const int nvb = nrow + (want_wb ? 1 : 0);
+        if (!ovf) ql_back_vec(L, Vb, nvb, rbuf, rpos, lane);
+        big_refl_vecs<4>(rec, k, lda, L.tau, Vb, nvb, false, lane);
+        __syncthreads();
+        if (want_wb)
+            for (int i = lane; i < k; i += 64) J.transm[(size_t)b * k + i] = L.Wb[i];
+        if (lane == 0 && J.infl_out)
+            J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed
+                                                          : adaptive_alpha(alpha, p1, p2, p3);
+    } else {
+        if (J.transm)
+            for (int i = lane; i < k; i += 64) J.transm[(size_t)b * k + i] = 0.0;
+        if (lane == 0 && J.infl_out)
+            J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
+    }
+    if (lane == 0 && J.status) J.status[b] = status;
+    if (nrow == 0) return;
+    const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
+    for (int w = lane; w < nrow * k; w += 64) {
+        const int r = w / k, m = w - r * k;
+        const bool upd = (r >= n3) || ((mask >> r) & 1u);
+        double a;
+        if (!upd)
+            a = L.M[r] + L.X[r * k + m];
+        else if (p > 0)
+            a = L.M[r] + (L.Cw[r] + L.R[r * ks + m]);
+        else
+            a = L.M[r] + sa * L.X[r * k + m];
+        double* dst = (r < n3) ? J.x3a + ((size_t)b * n3 + r) * k
+                               : J.x2a + ((size_t)ij * J.nv2d + (r - n3)) * k;
+        dst[m] = a;
+    }
+}
+
 // One wave per point: C and sums from the Gram workspace, eigendecomposition
 // (KR > 0: in registers for k <= KR; KR = 0: in LDS; BIG > 0: in the point's
 // workspace record in global memory, BIG rows per lane), W assembly, inflation,
@@ -1566,7 +1862,7 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
 template <int KR, int BIG = 0>
 __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
                                                       long gp0, double* __restrict__ ws,
-                                                      long gstride, int nvmax)
+                                                      long gstride, int nvmax, long rot_off)
 {
     extern __shared__ double smem[];
     const long gp = gp0 + blockIdx.x;
@@ -1577,6 +1873,12 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
     const int k = J.k, lda = BIG ? k : eig_lda(k);
     double* const rec = ws + (gp - gp0) * gstride;
     EigLds L = eig_carve(smem, k, nvmax, BIG ? rec : nullptr);
+    if constexpr (BIG > 0) {
+        if (!J.trans && !J.pao) {
+            big_vec_point(J, b, rec, rec + rot_off, gstride - rot_off, L, lane);
+            return;
+        }
+    }
     // Z_ij (component i of eigenvector j once the eigendecomposition is done)
     auto Zv = [&](int i, int j) -> double {
         return BIG ? L.Z[i + (size_t)j * lda] : L.Z[i * lda + j];
@@ -1775,9 +2077,12 @@ size_t lekf_solve_lds_bytes(int k, int nvmax)
 size_t lekf_solve_ws_bytes(long npt, int kmax)
 {
     if (kmax <= 32 || npt <= 0) return 0;
-    const size_t rec = (size_t)gram_stride(kmax) * sizeof(double);
+    const bool big = kmax > LEKF_KSPLIT;
+    const size_t rec = (size_t)(big ? big_stride(kmax) : gram_stride(kmax)) * sizeof(double);
     size_t per = g_lekf_tune.gram_ws_bytes / rec;
     if (per < 1) per = 1;
+    // large-k records are MBs each: keep at least 1024 points per launch
+    if (big && per < 1024) per = 1024;
     if ((size_t)npt < per) per = (size_t)npt;
     return per * rec;
 }
@@ -1802,7 +2107,7 @@ static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const
 #define LEKF_EIG_CASE(K)                                                                       \
     case K:                                                                                    \
         hipLaunchKernelGGL(lekf_eig_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
-                           djobs, njobs, g0, ws, gs, nvmax);                                  \
+                           djobs, njobs, g0, ws, gs, nvmax, 0L);                              \
         break;
         LEKF_EIG_CASE(36)
         LEKF_EIG_CASE(40)
@@ -1855,7 +2160,7 @@ template <int MODE>
 static hipError_t launch_big(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
                              void* ws, size_t ws_bytes, hipStream_t st)
 {
-    const long gs = gram_stride(kmax);
+    const long gs = big_stride(kmax);
     const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
     if (per < 1 || !ws) return hipErrorInvalidValue;
     const size_t elds = eig_lds_doubles_big(kmax, nvmax) * sizeof(double);
@@ -1867,7 +2172,7 @@ static hipError_t launch_big(const LekfJob* djobs, int njobs, long npt, int kmax
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((lekf_eig_kernel<0, 4>), dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,
-                           djobs, njobs, g0, (double*)ws, gs, nvmax);
+                           djobs, njobs, g0, (double*)ws, gs, nvmax, (long)gram_stride(kmax));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -129,3 +129,44 @@ def test_global_k256_mixed_error_budget(gpu, big, prec):
         spread = p["gues3d"][ij, lev, :, :3].std(axis=0, ddof=1)
         worst = max(worst, float((np.abs(got - ref[s]).max(axis=1) / spread).max()))
     assert worst <= 1e-4, worst
+
+
+def test_vector_path_transm_and_inflation(gpu):
+    """Jobs without trans/pao take the path that never forms V: w-bar (transm)
+    and the adaptive inflation against the reference letkf_core."""
+    from letkf_amd import core
+    from letkf_amd.device import DeviceArray, sync
+    k = 200
+    ps = [0, 3, 150, 260, 90, 400]
+    nobs = 1500
+    rng = np.random.default_rng(8)
+    Y = rng.standard_normal((nobs, k))
+    Y -= Y.mean(axis=1, keepdims=True)
+    R = rng.uniform(0.5, 2.0, nobs)
+    D = rng.standard_normal(nobs)
+    idx, rho, off = [], [], [0]
+    for p in ps:
+        idx.append(rng.choice(nobs, p, replace=False))
+        rho.append(rng.uniform(0.01, 1.0, p))
+        off.append(off[-1] + p)
+    t = DeviceArray.from_numpy
+    npt = len(ps)
+    infl_out = DeviceArray.zeros(npt)
+    transm = DeviceArray.zeros(npt * k)
+    dY, dR, dD = t(Y), t(R), t(D)
+    dOff = t(np.array(off), np.int64)
+    dIdx = t(np.concatenate(idx).astype(np.int32), np.int32)
+    dRho, dInfl = t(np.concatenate(rho)), t(np.full(npt, 1.04))
+    job = core.make_job(k=k, npt=npt, hdxf=dY, rdiag=dR, dep=dD, loc_off=dOff, loc_idx=dIdx,
+                        loc_rho=dRho, infl_in=dInfl, infl_out=infl_out, transm=transm)
+    core.letkf_core_batch([job])
+    sync()
+    tm = transm.numpy().reshape(npt, k)
+    io = infl_out.numpy()
+    for b, p in enumerate(ps):
+        sel = idx[b]
+        ref = oracle.letkf_core(Y[sel], R[sel], rho[b], D[sel], 1.04, transm=True,
+                                impl="ref" if oracle.ref_available() else "c")
+        scale = max(1.0, float(np.abs(ref["transm"]).max()))
+        assert np.abs(tm[b] - ref["transm"]).max() <= RTOL * scale, (b, p)
+        assert abs(io[b] - ref["parm_infl"]) <= RTOL * abs(ref["parm_infl"]), (b, p)
