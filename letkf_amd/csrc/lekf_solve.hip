@@ -711,27 +711,195 @@ __device__ __forceinline__ void blk_store(double* out, int k, int grow, int gcol
     }
 }
 
+// Large-k points with fewer local obs than members (0 < p < k) and no
+// trans/pao output are solved in the dual form: the p x p Gram
+// G = B B^T with B = diag(sqrt(w)) Y (one row per local obs), whose
+// eigenpairs give those of C = (k-1)/alpha I + B^T B.
+__device__ __forceinline__ bool big_dual(const LekfJob& J, int p, int k)
+{
+    return p > 0 && p < k && !J.trans && !J.pao;
+}
+
+// One upper 64 x 64 block of G (p x p, row-major at out) over the k members.
 template <int MODE>
-__global__ __launch_bounds__(64) void lekf_gram_blk_kernel(const LekfJob* __restrict__ jobs,
-                                                           int njobs, long gp0,
-                                                           double* __restrict__ ws, long gstride)
+__device__ void gram_dual_blk(const LekfJob& J, long off, int p, int k, int tblk, double* out,
+                              int lane)
 {
     constexpr int NB = kBlkNB;
-    const long gp = gp0 + blockIdx.x;
-    const int lane = threadIdx.x;
-    const int ji = lekf_job_of(jobs, njobs, gp);
-    const LekfJob& J = jobs[ji];
-    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int nbp = (p + 16 * NB - 1) / (16 * NB);
+    if (tblk >= nbp * (nbp + 1) / 2) return;
+    int bi, bj;
+    blk_pair(tblk, nbp, bi, bj);
+    const int cI = 16 * NB * bi, cJ = 16 * NB * bj;
+    const int lr = lane & 15, kq = lane >> 4;
+    const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
+    long orow[NB], ocol[NB];
+    double swr[NB], swc[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        const int i = cI + 16 * t + lr, j = cJ + 16 * t + lr;
+        orow[t] = ocol[t] = -1;
+        swr[t] = swc[t] = 0.0;
+        if (i < p) {
+            const long o = J.loc_idx[off + i];
+            const double rd = J.loc_rdiag ? J.loc_rdiag[off + i] : J.rdiag[o];
+            swr[t] = sqrt((wloc ? 1.0 : J.loc_rho[off + i]) / rd);
+            orow[t] = o;
+        }
+        if (j < p) {
+            const long o = J.loc_idx[off + j];
+            const double rd = J.loc_rdiag ? J.loc_rdiag[off + j] : J.rdiag[o];
+            swc[t] = sqrt((wloc ? 1.0 : J.loc_rho[off + j]) / rd);
+            ocol[t] = o;
+        }
+    }
+    if (MODE == 0) {
+        lekf_d4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_d4{0.0, 0.0, 0.0, 0.0};
+        for (int m0 = 0; m0 < k; m0 += 4) {
+            const int m = m0 + kq;
+            double xr[NB], xc[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                xr[t] = (orow[t] >= 0 && m < k)
+                            ? swr[t] * J.hdxf[orow[t] * J.y_so + (long)m * J.y_sm] : 0.0;
+                xc[t] = (ocol[t] >= 0 && m < k)
+                            ? swc[t] * J.hdxf[ocol[t] * J.y_so + (long)m * J.y_sm] : 0.0;
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt)
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[I], xc[Jt], acc[I][Jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = cI + 16 * I + kq + 4 * r, col = cJ + 16 * Jt + lr;
+                    if (row <= col && col < p) {
+                        out[(size_t)row * p + col] = acc[I][Jt][r];
+                        out[(size_t)col * p + row] = acc[I][Jt][r];
+                    }
+                }
+    } else if (MODE == 1) {
+        lekf_f4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+        float fr[NB], fc[NB];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            fr[t] = (float)swr[t];
+            fc[t] = (float)swc[t];
+        }
+        for (int m0 = 0; m0 < k; m0 += 4) {
+            const int m = m0 + kq;
+            float xr[NB], xc[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                xr[t] = (orow[t] >= 0 && m < k) ? fr[t] * J.hdxf32[orow[t] * (long)k + m] : 0.0f;
+                xc[t] = (ocol[t] >= 0 && m < k) ? fc[t] * J.hdxf32[ocol[t] * (long)k + m] : 0.0f;
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt)
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[I], xc[Jt], acc[I][Jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = cI + 16 * I + 4 * kq + r, col = cJ + 16 * Jt + lr;
+                    if (row <= col && col < p) {
+                        out[(size_t)row * p + col] = (double)acc[I][Jt][r];
+                        out[(size_t)col * p + row] = (double)acc[I][Jt][r];
+                    }
+                }
+    } else {
+        lekf_f4 acc[NB][NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt) acc[I][Jt] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+        float fr[NB], fc[NB];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            fr[t] = (float)swr[t];
+            fc[t] = (float)swc[t];
+        }
+        for (int m0 = 0; m0 < k; m0 += 32) {
+            lekf_bf8 ah[NB], al[NB], bh[NB], bl[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int m = m0 + 8 * kq + e;
+                    const float a1 = (orow[t] >= 0 && m < k) ? fr[t] * J.hdxf32[orow[t] * (long)k + m] : 0.0f;
+                    const float a2 = (ocol[t] >= 0 && m < k) ? fc[t] * J.hdxf32[ocol[t] * (long)k + m] : 0.0f;
+                    const __bf16 ah1 = (__bf16)a1;
+                    const __bf16 al1 = (__bf16)(a1 - (float)ah1);
+                    const __bf16 bh1 = (__bf16)a2;
+                    const __bf16 bl1 = (__bf16)(a2 - (float)bh1);
+                    ah[t][e] = ah1;
+                    al[t][e] = al1;
+                    bh[t][e] = bh1;
+                    bl[t][e] = bl1;
+                }
+            }
+#pragma unroll
+            for (int I = 0; I < NB; ++I)
+#pragma unroll
+                for (int Jt = 0; Jt < NB; ++Jt) {
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[I], bh[Jt], acc[I][Jt], 0, 0, 0);
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bl[Jt], acc[I][Jt], 0, 0, 0);
+                    acc[I][Jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bh[Jt], acc[I][Jt], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+#pragma unroll
+            for (int Jt = 0; Jt < NB; ++Jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = cI + 16 * I + 4 * kq + r, col = cJ + 16 * Jt + lr;
+                    if (row <= col && col < p) {
+                        out[(size_t)row * p + col] = (double)acc[I][Jt][r];
+                        out[(size_t)col * p + row] = (double)acc[I][Jt][r];
+                    }
+                }
+    }
+}
+
+// Block tblk (upper-triangle order) of point b's Gram into its workspace
+// record out: [C | b | sum w d^2 | P3] (64 x 64 blocks of the augmented
+// (k+1)^2 product), or a block of G in the dual form.
+template <int MODE>
+__device__ void gram_blk_point(const LekfJob& J, int b, int tblk, double* out, int lane)
+{
+    constexpr int NB = kBlkNB;
     const int k = J.k;
     const int nbk = (k + 1 + 16 * NB - 1) / (16 * NB);
-    if ((int)blockIdx.y >= nbk * (nbk + 1) / 2) return;  // block of a larger k of the launch
+    if (tblk >= nbk * (nbk + 1) / 2) return;  // block of a larger k of the launch
     int bi, bj;
-    blk_pair((int)blockIdx.y, nbk, bi, bj);
+    blk_pair(tblk, nbk, bi, bj);
     const int cI = 16 * NB * bi, cJ = 16 * NB * bj;
     const long off = J.loc_off[b];
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
     if (p <= 0) return;
-    double* out = ws + (gp - gp0) * gstride;
+    if (big_dual(J, p, k)) {
+        gram_dual_blk<MODE>(J, off, p, k, tblk, out, lane);
+        return;
+    }
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
     double p3 = 0.0;
@@ -891,7 +1059,7 @@ __global__ __launch_bounds__(64) void lekf_gram_blk_kernel(const LekfJob* __rest
                     blk_store(out, k, cI + 16 * I + 4 * kq + r, cJ + 16 * Jt + lr,
                               (double)acc[I][Jt][r]);
     }
-    if (blockIdx.y == 0) {
+    if (tblk == 0) {
         p3 = wave_sum(p3);
         if (lane == 0) out[(size_t)k * k + k + 1] = p3;
     }
@@ -912,20 +1080,21 @@ struct EigLds {
     double* R;    // nvmax rows of k (k + 1 apart in the large-k vector path)
     double* M;    // nvmax  row means
     double* Cw;   // nvmax  X' w-bar
+    double* sw;   // k  sqrt of the obs weights (large-k dual form)
 };
 
 __host__ __device__ inline int eig_lda(int k) { return k | 1; }
 
 __host__ __device__ inline size_t eig_lds_doubles(int k, int nvmax)
 {
-    return (size_t)k * eig_lda(k) + 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
+    return (size_t)k * eig_lda(k) + 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
 }
 
 // Z in LDS (zg == nullptr) or, for the large-k kernel, the point's workspace
 // record in global memory (zg)
 __host__ __device__ inline size_t eig_lds_doubles_big(int k, int nvmax)
 {
-    return 9 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
+    return 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
 }
 
 __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nullptr)
@@ -960,6 +1129,8 @@ __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nul
     L.M = s;
     s += nvmax;
     L.Cw = s;
+    s += nvmax;
+    L.sw = s;
     return L;
 }
 
@@ -1092,6 +1263,8 @@ __device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
 // access is a contiguous row segment.  Q is accumulated as above and then
 // transposed in place: on return Z[r + c * lda] = Q[r][c], the layout the
 // lane-per-row QL sweeps read contiguously.
+constexpr int kRowBatch = 8;  // rows in flight per lane in the global-memory reduction
+
 template <int NR, bool FORMQ = true>
 __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, double* tauv, int lane)
 {
@@ -1134,17 +1307,26 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
         double pp[NR];
 #pragma unroll
         for (int t = 0; t < NR; ++t) pp[t] = 0.0;
+        // rows are loaded kRowBatch at a time so that many loads are in flight
 #pragma unroll
         for (int cb = 0; cb < NR; ++cb) {
             const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
             if (cn <= 0) break;
-#pragma unroll 4
-            for (int cc = 0; cc < cn; ++cc) {
-                const double vc = lane_bcast(v[cb], cc);
-                const double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+            for (int c0 = 0; c0 < cn; c0 += kRowBatch) {
+                double x[kRowBatch][NR];
 #pragma unroll
-                for (int t = 0; t < NR; ++t)
-                    if (lane + 64 * t < m) pp[t] = fma(row[lane + 64 * t], vc, pp[t]);
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const double* row = A + (size_t)(o + 64 * cb + c0 + u) * lda + o;
+#pragma unroll
+                    for (int t = 0; t < NR; ++t)
+                        x[u][t] = (c0 + u < cn && lane + 64 * t < m) ? row[lane + 64 * t] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const double vc = lane_bcast(v[cb], (c0 + u) & 63);
+#pragma unroll
+                    for (int t = 0; t < NR; ++t) pp[t] = fma(x[u][t], vc, pp[t]);
+                }
             }
         }
         double pv = 0.0;
@@ -1161,18 +1343,26 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
         for (int cb = 0; cb < NR; ++cb) {
             const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
             if (cn <= 0) break;
-#pragma unroll 4
-            for (int cc = 0; cc < cn; ++cc) {
-                const double vc = lane_bcast(v[cb], cc);
-                const double wc = lane_bcast(w[cb], cc);
-                double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
+            for (int c0 = 0; c0 < cn; c0 += kRowBatch) {
+                double x[kRowBatch][NR];
 #pragma unroll
-                for (int t = 0; t < NR; ++t)
-                    if (lane + 64 * t < m) {
-                        const int r = lane + 64 * t;
-                        row[r] = __dsub_rn(row[r],
-                                           __dadd_rn(__dmul_rn(vc, w[t]), __dmul_rn(wc, v[t])));
-                    }
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const double* row = A + (size_t)(o + 64 * cb + c0 + u) * lda + o;
+#pragma unroll
+                    for (int t = 0; t < NR; ++t)
+                        x[u][t] = (c0 + u < cn && lane + 64 * t < m) ? row[lane + 64 * t] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const double vc = lane_bcast(v[cb], (c0 + u) & 63);
+                    const double wc = lane_bcast(w[cb], (c0 + u) & 63);
+                    double* row = A + (size_t)(o + 64 * cb + c0 + u) * lda + o;
+#pragma unroll
+                    for (int t = 0; t < NR; ++t)
+                        if (c0 + u < cn && lane + 64 * t < m)
+                            row[lane + 64 * t] = __dsub_rn(
+                                x[u][t], __dadd_rn(__dmul_rn(vc, w[t]), __dmul_rn(wc, v[t])));
+                }
             }
         }
         __syncthreads();
@@ -1212,7 +1402,7 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
         for (int cb = 0; cb < NR; ++cb) {
             const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
             if (cn <= 0) break;
-#pragma unroll 4
+#pragma unroll 16
             for (int cc = 0; cc < cn; ++cc) {
                 const double vc = lane_bcast(v[cb], cc);
                 const double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
@@ -1227,7 +1417,7 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
         for (int cb = 0; cb < NR; ++cb) {
             const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
             if (cn <= 0) break;
-#pragma unroll 4
+#pragma unroll 16
             for (int cc = 0; cc < cn; ++cc) {
                 const double vc = lane_bcast(v[cb], cc);
                 double* row = A + (size_t)(o + 64 * cb + cc) * lda + o;
@@ -1853,29 +2043,205 @@ __device__ void big_vec_point(const LekfJob& J, int b, double* rec, double* rbuf
     }
 }
 
+// One point of the large-k dual form (0 < p < k, no trans/pao).  With
+// B = diag(sqrt w) Y (p x k), G = B B^T = U S U^T (in the workspace record,
+// p x p) and gamma = (k-1)/alpha, C = gamma I + B^T B has the eigenvalues
+// gamma + s_j (eigenvectors B^T u_j / sqrt(s_j)) and gamma (k - p times), so
+//   f(C) x = f(gamma) x + B^T U diag((f(gamma + s) - f(gamma)) / s) U^T B x
+// for f(l) = sqrt((k-1)/l), and C^-1 B^T = B^T (gamma I + G)^-1 gives
+//   w-bar = B^T U diag(1/(gamma + s)) U^T (sqrt(w) d).
+// The p x p eigenproblem runs exactly as the k x k one of big_vec_point,
+// carrying sqrt(w) d and B x'_r through Q^T and the QL rotations.
+__device__ void big_dual_point(const LekfJob& J, int b, double* rec, double* rbuf, long rcap,
+                               EigLds& L, int lane)
+{
+    const int k = J.k;
+    const long off = J.loc_off[b];
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
+    double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
+    int status = 0;
+    const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
+    const bool apply = (J.flags & LEKF_F_APPLY) != 0;
+    const int lev = apply ? b / J.nij : 0, ij = apply ? b - lev * J.nij : 0;
+    const int n3 = apply ? J.nv3d : 0, n2 = (apply && lev == 0) ? J.nv2d : 0;
+    const int nrow = n3 + n2;
+    const int ks = k + 1;
+    if (nrow > 0) {
+        for (int w = lane; w < nrow * k; w += 64) {
+            const int r = w / k, m = w - r * k;
+            const double* src = (r < n3) ? J.x3b + ((size_t)b * n3 + r) * k
+                                         : J.x2b + ((size_t)ij * J.nv2d + (r - n3)) * k;
+            L.X[r * k + m] = src[m];
+        }
+        __syncthreads();
+        for (int r = lane; r < nrow; r += 64) {
+            double s = L.X[r * k];
+            for (int m = 1; m < k; ++m) s += L.X[r * k + m];
+            L.M[r] = s / (double)k;
+        }
+        __syncthreads();
+        for (int w = lane; w < nrow * k; w += 64) L.X[w] -= L.M[w / k];
+        __syncthreads();
+    }
+    // weights, sqrt(w) d, P1 = sum w d^2, P3 = sum rho (common_letkf.f90:219-227)
+    double p1 = 0.0, p3 = 0.0;
+    for (int i = lane; i < p; i += 64) {
+        const long o = J.loc_idx[off + i];
+        const double rho = J.loc_rho[off + i];
+        const double rd = J.loc_rdiag ? J.loc_rdiag[off + i] : J.rdiag[o];
+        const double w = (wloc ? 1.0 : rho) / rd;
+        const double dd = J.dep[o];
+        L.sw[i] = sqrt(w);
+        L.B[i] = L.sw[i] * dd;
+        p1 = fma(w * dd, dd, p1);
+        p3 += rho;
+    }
+    p1 = wave_sum(p1);
+    p3 = wave_sum(p3);
+    double tr = 0.0;
+    for (int i = 0; i < p; ++i) tr += rec[(size_t)i * p + i];
+    const double p2 = tr / (double)(k - 1);
+    if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
+    const double gam = (double)(k - 1) / alpha;
+    __syncthreads();
+    // y_r = B x'_r: one local obs row at a time, lanes over members
+    for (int i = 0; i < p; ++i) {
+        const long o = J.loc_idx[off + i];
+        double y[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int m = lane + 64 * t;
+            y[t] = m < k ? J.hdxf[o * J.y_so + (long)m * J.y_sm] : 0.0;
+        }
+        for (int r = 0; r < nrow; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (lane + 64 * t < k) s = fma(y[t], L.X[r * k + lane + 64 * t], s);
+            s = wave_sum(s);
+            if (lane == 0) L.R[r * ks + i] = L.sw[i] * s;
+        }
+    }
+    __syncthreads();
+    big_tridiag<4, false>(rec, p, p, L.d, L.e, L.tau, lane);
+    const VecSet Vf{L.R, nrow, ks, L.B, nullptr};
+    big_refl_vecs<4>(rec, p, p, L.tau, Vf, nrow + 1, true, lane);
+    __syncthreads();
+    long rpos = 0;
+    bool ovf = false;
+    if (eig_ql_vec(L, p, lane, Vf, nrow + 1, rbuf, rcap, rpos, ovf)) status |= LEKF_PT_NOCONV;
+    if (ovf) status |= LEKF_PT_NOCONV;
+    __syncthreads();
+    // eigenvalues of C: gamma + s_j and gamma; truncation test (common_mtx.f90:66-74)
+    double smax = L.d[0], smin = L.d[0];
+    for (int j = 1; j < p; ++j) {
+        smax = L.d[j] > smax ? L.d[j] : smax;
+        smin = L.d[j] < smin ? L.d[j] : smin;
+    }
+    const double lmax = gam + (smax > 0.0 ? smax : 0.0);
+    const double lmin = smin < 0.0 ? gam + smin : gam;
+    if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08) status |= LEKF_PT_RANKDEF;
+    // c_b = (gamma + s)^-1 U^T sqrt(w) d;  X' w-bar = (U^T B x')^T c_b
+    for (int j = lane; j < p; j += 64) L.U[j] = L.B[j] / (gam + L.d[j]);
+    __syncthreads();
+    for (int r = lane; r < nrow; r += 64) {
+        double s = 0.0;
+        for (int j = 0; j < p; ++j) s = fma(L.R[r * ks + j], L.U[j], s);
+        L.Cw[r] = s;
+    }
+    const bool want_wb = J.transm != nullptr;
+    if (want_wb)
+        for (int j = lane; j < p; j += 64) L.Wb[j] = L.U[j];
+    __syncthreads();
+    // (f(gamma + s) - f(gamma)) / s without cancellation
+    const double sg = sqrt(gam), skm1 = sqrt((double)(k - 1));
+    for (int w = lane; w < nrow * p; w += 64) {
+        const int r = w / p, j = w - r * p;
+        const double sgs = sqrt(gam + L.d[j]);
+        L.R[r * ks + j] *= -skm1 / (sg * sgs * (sg + sgs));
+    }
+    __syncthreads();
+    const VecSet Vb{L.R, nrow, ks, L.Wb, nullptr};
+    const int nvb = nrow + (want_wb ? 1 : 0);
+    if (!ovf) ql_back_vec(L, Vb, nvb, rbuf, rpos, lane);
+    big_refl_vecs<4>(rec, p, p, L.tau, Vb, nvb, false, lane);
+    __syncthreads();
+    // coefficients of the local obs rows: sqrt(w_i) times the p-vectors
+    for (int w = lane; w < nrow * p; w += 64) {
+        const int r = w / p, i = w - r * p;
+        L.R[r * ks + i] *= L.sw[i];
+    }
+    if (want_wb)
+        for (int i = lane; i < p; i += 64) L.Wb[i] *= L.sw[i];
+    __syncthreads();
+    if (want_wb) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < p; ++i) {
+            const long o = J.loc_idx[off + i];
+            const double c = L.Wb[i];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int m = lane + 64 * t;
+                if (m < k) acc[t] = fma(c, J.hdxf[o * J.y_so + (long)m * J.y_sm], acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (lane + 64 * t < k) J.transm[(size_t)b * k + lane + 64 * t] = acc[t];
+    }
+    if (lane == 0 && J.infl_out)
+        J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed
+                                                      : adaptive_alpha(alpha, p1, p2, p3);
+    if (lane == 0 && J.status) J.status[b] = status;
+    if (nrow == 0) return;
+    const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
+    const double fg = sqrt((double)(k - 1) / gam);
+    for (int r = 0; r < nrow; ++r) {
+        const bool upd = (r >= n3) || ((mask >> r) & 1u);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (upd)
+            for (int i = 0; i < p; ++i) {
+                const long o = J.loc_idx[off + i];
+                const double c = L.R[r * ks + i];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int m = lane + 64 * t;
+                    if (m < k) acc[t] = fma(c, J.hdxf[o * J.y_so + (long)m * J.y_sm], acc[t]);
+                }
+            }
+        double* dst = (r < n3) ? J.x3a + ((size_t)b * n3 + r) * k
+                               : J.x2a + ((size_t)ij * J.nv2d + (r - n3)) * k;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int m = lane + 64 * t;
+            if (m < k) {
+                const double xp = L.X[r * k + m];
+                dst[m] = upd ? L.M[r] + (L.Cw[r] + fma(fg, xp, acc[t])) : L.M[r] + xp;
+            }
+        }
+    }
+}
+
 // One wave per point: C and sums from the Gram workspace, eigendecomposition
 // (KR > 0: in registers for k <= KR; KR = 0: in LDS; BIG > 0: in the point's
 // workspace record in global memory, BIG rows per lane), W assembly, inflation,
 // the optional trans/transm/pao outputs and the fused apply
 // Xa = x-mean + X' T evaluated as x-mean + (X' V) diag(sqrt((k-1)/L)) V^T
 // + (X' w-bar) 1^T.
-template <int KR, int BIG = 0>
-__global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
-                                                      long gp0, double* __restrict__ ws,
-                                                      long gstride, int nvmax, long rot_off)
+template <int KR, int BIG>
+__device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstride, int nvmax,
+                          long rot_off, double* smem, int lane)
 {
-    extern __shared__ double smem[];
-    const long gp = gp0 + blockIdx.x;
-    const int lane = threadIdx.x;
-    const int ji = lekf_job_of(jobs, njobs, gp);
-    const LekfJob& J = jobs[ji];
-    const int b = (int)(gp - J.pt_begin + J.pt0);
     const int k = J.k, lda = BIG ? k : eig_lda(k);
-    double* const rec = ws + (gp - gp0) * gstride;
     EigLds L = eig_carve(smem, k, nvmax, BIG ? rec : nullptr);
     if constexpr (BIG > 0) {
         if (!J.trans && !J.pao) {
-            big_vec_point(J, b, rec, rec + rot_off, gstride - rot_off, L, lane);
+            const long o0 = J.loc_off[b];
+            const int pp = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - o0);
+            if (big_dual(J, pp, k))
+                big_dual_point(J, b, rec, rec + rot_off, gstride - rot_off, L, lane);
+            else
+                big_vec_point(J, b, rec, rec + rot_off, gstride - rot_off, L, lane);
             return;
         }
     }
@@ -2044,6 +2410,43 @@ __global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict_
     }
 }
 
+template <int KR>
+__global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                      long gp0, double* __restrict__ ws,
+                                                      long gstride, int nvmax)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    eig_point<KR, 0>(J, (int)(gp - J.pt_begin + J.pt0), ws + (gp - gp0) * gstride, gstride, nvmax,
+                     0L, smem, threadIdx.x);
+}
+
+// LEKF_KSPLIT < k: one persistent wave per workgroup slot strides over the
+// points; for each it forms the Gram blocks on MFMA into the slot's
+// workspace record, then runs the eigen/assembly/apply steps in place.  The
+// slot's record is reused from point to point, so the workspace is
+// slots x record whatever the number of points, and one launch covers them
+// all (points of very different cost balance out over the stride).
+template <int MODE>
+__global__ __launch_bounds__(64) void lekf_big_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                      long npt, double* __restrict__ ws,
+                                                      long gstride, int nvmax, long rot_off,
+                                                      int nblk)
+{
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    double* const rec = ws + (size_t)blockIdx.x * gstride;
+    for (long gp = blockIdx.x; gp < npt; gp += gridDim.x) {
+        const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+        const int b = (int)(gp - J.pt_begin + J.pt0);
+        __syncthreads();  // the previous point is done with the record
+        for (int t = 0; t < nblk; ++t) gram_blk_point<MODE>(J, b, t, rec, lane);
+        __syncthreads();
+        eig_point<0, 4>(J, b, rec, gstride, nvmax, rot_off, smem, lane);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2107,7 +2510,7 @@ static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const
 #define LEKF_EIG_CASE(K)                                                                       \
     case K:                                                                                    \
         hipLaunchKernelGGL(lekf_eig_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
-                           djobs, njobs, g0, ws, gs, nvmax, 0L);                              \
+                           djobs, njobs, g0, ws, gs, nvmax);                                  \
         break;
         LEKF_EIG_CASE(36)
         LEKF_EIG_CASE(40)
@@ -2161,22 +2564,20 @@ static hipError_t launch_big(const LekfJob* djobs, int njobs, long npt, int kmax
                              void* ws, size_t ws_bytes, hipStream_t st)
 {
     const long gs = big_stride(kmax);
-    const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
-    if (per < 1 || !ws) return hipErrorInvalidValue;
+    const long slots = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
+    if (slots < 1 || !ws) return hipErrorInvalidValue;
     const size_t elds = eig_lds_doubles_big(kmax, nvmax) * sizeof(double);
-    const int nblk = gram_nblk(kmax);
-    for (long g0 = 0; g0 < npt; g0 += per) {
-        const long nb = per < npt - g0 ? per : npt - g0;
-        hipLaunchKernelGGL((lekf_gram_blk_kernel<MODE>), dim3((unsigned)nb, (unsigned)nblk),
-                           dim3(LEKF_WAVE), 0, st, djobs, njobs, g0, (double*)ws, gs);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((lekf_eig_kernel<0, 4>), dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,
-                           djobs, njobs, g0, (double*)ws, gs, nvmax, (long)gram_stride(kmax));
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    // as many workgroups as can be resident at once (LDS bounds the count)
+    int dev = 0, ncu = 256, per_cu = 1;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lekf_big_kernel<MODE>, LEKF_WAVE, elds);
+    long nwg = (long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 256);
+    if (nwg > slots) nwg = slots;
+    if (nwg > npt) nwg = npt;
+    hipLaunchKernelGGL((lekf_big_kernel<MODE>), dim3((unsigned)nwg), dim3(LEKF_WAVE), elds, st, djobs,
+                       njobs, npt, (double*)ws, gs, nvmax, (long)gram_stride(kmax), gram_nblk(kmax));
+    return hipGetLastError();
 }
 
 template <int MODE>
