@@ -303,20 +303,106 @@ def gfs_bench(steps=2, precs=("f64", "f32", "bf16x3")):
             "nobs_max": 1000, "gen_s": gen_s, "forms": res}
 
 
+def speedy_k256_bench(steps=2, precs=("f64", "f32", "bf16x3"), cpu_sample=0):
+    """Large-k variant of the SPEEDY-sized config (configs[3], last sentence): the same
+    96x48x7 grid, fields and obs with k=256 members.  Points with fewer local obs than
+    members take the dual (p x p) form, the rest the k x k form; each form's analysis is
+    compared with the FP64 GPU analysis."""
+    from letkf_amd import das, synth
+    from letkf_amd.device import Event
+    t0 = time.perf_counter()
+    p = synth.speedy(ne=256)
+    gen_s = time.perf_counter() - t0
+    npt = p["nij"] * p["nlev"]
+    spread = p["gues3d"].std(axis=2, ddof=1)
+    res, ref, keep = {}, None, None
+    for prec in precs:
+        h = das.GlobalLetkf(lon=p["lon"], lat=p["lat"], sig=p["sig"], ne=p["ne"], nv3d=4, nv2d=1,
+                            sigma_obs=p["sigma_obs"], sigma_obsv=p["sigma_obsv"],
+                            gross_error=p["gross_error"], prec=prec, infl_value=p["infl"])
+        h.set_ensemble(p["gues3d"], p["gues2d"])
+        o = p["obs"]
+        h.set_obs(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"], o["err"], o["hxf"])
+        h.run()
+        h.sync()
+        st = h.stream()
+        lat = []
+        for _ in range(steps):
+            e0, e1 = Event(), Event()
+            t1 = time.perf_counter()
+            e0.record(st)
+            h.run()
+            e1.record(st)
+            h.sync()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        a3, _ = h.analysis()
+        nl = h.nobsl()
+        r = {"local_analyses_per_s": npt / (np.mean(lat) / 1e3), "cycle_ms": float(np.mean(lat)),
+             "mean_nobsl": float(nl.mean()), "frac_points_dual": float(np.mean((nl > 0) & (nl < 256)))}
+        if ref is None:
+            ref = a3
+            keep = h
+        else:
+            r["err_vs_f64_over_bg_spread"] = float((np.abs(a3 - ref).max(axis=2) /
+                                                   np.maximum(spread, 1e-300)).max())
+            h.close()
+        res[prec] = r
+        print(f"[bench] speedy k=256 {prec}: {r['cycle_ms']:.0f} ms/cycle", file=sys.stderr,
+              flush=True)
+    for prec in res:
+        res[prec]["speedup_vs_f64"] = res[prec]["local_analyses_per_s"] / res["f64"]["local_analyses_per_s"]
+    out = {"grid": [96, 48, 7], "k": 256, "local_analyses_per_cycle": npt, "gen_s": gen_s,
+           "forms": res}
+    if cpu_sample and keep is not None:
+        out["cpu_baseline"] = k256_cpu_sample(p, keep, cpu_sample)
+    keep.close()
+    return out
+
+
+def k256_cpu_sample(p, h, npts):
+    """Reference letkf_core + apply (u, v, T, q) on a seeded sample of the k=256 points,
+    on the local lists the GPU selected (selection is pinned bit-exact by the tests)."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    sobs, _, _ = oracle.set_letkf_obs_global(p["obs"], p["lon"], p["lat"], p["gross_error"])
+    nij = p["nij"]
+    rng = np.random.default_rng(9)
+    pts = np.sort(rng.choice(nij * p["nlev"], npts, replace=False))
+    lists = [h.local(int(b)) for b in pts]
+    off = np.cumsum([0] + [l[0].size for l in lists])
+    xb = np.stack([p["gues3d"][int(b) % nij, int(b) // nij, :, :].T for b in pts])
+    t0 = time.perf_counter()
+    oracle.ref_solve_apply(off, np.concatenate([l[0] for l in lists]),
+                           np.concatenate([l[1] for l in lists]),
+                           np.ascontiguousarray(sobs["hdxf"].T), sobs["err"] ** 2, sobs["dep"],
+                           np.ascontiguousarray(xb), np.full(npts, p["infl"]))
+    t = time.perf_counter() - t0
+    return {"value": npts / t, "unit": "local analyses/s", "cores": oracle.ref_threads(),
+            "kind": "reference",
+            "sample": (f"{npts} seeded points of the k=256 SPEEDY-sized grid on their local "
+                       f"lists (mean {np.mean(np.diff(off)):.0f} obs): reference Fortran "
+                       f"letkf_core + EISPACK + reference DGEMM + apply to 4 variables, "
+                       f"{t:.2f} s; selection not timed")}
+
+
 def serve_bench(duration_s=20.0, rate=2.0, precs=("f64", "bf16x3"), seed=5):
     """Served analysis stream: Poisson arrivals of LETKF cycles of mixed sizes,
-    each with a 30-s deadline.  Mix (by arrival): 50% Lorenz-96 (N in 40..960,
-    k in 20..80), 35% SPEEDY-sized (k=20), 15% regional radar on 64x64x60 columns (k=50).
-    Every job is a fresh instance with its own background and obs."""
+    each with a 30-s deadline.  Mix (by arrival): 45% Lorenz-96 (N in 40..960,
+    k in 20..200), 40% SPEEDY-sized (k=20 or 64), 15% regional radar on 64x64x60
+    columns (k=50).  Every job is a fresh instance with its own background and obs."""
     from letkf_amd import serve, synth
     rng = np.random.default_rng(seed)
     tmpl = {"l96_40_20": serve.l96_job, "l96_240_40": serve.l96_job, "l96_960_80": serve.l96_job,
-            "speedy_20": serve.global_job, "radar_64_50": serve.regional_job}
+            "l96_960_200": serve.l96_job, "speedy_20": serve.global_job,
+            "speedy_64": serve.global_job, "radar_64_50": serve.regional_job}
     probs = {"l96_40_20": synth.l96(nx=40, ne=20), "l96_240_40": synth.l96(nx=240, ne=40, seed=2),
-             "l96_960_80": synth.l96(nx=960, ne=80, seed=3), "speedy_20": synth.speedy(),
+             "l96_960_80": synth.l96(nx=960, ne=80, seed=3),
+             "l96_960_200": synth.l96(nx=960, ne=200, seed=6), "speedy_20": synth.speedy(),
+             "speedy_64": synth.speedy(ne=64, seed=8),
              "radar_64_50": synth.radar(sub=(64, 64, 60), seed=4)}
     names = list(tmpl)
-    weights = np.array([0.2, 0.15, 0.15, 0.35, 0.15])
+    weights = np.array([0.15, 0.1, 0.1, 0.1, 0.25, 0.15, 0.15])
     arr = serve.poisson_arrivals(rate, duration_s, seed)
     kinds = rng.choice(len(names), size=arr.size, p=weights / weights.sum())
     out = {}
@@ -476,6 +562,7 @@ def main():
         sub = (256, 256, 60) if args.radar_full else (128, 128, 60)
         out["radar"] = radar_bench(sub, steps=3, cpu_sample=0 if args.no_cpu else 1500)
         out["gfs"] = gfs_bench()
+        out["speedy_k256"] = speedy_k256_bench(cpu_sample=0 if args.no_cpu else 480)
         out["serve"] = serve_bench(duration_s=args.serve_s)
     print(json.dumps(out), flush=True)
 

@@ -1444,6 +1444,151 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
     __syncthreads();
 }
 
+// The same reduction for the vector paths, with a third of the memory
+// traffic: only the upper triangle (row c, columns >= c) is kept, and step
+// s's rank-2 update is applied during step s+1's single pass over the
+// trailing block, which also forms step s+1's product B v (p_r gets
+// A[c][r] v_c from rows c < r and the row-r sums, reduced across lanes).
+// vP, wP (pending update) and vN (current reflector) are LDS vectors of n
+// indexed absolutely.  On exit d, e, tau and the reflectors (row s,
+// columns s+1..) are as big_tridiag leaves them, the elementwise update
+// being rounded identically.
+template <int NR>
+__device__ void big_tridiag_sym(double* A, int n, int lda, double* d, double* e, double* tauv,
+                                double* vP, double* wP, double* vN, int lane)
+{
+    for (int r = lane; r < n; r += 64) {
+        vP[r] = 0.0;
+        wP[r] = 0.0;
+    }
+    __syncthreads();
+    for (int s = 0; s + 2 < n; ++s) {
+        const int o = s + 1, m = n - o;
+        // row s with the pending update: diagonal and x = A[s][o..]
+        const double vps = vP[s], wps = wP[s];
+        double* rows = A + (size_t)s * lda;
+        double x[NR];
+        double s2 = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int i = lane + 64 * t, r = o + i;
+            x[t] = 0.0;
+            if (i < m)
+                x[t] = __dsub_rn(rows[r], __dadd_rn(__dmul_rn(vps, wP[r]), __dmul_rn(wps, vP[r])));
+            if (i > 0) s2 = fma(x[t], x[t], s2);
+        }
+        const double dss = __dsub_rn(rows[s], __dadd_rn(__dmul_rn(vps, wps), __dmul_rn(wps, vps)));
+        const double sigma = wave_sum(s2);
+        const double alpha = lane_bcast(x[0], 0);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (sigma > 0.0) {
+            const double nrm = sqrt(fma(alpha, alpha, sigma));
+            beta = alpha <= 0.0 ? nrm : -nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        double v[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int i = lane + 64 * t;
+            v[t] = i == 0 ? 1.0 : (tau == 0.0 ? 0.0 : x[t] * scal);
+            if (i < m) {
+                rows[o + i] = v[t];
+                vN[o + i] = v[t];
+            }
+        }
+        if (lane == 0) {
+            d[s] = dss;
+            e[s] = beta;
+            tauv[s] = tau;
+        }
+        __syncthreads();
+        // one pass over rows c = o.. (columns r >= c): pending update, store,
+        // and p = B v of this step
+        double wpr[NR], vpr[NR], vnr[NR], pp[NR];
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int i = lane + 64 * t, r = o + i;
+            wpr[t] = i < m ? wP[r] : 0.0;
+            vpr[t] = i < m ? vP[r] : 0.0;
+            vnr[t] = i < m ? v[t] : 0.0;
+            pp[t] = 0.0;
+        }
+#pragma unroll
+        for (int cb = 0; cb < NR; ++cb) {
+            const int cn = m - 64 * cb < 64 ? m - 64 * cb : 64;
+            if (cn <= 0) break;
+            for (int c0 = 0; c0 < cn; c0 += kRowBatch) {
+                double a[kRowBatch][NR];
+#pragma unroll
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const int ic = 64 * cb + c0 + u;  // row index relative to o
+                    const double* row = A + (size_t)(o + ic) * lda + o;
+#pragma unroll
+                    for (int t = 0; t < NR; ++t) {
+                        const int i = lane + 64 * t;
+                        a[u][t] = (c0 + u < cn && t >= cb && i >= ic && i < m) ? row[i] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kRowBatch; ++u) {
+                    const int ic = 64 * cb + c0 + u;
+                    const bool rv = c0 + u < cn;
+                    const int c = o + (rv ? ic : 0);
+                    const double vpc = vP[c], wpc = wP[c], vnc = vN[c];
+                    double* row = A + (size_t)(o + ic) * lda + o;
+                    double rs = 0.0;
+#pragma unroll
+                    for (int t = 0; t < NR; ++t) {
+                        const int i = lane + 64 * t;
+                        if (rv && t >= cb && i >= ic && i < m) {
+                            const double an = __dsub_rn(
+                                a[u][t], __dadd_rn(__dmul_rn(vpc, wpr[t]), __dmul_rn(wpc, vpr[t])));
+                            row[i] = an;
+                            if (i > ic) {
+                                pp[t] = fma(an, vnc, pp[t]);
+                                rs = fma(an, vnr[t], rs);
+                            } else {
+                                rs = fma(an, vnc, rs);
+                            }
+                        }
+                    }
+                    rs = wave_sum(rs);
+                    if (rv && lane == ((c0 + u) & 63)) pp[cb] += rs;
+                }
+            }
+        }
+        __syncthreads();  // every lane is done reading this step's vP, wP, vN
+        double pv = 0.0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            pp[t] *= tau;
+            pv = fma(pp[t], v[t], pv);
+        }
+        const double K = 0.5 * tau * wave_sum(pv);
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int i = lane + 64 * t;
+            if (i < m) {
+                vP[o + i] = v[t];
+                wP[o + i] = fma(-K, v[t], pp[t]);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        const int a0 = n - 2, a1 = n - 1;
+        d[a0] = __dsub_rn(A[(size_t)a0 * lda + a0],
+                          __dadd_rn(__dmul_rn(vP[a0], wP[a0]), __dmul_rn(wP[a0], vP[a0])));
+        e[a0] = __dsub_rn(A[(size_t)a0 * lda + a1],
+                          __dadd_rn(__dmul_rn(vP[a0], wP[a1]), __dmul_rn(wP[a0], vP[a1])));
+        d[a1] = __dsub_rn(A[(size_t)a1 * lda + a1],
+                          __dadd_rn(__dmul_rn(vP[a1], wP[a1]), __dmul_rn(wP[a1], vP[a1])));
+        e[a1] = 0.0;
+    }
+    __syncthreads();
+}
+
 constexpr int kQlMaxIter = 60;
 
 // ql_apply for Z in global memory with Z[r + i * lda] = Z_ri (rows of a
@@ -1975,7 +2120,7 @@ __device__ void big_vec_point(const LekfJob& J, int b, double* rec, double* rbuf
         for (int i = lane; i < k; i += 64) rec[(size_t)i * lda + i] += (double)(k - 1) * rho;
         for (int w = lane; w < nrow * k; w += 64) L.R[(w / k) * ks + w % k] = L.X[w];
         __syncthreads();
-        big_tridiag<4, false>(rec, k, lda, L.d, L.e, L.tau, lane);
+        big_tridiag_sym<4>(rec, k, lda, L.d, L.e, L.tau, L.rot, L.rot + k, L.U, lane);
         const VecSet Vf{L.R, nrow, ks, L.B, nullptr};
         big_refl_vecs<4>(rec, k, lda, L.tau, Vf, nrow + 1, true, lane);
         __syncthreads();
@@ -2123,7 +2268,7 @@ __device__ void big_dual_point(const LekfJob& J, int b, double* rec, double* rbu
         }
     }
     __syncthreads();
-    big_tridiag<4, false>(rec, p, p, L.d, L.e, L.tau, lane);
+    big_tridiag_sym<4>(rec, p, p, L.d, L.e, L.tau, L.rot, L.rot + k, L.U, lane);
     const VecSet Vf{L.R, nrow, ks, L.B, nullptr};
     big_refl_vecs<4>(rec, p, p, L.tau, Vf, nrow + 1, true, lane);
     __syncthreads();
