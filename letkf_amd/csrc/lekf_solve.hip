@@ -126,10 +126,30 @@ __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int 
 // FP32: v_mfma_f32_16x16x4f32 (operands rounded to FP32).
 // BF16x3: each FP32 operand split x = h + l (h = bf16(x), l = bf16(x - h));
 // A*B ~ Ah*Bh + Ah*Bl + Al*Bh on v_mfma_f32_16x16x32_bf16.
+// FP32 form: the FP32 accumulators are added into FP64 ones (accd) every
+// kMixFlush obs per wave, so no FP32 sum runs over more than kMixFlush
+// products and the rounding of long local lists (p ~ 10^3) does not build
+// up.  The BF16x3 form keeps FP32 accumulation: its two-term split carries
+// about 16 significant bits, which dominates its error, and the FP64
+// accumulators would cost it occupancy.
+constexpr int kMixFlush = 64;
+
+template <int NT>
+__device__ __forceinline__ void mix_flush(lekf_f4* acc, lekf_d4* accd)
+{
+#pragma unroll
+    for (int a = 0; a < NT * (NT + 1) / 2; ++a) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accd[a][r] += (double)acc[a][r];
+        acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
 template <int NT>
 __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int k, int lane,
-                                         lekf_f4* acc, double& p3, int wv, int nw)
+                                         lekf_f4* acc, lekf_d4* accd, double& p3, int wv, int nw)
 {
+    int steps = 0;
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
 #pragma unroll 2
@@ -169,7 +189,12 @@ __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int 
                 ++a;
             }
         }
+        if (++steps == kMixFlush / 4) {
+            mix_flush<NT>(acc, accd);
+            steps = 0;
+        }
     }
+    mix_flush<NT>(acc, accd);
 }
 
 // Split-BF16x3: K = 32 obs per MFMA step.  Lane l supplies 8 consecutive k
@@ -466,11 +491,16 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
             lekf_f4 acc[NT * (NT + 1) / 2];
 #pragma unroll
             for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
-            if (MODE == 1)
-                gram_f32<NT>(J, off, p, k, lane, acc, p3, 0, 1);
-            else
+            if (MODE == 1) {
+                lekf_d4 accd[NT * (NT + 1) / 2];
+#pragma unroll
+                for (int a = 0; a < NT * (NT + 1) / 2; ++a) accd[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
+                gram_f32<NT>(J, off, p, k, lane, acc, accd, p3, 0, 1);
+                store_gram<NT, false>(accd, k, lda, lane, L.A, L.B, &L.sc[0], false);
+            } else {
                 gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, 0, 1);
-            store_gram<NT, false>(acc, k, lda, lane, L.A, L.B, &L.sc[0], false);
+                store_gram<NT, false>(acc, k, lda, lane, L.A, L.B, &L.sc[0], false);
+            }
         }
         p3 = wave_sum(p3);
         __syncthreads();
@@ -645,13 +675,21 @@ __global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
         lekf_f4 acc[NT * (NT + 1) / 2];
 #pragma unroll
         for (int a = 0; a < NT * (NT + 1) / 2; ++a) acc[a] = lekf_f4{0.f, 0.f, 0.f, 0.f};
-        if (MODE == 1)
-            gram_f32<NT>(J, off, p, k, lane, acc, p3, wv, W);
-        else
+        if (MODE == 1) {
+            lekf_d4 accd[NT * (NT + 1) / 2];
+#pragma unroll
+            for (int a = 0; a < NT * (NT + 1) / 2; ++a) accd[a] = lekf_d4{0.0, 0.0, 0.0, 0.0};
+            gram_f32<NT>(J, off, p, k, lane, acc, accd, p3, wv, W);
+            for (int w = 0; w < W; ++w) {
+                if (wv == w) store_gram<NT, false>(accd, k, lda, lane, A, B, &sc[0], w > 0);
+                __syncthreads();
+            }
+        } else {
             gram_bf16x3<NT>(J, off, p, k, lane, acc, p3, wv, W);
-        for (int w = 0; w < W; ++w) {
-            if (wv == w) store_gram<NT, false>(acc, k, lda, lane, A, B, &sc[0], w > 0);
-            __syncthreads();
+            for (int w = 0; w < W; ++w) {
+                if (wv == w) store_gram<NT, false>(acc, k, lda, lane, A, B, &sc[0], w > 0);
+                __syncthreads();
+            }
         }
     }
     p3 = wave_sum(p3);
