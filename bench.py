@@ -297,7 +297,9 @@ def gfs_bench(steps=2, precs=("f64", "f32", "bf16x3")):
         res[prec] = r
         print(f"[bench] gfs {prec}: {r['cycle_ms']:.0f} ms/cycle", file=sys.stderr, flush=True)
     for prec in res:
-        res[prec]["speedup_vs_f64"] = res[prec]["local_analyses_per_s"] / res["f64"]["local_analyses_per_s"]
+        if "f64" in res:
+            res[prec]["speedup_vs_f64"] = (res[prec]["local_analyses_per_s"] /
+                                           res["f64"]["local_analyses_per_s"])
     return {"grid": [p["lon"].size, p["lat"].size, p["sig"].size], "k": p["ne"],
             "local_analyses_per_cycle": npt, "nobs": int(p["obs"]["elm"].size),
             "nobs_max": 1000, "gen_s": gen_s, "forms": res}
@@ -350,12 +352,15 @@ def speedy_k256_bench(steps=2, precs=("f64", "f32", "bf16x3"), cpu_sample=0):
         print(f"[bench] speedy k=256 {prec}: {r['cycle_ms']:.0f} ms/cycle", file=sys.stderr,
               flush=True)
     for prec in res:
-        res[prec]["speedup_vs_f64"] = res[prec]["local_analyses_per_s"] / res["f64"]["local_analyses_per_s"]
+        if "f64" in res:
+            res[prec]["speedup_vs_f64"] = (res[prec]["local_analyses_per_s"] /
+                                           res["f64"]["local_analyses_per_s"])
     out = {"grid": [96, 48, 7], "k": 256, "local_analyses_per_cycle": npt, "gen_s": gen_s,
            "forms": res}
     if cpu_sample and keep is not None:
         out["cpu_baseline"] = k256_cpu_sample(p, keep, cpu_sample)
-    keep.close()
+    if keep is not None:
+        keep.close()
     return out
 
 
@@ -466,6 +471,12 @@ def main():
     ap.add_argument("--radar-full", action="store_true",
                     help="radar on the full 256x256x60 grid instead of 128x128x60")
     ap.add_argument("--serve-s", type=float, default=20.0, help="served-stream duration")
+    ap.add_argument("--only", choices=["radar", "radar-full", "gfs", "k256", "serve"],
+                    help="run one of the other configurations alone and print its JSON "
+                         "(for profiling); the default run prints the SPEEDY line with all "
+                         "of them attached")
+    ap.add_argument("--precs", default="f64,f32,bf16x3",
+                    help="arithmetic forms for --only (comma separated)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the LETKF local analyses of a job run on a single "
@@ -480,6 +491,19 @@ def main():
         sys.exit(1)
     if args.copy_calib:
         copy_calibration()
+        return
+    if args.only:
+        precs = tuple(args.precs.split(","))
+        if args.only in ("radar", "radar-full"):
+            sub = (256, 256, 60) if args.only == "radar-full" else (128, 128, 60)
+            r = radar_bench(sub, steps=args.steps, precs=precs)
+        elif args.only == "gfs":
+            r = gfs_bench(steps=args.steps, precs=precs)
+        elif args.only == "k256":
+            r = speedy_k256_bench(steps=args.steps, precs=precs)
+        else:
+            r = serve_bench(duration_s=args.serve_s, precs=precs)
+        print(json.dumps(r), flush=True)
         return
     t0 = time.perf_counter()
     p = synth.speedy()
