@@ -47,6 +47,11 @@ extern "C" {
 #define LEKF_INFL_ADAPTIVE 0  /* letkf_core's adaptive update kept (speedy/letkf/letkf_tools.f90:160-162) */
 #define LEKF_INFL_FIXED 1     /* reset to infl_value after each point (lorenz96/letkf/letkf.f90:256) */
 
+/* relaxation of the analysis (lekf_set_relax) */
+#define LEKF_RELAX_NONE 0
+#define LEKF_RELAX_RTPP 1  /* relaxation to prior perturbations (Zhang et al. 2005) */
+#define LEKF_RELAX_RTPS 2  /* relaxation to prior spread (Whitaker and Hamill 2012) */
+
 /* job flags (lekf_job.flags) */
 #define LEKF_F_WLOC 1        /* rdiag already err^2/rho (rdiag_wloc=.true.) */
 #define LEKF_F_MINFL 2       /* minfl present */
@@ -225,6 +230,17 @@ int lekf_das_run(lekf_handle *h);
  * lekf_cycle_swap runs eagerly and captures the graph; later runs launch it.
  * The graph holds no memset or allocation. */
 int lekf_set_graph(lekf_handle *h, int enable);
+/* Driver options of the SCALE/GFS das_letkf applied after the local solves
+ * (scale/letkf/letkf_tools.f90:215-268): mode LEKF_RELAX_RTPP relaxes the
+ * analysis perturbations to the background's with weight alpha
+ * (RELAX_ALPHA, weight_RTPP :1224-1236), LEKF_RELAX_RTPS relaxes the spread
+ * (RELAX_ALPHA_SPREAD, weight_RTPS :1240-1268); q_sprd_max > 0 limits
+ * spread / mean of 3-D row iq (Q_SPRD_MAX :255-268); boundary_width > 0
+ * (regional handles, metres) scales the update down within that distance of
+ * the lateral boundary (relax_beta's BOUNDARY_BUFFER_WIDTH :1203-1216).
+ * Default: all off. */
+int lekf_set_relax(lekf_handle *h, int mode, double alpha, double q_sprd_max, int iq,
+                   double boundary_width);
 /* the two halves of lekf_das_run: selection kernels, then the local solves + apply */
 int lekf_das_select(lekf_handle *h);
 int lekf_das_solve(lekf_handle *h);

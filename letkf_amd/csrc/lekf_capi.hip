@@ -17,6 +17,7 @@
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
 #include "lekf_obsope.hpp"
+#include "lekf_relax.hpp"
 #include "lekf_tune.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
@@ -519,6 +520,9 @@ struct lekf_handle {
     bool use_graph = false, graph_ok = false;
     hipGraphExec_t gexec = nullptr;
     long graph_epoch = 0;  // tuning epoch the graph was captured under
+    // relaxation pass after the solves (lekf_set_relax)
+    int relax_mode = LEKF_RELAX_NONE, relax_iq = -1;
+    double relax_alpha = 0.0, relax_qmax = 0.0, relax_width = 0.0;
     // local lists
     DevBuf cand_cnt, cand_off, loc_off, loc_cnt, loc_idx, loc_rho, status;
     long loc_cap = 0;
@@ -1295,6 +1299,40 @@ int lekf_das_select(lekf_handle* h)
     return run_select(h);
 }
 
+static bool relax_active(const lekf_handle* h)
+{
+    return h->relax_mode != LEKF_RELAX_NONE || h->relax_qmax > 0.0 || h->relax_width > 0.0;
+}
+
+// the relaxation pass over every point's analysis rows (after all solves)
+static int run_relax(lekf_handle* h)
+{
+    if (!relax_active(h)) return LEKF_OK;
+    RelaxArgs A{};
+    A.npt = (long)h->nij * h->nlev;
+    A.nij = h->nij;
+    A.ne = h->ne;
+    A.nv3d = h->nv3d;
+    A.nv2d = h->nv2d;
+    A.mode = h->relax_mode;
+    A.alpha = h->relax_alpha;
+    A.qmax = h->relax_qmax;
+    A.iq = h->relax_iq;
+    A.width = h->relax_width;
+    if (h->kind == 2) {
+        A.nx = h->r.nx;
+        A.ny = h->r.ny;
+        A.dx = h->r.dx;
+        A.dy = h->r.dy;
+    }
+    A.x3b = h->x3b.as<double>();
+    A.x3a = h->x3a.as<double>();
+    A.x2b = h->nv2d ? h->x2b.as<double>() : nullptr;
+    A.x2a = h->nv2d ? h->x2a.as<double>() : nullptr;
+    HIPCHK(lekf_relax_launch(A, h->st));
+    return LEKF_OK;
+}
+
 int lekf_das_solve(lekf_handle* h)
 {
     if (!h || !h->have_ens || !h->have_obs) return LEKF_ERR_ARG;
@@ -1303,8 +1341,10 @@ int lekf_das_solve(lekf_handle* h)
         if (h->chunk_pts.size() != 2) return LEKF_ERR_ARG;
         npt = h->chunk_pts[1];
     }
-    return run_solve(h->djob.as<lekf_job>(), 1, npt, h->ne, job_rows(h->job), mode_of_prec(h->prec),
-                     h->gws, h->st);
+    const int rc = run_solve(h->djob.as<lekf_job>(), 1, npt, h->ne, job_rows(h->job),
+                             mode_of_prec(h->prec), h->gws, h->st);
+    if (rc) return rc;
+    return run_relax(h);
 }
 
 int das_run_eager(lekf_handle* h)
@@ -1318,7 +1358,7 @@ int das_run_eager(lekf_handle* h)
                            h->ne, job_rows(h->job), mode_of_prec(h->prec), h->gws, h->st);
             if (rc) return rc;
         }
-        return LEKF_OK;
+        return run_relax(h);
     }
     int rc = lekf_das_select(h);
     if (rc) return rc;
@@ -1356,6 +1396,25 @@ int lekf_das_run(lekf_handle* h)
     HIPCHK(ei);
     h->graph_ok = true;
     h->graph_epoch = epoch;
+    return LEKF_OK;
+}
+
+int lekf_set_relax(lekf_handle* h, int mode, double alpha, double q_sprd_max, int iq,
+                   double boundary_width)
+{
+    if (!h) return LEKF_ERR_ARG;
+    if (mode < LEKF_RELAX_NONE || mode > LEKF_RELAX_RTPS || !(alpha >= 0.0) ||
+        !(q_sprd_max >= 0.0) || !(boundary_width >= 0.0))
+        return LEKF_ERR_ARG;
+    if (q_sprd_max > 0.0 && (iq < 0 || iq >= h->nv3d)) return LEKF_ERR_ARG;
+    if (boundary_width > 0.0 && h->kind != 2) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    graph_drop(h);
+    h->relax_mode = mode;
+    h->relax_alpha = alpha;
+    h->relax_qmax = q_sprd_max;
+    h->relax_iq = q_sprd_max > 0.0 ? iq : -1;
+    h->relax_width = boundary_width;
     return LEKF_OK;
 }
 
@@ -1595,6 +1654,10 @@ int run_batch_on(lekf_handle** hs, int n, hipStream_t st)
             if (mode_of_prec(h->prec) == mode) J.push_back(h->job);
         if (J.empty()) continue;
         const int rc = launch_grouped(J, mode, B.jobs[mode], B.ws, st);
+        if (rc) return rc;
+    }
+    for (lekf_handle* h : batched) {
+        const int rc = run_relax(h);
         if (rc) return rc;
     }
     return LEKF_OK;

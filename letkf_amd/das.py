@@ -56,6 +56,13 @@ class _Handle:
     def run(self):
         check(_lib.lib().lekf_das_run(self.h), "das_run")
 
+    def set_relax(self, mode="none", alpha=0.0, q_sprd_max=0.0, iq=-1, boundary_width=0.0):
+        """SCALE/GFS driver options after the local solves: mode "rtpp" / "rtps" with
+        weight alpha, the q spread limit on 3-D row iq, and (regional) the lateral
+        boundary buffer width in metres."""
+        check(_lib.lib().lekf_set_relax(self.h, _lib.RELAX[mode], float(alpha), float(q_sprd_max),
+                                        int(iq), float(boundary_width)), "set_relax")
+
     def set_graph(self, enable=True):
         """Replay each cycle's launches as one HIP graph (captured on the first run
         after any change of inputs)."""

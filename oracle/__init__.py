@@ -50,6 +50,9 @@ def _lib_c():
                                       _dp, c_int, _dp]
         _c.orc_mtx_eigen.restype = c_int
         _c.orc_mtx_eigen.argtypes = [c_int, _dp, _dp, _dp, _ip]
+        _c.orc_relax_point.restype = None
+        _c.orc_relax_point.argtypes = [c_int, c_int, _dp, _dp, _dp, _dp, c_int, c_double,
+                                       c_double, c_double, c_int, _dp, _dp]
         _c.orc_distll_1.restype = c_double
         _c.orc_distll_1.argtypes = [c_double] * 4
         _c.orc_obs_local_l96.restype = c_int
@@ -159,6 +162,25 @@ def letkf_core(hdxb, rdiag, rloc, dep, parm_infl, *, transm=False, pao=False, rd
     # column-major (ne, ne) buffers: transpose the row-major view back
     return dict(trans=trans.T.copy(), transm=tm if transm else None,
                 pao=pa.T.copy() if pao else None, parm_infl=float(parm[0]), status=int(st))
+
+
+RELAX_NONE, RELAX_RTPP, RELAX_RTPS = 0, 1, 2
+
+
+def relax_point(xb, trans, transm, pa, mode, alpha, beta=1.0, q_sprd_max=0.0, q_row=-1):
+    """Relaxed analysis of one point (scale/letkf/letkf_tools.f90:215-268).
+
+    xb: (nrow, k) background rows; trans = Wa, transm = w-bar, pa = Pa from
+    letkf_core(..., transm=True, pao=True).  Returns (xa (nrow, k), infl (nrow)).
+    """
+    xb = _f64(np.atleast_2d(xb))
+    nrow, k = xb.shape
+    xa = np.zeros_like(xb)
+    infl = np.zeros(nrow)
+    _lib_c().orc_relax_point(k, nrow, _d(xb), _d(_f64(np.asarray(trans).T)), _d(_f64(transm)),
+                             _d(_f64(np.asarray(pa).T)), int(mode), float(alpha), float(beta),
+                             float(q_sprd_max), int(q_row), _d(xa), _d(infl))
+    return xa, infl
 
 
 def mtx_eigen(a, impl="c"):

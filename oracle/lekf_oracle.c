@@ -24,6 +24,10 @@
  *          the grid indices, the longitude wrap to column 1, latitude-table
  *          points mapping to integer rj, the obsope vertical and type rules
  *          (tests/test_oracle_obsope.py).
+ *   orc_relax_point -> RTPP with alpha 0 / 1, RTPS restoring the background
+ *          spread at alpha 1, beta = 0 giving the background, the spread/mean
+ *          of the limited q row, and a numpy restatement
+ *          (tests/test_oracle_relax.py).
  */
 #include <float.h>
 #include <math.h>
@@ -1091,3 +1095,78 @@ int orc_trans_xtoy(int elm, double rlon, double rlat, double rlev, double odat, 
     if (rk_out) *rk_out = rk;
     return 0;
 }
+
+/*
+ * Relaxation of one point's analysis, scale/letkf/letkf_tools.f90:215-268
+ * (weight_RTPP :1224-1236, weight_RTPS :1240-1268, relax_beta's beta, and the
+ * q spread limit).  xb: nrow background rows of k members (xb[r*k + m]);
+ * trans = Wa (k x k, column-major, letkf_core with transm), transm = w-bar,
+ * pa (k x k, column-major); mode 0 none, 1 RTPP, 2 RTPS; q_row = the row of
+ * the 3-D q variable (< 0: none).  Writes xa[r*k + m]; infl_out[r] (nullable)
+ * gets the RTPS inflation of each row.
+ */
+void orc_relax_point(int k, int nrow, const double *xb, const double *trans, const double *transm,
+                     const double *pa, int mode, double alpha, double beta, double q_sprd_max,
+                     int q_row, double *xa, double *infl_out)
+{
+    double *g = (double *)malloc(sizeof(double) * k);
+    double *wr = (double *)malloc(sizeof(double) * k * k);
+    double *tr = (double *)malloc(sizeof(double) * k * k);
+    for (int r = 0; r < nrow; ++r) {
+        const double *x = xb + (size_t)r * k;
+        double *a = xa + (size_t)r * k;
+        double mean = 0.0;
+        for (int m = 0; m < k; ++m) mean += x[m];
+        mean /= (double)k;
+        for (int m = 0; m < k; ++m) g[m] = x[m] - mean;
+        double infl = 1.0;
+        if (beta == 0.0) {
+            for (int m = 0; m < k; ++m) a[m] = mean + g[m];
+        } else {
+            if (mode == 1) {
+                for (int i = 0; i < k * k; ++i) wr[i] = (1.0 - alpha) * trans[i];
+                for (int m = 0; m < k; ++m) wr[m + m * k] += alpha;
+            } else if (mode == 2) {
+                double var_g = 0.0, var_a = 0.0;
+                for (int m = 0; m < k; ++m) {
+                    var_g += g[m] * g[m];
+                    for (int kk = 0; kk < k; ++kk) var_a += g[kk] * pa[kk + m * k] * g[m];
+                }
+                if (var_g > 0.0 && var_a > 0.0) {
+                    infl = alpha * sqrt(var_g / (var_a * (double)(k - 1))) - alpha + 1.0;
+                    for (int i = 0; i < k * k; ++i) wr[i] = trans[i] * infl;
+                } else {
+                    for (int i = 0; i < k * k; ++i) wr[i] = trans[i];
+                }
+            } else {
+                for (int i = 0; i < k * k; ++i) wr[i] = trans[i];
+            }
+            for (int m = 0; m < k; ++m) {
+                for (int kk = 0; kk < k; ++kk) tr[kk + m * k] = (wr[kk + m * k] + transm[kk]) * beta;
+                tr[m + m * k] += 1.0 - beta;
+            }
+            for (int m = 0; m < k; ++m) {
+                double s = mean;
+                for (int kk = 0; kk < k; ++kk) s += g[kk] * tr[kk + m * k];
+                a[m] = s;
+            }
+        }
+        if (infl_out) infl_out[r] = infl;
+        if (q_sprd_max > 0.0 && r == q_row) {
+            double qm = 0.0, qs = 0.0;
+            for (int m = 0; m < k; ++m) qm += a[m];
+            qm /= (double)k;
+            for (int m = 0; m < k; ++m) {
+                g[m] = a[m] - qm;
+                qs += g[m] * g[m];
+            }
+            qs = sqrt(qs / (double)(k - 1)) / qm;
+            if (qs > q_sprd_max)
+                for (int m = 0; m < k; ++m) a[m] = qm + g[m] * q_sprd_max / qs;
+        }
+    }
+    free(g);
+    free(wr);
+    free(tr);
+}
+
