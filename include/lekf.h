@@ -219,9 +219,18 @@ int lekf_obsope_global(lekf_handle *h, int nobs, const double *elm, const double
 int lekf_set_obs_regional(lekf_handle *h, int nobs, const double *ri, const double *rj,
                           const double *z, const double *err, const double *dep, const int *bucket,
                           const int *qc, const double *hdxf);
-/* Lorenz-96 obs: grid index gidx (0-based), value y, error err, hxf(ny,ne) */
+/* Lorenz-96 obs: grid index gidx (0-based), value y, error err, hxf(ny,ne)
+ * column-major.  hxf NULL: H(x) is taken on the GPU from the resident
+ * background (H selects grid points, lorenz96/obs/h_ope.f90:14-39). */
 int lekf_set_obs_l96(lekf_handle *h, int ny, const int *gidx, const double *y, const double *err,
                      const double *hxf);
+/* Lorenz-96 forecast of a served cycle: kt RK4 steps (tinteg_rk4 /
+ * lorenz96_core, lorenz96/model/lorenz96.f90:93-128,290-309) of every member
+ * of the latest analysis, with time step dt and forcing force; the forecast
+ * becomes the background of the next cycle (the analysis stays readable) and
+ * the updated inflation its input.  Each member's trajectory is bit-identical
+ * to the reference model's. */
+int lekf_l96_forecast(lekf_handle *h, int kt, double dt, double force);
 /* one analysis (selection + local solves + apply) on the handle's stream;
  * asynchronous.  The analysis stays on the device. */
 int lekf_das_run(lekf_handle *h);

@@ -206,6 +206,8 @@ def lib():
     L.lekf_mem_info.argtypes = [POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t)]
     L.lekf_das_run_batch.restype = c_int
     L.lekf_das_run_batch.argtypes = [POINTER(c_void_p), c_int, c_void_p]
+    L.lekf_l96_forecast.restype = c_int
+    L.lekf_l96_forecast.argtypes = [c_void_p, c_int, c_double, c_double]
     L.lekf_set_relax.restype = c_int
     L.lekf_set_relax.argtypes = [c_void_p, c_int, c_double, c_double, c_int, c_double]
     L.lekf_tune_set.restype = c_int
@@ -229,7 +231,7 @@ EXPORTED = [
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
-    "lekf_tune_get", "lekf_set_relax",
+    "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast",
 ]
 
 

@@ -18,6 +18,7 @@
 #include "lekf_regional.hpp"
 #include "lekf_obsope.hpp"
 #include "lekf_relax.hpp"
+#include "lekf_l96.hpp"
 #include "lekf_tune.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
@@ -1258,6 +1259,9 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
                      const double* hxf)
 {
     if (!h || h->kind != 1 || ny < 0) return LEKF_ERR_ARG;
+    if (!hxf && !h->have_ens) return LEKF_ERR_ARG;
+    for (int n = 0; n < ny; ++n)
+        if (gidx[n] < 0 || gidx[n] >= h->nij) return LEKF_ERR_ARG;
     if (h) graph_drop(h);
     hipStream_t st = h->st;
     const int ne = h->ne;
@@ -1265,9 +1269,16 @@ int lekf_set_obs_l96(lekf_handle* h, int ny, const int* gidx, const double* y, c
     int rc;
     if ((rc = upload(h->r_gidx, gidx, sizeof(int) * ny, st)) ||
         (rc = upload(h->r_y, y, sizeof(double) * ny, st)) ||
-        (rc = upload(h->r_err, err, sizeof(double) * ny, st)) ||
-        (rc = upload(h->r_hxf, hxf, sizeof(double) * ny * (size_t)ne, st)))
+        (rc = upload(h->r_err, err, sizeof(double) * ny, st)))
         return rc;
+    if (hxf) {
+        if ((rc = upload(h->r_hxf, hxf, sizeof(double) * ny * (size_t)ne, st))) return rc;
+    } else {
+        // H(x) of the resident background (lorenz96/obs/h_ope.f90:14-39)
+        HIPCHK(h->r_hxf.ensure(sizeof(double) * n * ne));
+        HIPCHK(lekf_l96_hx_launch(ny, ne, h->r_gidx.as<int>(), h->x3b.as<double>(),
+                                  h->r_hxf.as<double>(), st));
+    }
     HIPCHK(h->o_hdxf.ensure(sizeof(double) * n * ne));
     if (h->prec != LEKF_PREC_F64) HIPCHK(h->o_hdxf32.ensure(sizeof(float) * n * ne));
     HIPCHK(h->o_dep.ensure(sizeof(double) * n));
@@ -1819,6 +1830,18 @@ int lekf_cycle_swap(lekf_handle* h)
     graph_drop(h);
     std::swap(h->x3b, h->x3a);
     std::swap(h->x2b, h->x2a);
+    std::swap(h->infl_in, h->infl_out);
+    if (h->have_obs) return upload_jobs(h);
+    return LEKF_OK;
+}
+
+int lekf_l96_forecast(lekf_handle* h, int kt, double dt, double force)
+{
+    if (!h || h->kind != 1 || kt < 0 || !h->have_ens) return LEKF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    graph_drop(h);
+    HIPCHK(lekf_l96_rk4_launch(h->x3a.as<double>(), h->x3b.as<double>(), h->nij, h->ne, kt, dt,
+                               force, h->st));
     std::swap(h->infl_in, h->infl_out);
     if (h->have_obs) return upload_jobs(h);
     return LEKF_OK;

@@ -16,6 +16,10 @@
 #include "lekf_dev.hpp"
 #include "lekf_cull.hpp"
 
+// Every FP64 operation in this file is rounded as written: no FMA contraction
+// (the reference's results are reproduced operation by operation).
+#pragma clang fp contract(off)
+
 namespace {
 
 __device__ __forceinline__ unsigned long long lane_mask_lt(int lane)
@@ -150,11 +154,11 @@ __global__ void cull_global_count(GlobalCullArgs A)
 __device__ __forceinline__ double dist_gc(double sin_o, double cos_o, double lonr_o, double sin_g,
                                           double cos_g, double lonr_g)
 {
-    const double c = cos(__dsub_rn(lonr_g, lonr_o));
-    double cosd = __dadd_rn(__dmul_rn(sin_o, sin_g), __dmul_rn(__dmul_rn(cos_o, cos_g), c));
+    const double c = cos(rn_sub(lonr_g, lonr_o));
+    double cosd = rn_add(rn_mul(sin_o, sin_g), rn_mul(rn_mul(cos_o, cos_g), c));
     cosd = cosd < 1.0 ? cosd : 1.0;
     cosd = cosd > -1.0 ? cosd : -1.0;
-    return __dmul_rn(acos(cosd), LEKF_RE_REF);
+    return rn_mul(acos(cosd), LEKF_RE_REF);
 }
 
 __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
@@ -189,8 +193,8 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
     const long base = A.cand_off[ij] * (long)nlev - A.list_base;
     const double lat1 = A.col_lat[ij];
     const double r180 = 1.0 / 180.0;
-    const double lonr_g = __dmul_rn(__dmul_rn(A.col_lon[ij], LEKF_PI_REF), r180);
-    const double latr_g = __dmul_rn(__dmul_rn(lat1, LEKF_PI_REF), r180);
+    const double lonr_g = rn_mul(rn_mul(A.col_lon[ij], LEKF_PI_REF), r180);
+    const double latr_g = rn_mul(rn_mul(lat1, LEKF_PI_REF), r180);
     const double sin_g = sin(latr_g), cos_g = cos(latr_g);
     int seg = 0;
     for (int q0 = 0; q0 < ncand; q0 += 64) {
@@ -213,10 +217,10 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
             double dlev = 0.0;
             if (hok) {
                 if (typ == LEKF_TYP_PS) {
-                    dlev = (l > 0) ? fabs(__dsub_rn(A.o_lndat[o], A.logpfm[(size_t)l * A.nij + ij]))
+                    dlev = (l > 0) ? fabs(rn_sub(A.o_lndat[o], A.logpfm[(size_t)l * A.nij + ij]))
                                    : 0.0;
                 } else {
-                    dlev = fabs(__dsub_rn(A.o_lnlev[o], A.logpfm[(size_t)l * A.nij + ij]));
+                    dlev = fabs(rn_sub(A.o_lnlev[o], A.logpfm[(size_t)l * A.nij + ij]));
                 }
                 sel = !(dlev > A.dist_zerov);
             }
@@ -464,10 +468,10 @@ __global__ void obs_scatter(ObsPrepArgs A, int nbinned)
     }
     A.o_typ[m] = typ;
     const double r180 = 1.0 / 180.0;
-    const double latr = __dmul_rn(__dmul_rn(A.olat[n], LEKF_PI_REF), r180);
+    const double latr = rn_mul(rn_mul(A.olat[n], LEKF_PI_REF), r180);
     A.o_sinlat[m] = sin(latr);
     A.o_coslat[m] = cos(latr);
-    A.o_lonr[m] = __dmul_rn(__dmul_rn(A.olon[n], LEKF_PI_REF), r180);
+    A.o_lonr[m] = rn_mul(rn_mul(A.olon[n], LEKF_PI_REF), r180);
     A.o_lnlev[m] = (typ == 5) ? 0.0 : log(A.olev[n]);
     A.o_lndat[m] = (typ == 5) ? log(A.dat[n]) : 0.0;
 }

@@ -31,6 +31,32 @@ typedef __bf16 lekf_bf8 __attribute__((ext_vector_type(8)));
 
 #define LEKF_WAVE 64
 
+// Round-to-nearest FP64 operations that the compiler must not fuse into FMAs.
+// Device code is compiled with fp-contract=fast, and the __d*_rn functions of
+// the HIP headers are then plain operators that a neighbouring add may absorb
+// into an FMA; code that must round every product and sum as the reference
+// does (selection distances, obs operator, L96 model) uses these instead.
+__device__ __forceinline__ double rn_mul(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double rn_add(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double rn_sub(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double rn_div(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a / b;
+}
+
 // Jacobi: rotate a pair while |a_pq| > tol * sqrt(a_pp a_qq); stop after a
 // sweep with no rotation.
 #define LEKF_JACOBI_TOL 1.0e-15

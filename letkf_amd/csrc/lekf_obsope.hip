@@ -21,11 +21,15 @@
 #include "lekf_dev.hpp"
 #include "lekf_obsope.hpp"
 
+// Every FP64 operation in this file is rounded as written: no FMA contraction
+// (the reference's results are reproduced operation by operation).
+#pragma clang fp contract(off)
+
 namespace {
 
-__device__ __forceinline__ double mul(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ double add(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double sub(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ double mul(double a, double b) { return rn_mul(a, b); }
+__device__ __forceinline__ double add(double a, double b) { return rn_add(a, b); }
+__device__ __forceinline__ double sub(double a, double b) { return rn_sub(a, b); }
 
 // bilinear weights sum in the reference's order: v00 (1-ai)(1-aj) + v10 ai (1-aj)
 // + v01 (1-ai) aj + v11 ai aj
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(64) void obsope_global(ObsOpArgs A)
     if (rlon == 0.0 || rlon == 360.0)
         ri = (double)(nlon + 1);
     else
-        ri = add(mul(__ddiv_rn(rlon, 360.0), (double)nlon), 1.0);
+        ri = add(mul(rn_div(rlon, 360.0), (double)nlon), 1.0);
     if (ceil(ri) < 2 || nlon + 1 < ceil(ri)) ok = false;
     // latitude -> rj
     double rj = 0.0;
@@ -63,12 +67,12 @@ __global__ __launch_bounds__(64) void obsope_global(ObsOpArgs A)
         for (; j <= nlat; ++j)
             if (rlat < A.lat[j - 1]) break;
         if (j == 1) {
-            rj = __ddiv_rn(add(rlat, 90.0), add(A.lat[0], 90.0));
+            rj = rn_div(add(rlat, 90.0), add(A.lat[0], 90.0));
         } else if (j == nlat + 1) {
-            const double aj = __ddiv_rn(sub(rlat, A.lat[nlat - 1]), sub(90.0, A.lat[nlat - 1]));
+            const double aj = rn_div(sub(rlat, A.lat[nlat - 1]), sub(90.0, A.lat[nlat - 1]));
             rj = add((double)nlat, aj);
         } else {
-            const double aj = __ddiv_rn(sub(rlat, A.lat[j - 2]), sub(A.lat[j - 1], A.lat[j - 2]));
+            const double aj = rn_div(sub(rlat, A.lat[j - 2]), sub(A.lat[j - 1], A.lat[j - 2]));
             rj = add((double)(j - 1), aj);
         }
         if (ceil(rj) < 2 || nlat < ceil(rj)) ok = false;
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(64) void obsope_global(ObsOpArgs A)
                     pk1 = pk;
                 }
                 if (k == nlev) pk = plev(nlev);
-                const double ak = __ddiv_rn(sub(lr, pk1), sub(pk, pk1));
+                const double ak = rn_div(sub(lr, pk1), sub(pk, pk1));
                 rk = add((double)(k - 1), ak);
                 if (ceil(rk) > nlev) okm = false;
                 if (okm && ceil(rk) < 2) {

@@ -22,6 +22,10 @@
 #include "lekf_regional.hpp"
 #include "lekf_tune.hpp"
 
+// Every FP64 operation in this file is rounded as written: no FMA contraction
+// (the reference's results are reproduced operation by operation).
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr double kNdMax = 13.33333333;  // scale/letkf/letkf_obs.f90:30
@@ -87,12 +91,12 @@ __device__ __forceinline__ Cand eval_rec(const RegionalArgs& A, const ObsRec& r,
                                          double rj, double rz)
 {
     Cand c;
-    const double rdx = __dmul_rn(__dsub_rn(ri, r.ri), A.dx);
-    const double rdy = __dmul_rn(__dsub_rn(rj, r.rj), A.dy);
-    const double nd_h = __ddiv_rn(__dsqrt_rn(__dadd_rn(__dmul_rn(rdx, rdx), __dmul_rn(rdy, rdy))),
+    const double rdx = rn_mul(rn_sub(ri, r.ri), A.dx);
+    const double rdy = rn_mul(rn_sub(rj, r.rj), A.dy);
+    const double nd_h = rn_div(__dsqrt_rn(rn_add(rn_mul(rdx, rdx), rn_mul(rdy, rdy))),
                                   A.sigma_h);
-    const double nd_v = __ddiv_rn(fabs(__dsub_rn(r.z, rz)), A.sigma_v);
-    const double nd = __dadd_rn(__dmul_rn(nd_h, nd_h), __dmul_rn(nd_v, nd_v));
+    const double nd_v = rn_div(fabs(rn_sub(r.z, rz)), A.sigma_v);
+    const double nd = rn_add(rn_mul(nd_h, nd_h), rn_mul(nd_v, nd_v));
     c.o = o;
     c.ok = !(nd > kNdMax);
     c.b = r.b;
@@ -101,7 +105,7 @@ __device__ __forceinline__ Cand eval_rec(const RegionalArgs& A, const ObsRec& r,
     c.bits = 0;
     if (c.ok) {
         c.rho = exp(-0.5 * nd);
-        c.rd = __ddiv_rn(__dmul_rn(r.err, r.err), c.rho);
+        c.rd = rn_div(rn_mul(r.err, r.err), c.rho);
         c.bits = (unsigned long long)__double_as_longlong(c.rd);
     }
     return c;
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
             for (int i = tid; i < n; i += kSelT) {
                 const int o = A.in_idx[off + i];
                 const double rho = A.in_rho[off + i];
-                skey[i] = (unsigned long long)__double_as_longlong(__ddiv_rn(A.rdiag[o], rho));
+                skey[i] = (unsigned long long)__double_as_longlong(rn_div(A.rdiag[o], rho));
                 sob[i] = o;
                 srho[i] = rho;
             }

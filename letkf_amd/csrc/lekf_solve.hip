@@ -1231,10 +1231,10 @@ __device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
             const double wc = lane_bcast(c < 64 ? w0 : w1, c & 63);
             double* row = A + (o + c) * lda + o;
             if (lane < m)
-                row[lane] = __dsub_rn(row[lane], __dadd_rn(__dmul_rn(vc, w0), __dmul_rn(wc, v0)));
+                row[lane] = rn_sub(row[lane], rn_add(rn_mul(vc, w0), rn_mul(wc, v0)));
             if (lane + 64 < m)
                 row[lane + 64] =
-                    __dsub_rn(row[lane + 64], __dadd_rn(__dmul_rn(vc, w1), __dmul_rn(wc, v1)));
+                    rn_sub(row[lane + 64], rn_add(rn_mul(vc, w1), rn_mul(wc, v1)));
         }
         __syncthreads();
     }
@@ -1398,8 +1398,8 @@ __device__ void big_tridiag(double* A, int n, int lda, double* d, double* e, dou
 #pragma unroll
                     for (int t = 0; t < NR; ++t)
                         if (c0 + u < cn && lane + 64 * t < m)
-                            row[lane + 64 * t] = __dsub_rn(
-                                x[u][t], __dadd_rn(__dmul_rn(vc, w[t]), __dmul_rn(wc, v[t])));
+                            row[lane + 64 * t] = rn_sub(
+                                x[u][t], rn_add(rn_mul(vc, w[t]), rn_mul(wc, v[t])));
                 }
             }
         }
@@ -1512,10 +1512,10 @@ __device__ void big_tridiag_sym(double* A, int n, int lda, double* d, double* e,
             const int i = lane + 64 * t, r = o + i;
             x[t] = 0.0;
             if (i < m)
-                x[t] = __dsub_rn(rows[r], __dadd_rn(__dmul_rn(vps, wP[r]), __dmul_rn(wps, vP[r])));
+                x[t] = rn_sub(rows[r], rn_add(rn_mul(vps, wP[r]), rn_mul(wps, vP[r])));
             if (i > 0) s2 = fma(x[t], x[t], s2);
         }
-        const double dss = __dsub_rn(rows[s], __dadd_rn(__dmul_rn(vps, wps), __dmul_rn(wps, vps)));
+        const double dss = rn_sub(rows[s], rn_add(rn_mul(vps, wps), rn_mul(wps, vps)));
         const double sigma = wave_sum(s2);
         const double alpha = lane_bcast(x[0], 0);
         double tau = 0.0, beta = alpha, scal = 0.0;
@@ -1580,8 +1580,8 @@ __device__ void big_tridiag_sym(double* A, int n, int lda, double* d, double* e,
                     for (int t = 0; t < NR; ++t) {
                         const int i = lane + 64 * t;
                         if (rv && t >= cb && i >= ic && i < m) {
-                            const double an = __dsub_rn(
-                                a[u][t], __dadd_rn(__dmul_rn(vpc, wpr[t]), __dmul_rn(wpc, vpr[t])));
+                            const double an = rn_sub(
+                                a[u][t], rn_add(rn_mul(vpc, wpr[t]), rn_mul(wpc, vpr[t])));
                             row[i] = an;
                             if (i > ic) {
                                 pp[t] = fma(an, vnc, pp[t]);
@@ -1616,12 +1616,12 @@ __device__ void big_tridiag_sym(double* A, int n, int lda, double* d, double* e,
     }
     if (lane == 0) {
         const int a0 = n - 2, a1 = n - 1;
-        d[a0] = __dsub_rn(A[(size_t)a0 * lda + a0],
-                          __dadd_rn(__dmul_rn(vP[a0], wP[a0]), __dmul_rn(wP[a0], vP[a0])));
-        e[a0] = __dsub_rn(A[(size_t)a0 * lda + a1],
-                          __dadd_rn(__dmul_rn(vP[a0], wP[a1]), __dmul_rn(wP[a0], vP[a1])));
-        d[a1] = __dsub_rn(A[(size_t)a1 * lda + a1],
-                          __dadd_rn(__dmul_rn(vP[a1], wP[a1]), __dmul_rn(wP[a1], vP[a1])));
+        d[a0] = rn_sub(A[(size_t)a0 * lda + a0],
+                          rn_add(rn_mul(vP[a0], wP[a0]), rn_mul(wP[a0], vP[a0])));
+        e[a0] = rn_sub(A[(size_t)a0 * lda + a1],
+                          rn_add(rn_mul(vP[a0], wP[a1]), rn_mul(wP[a0], vP[a1])));
+        d[a1] = rn_sub(A[(size_t)a1 * lda + a1],
+                          rn_add(rn_mul(vP[a1], wP[a1]), rn_mul(wP[a1], vP[a1])));
         e[a1] = 0.0;
     }
     __syncthreads();
@@ -1833,7 +1833,7 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
             if (t > j && t < n) {
                 const double vt = lane_bcast(a[t], j);
                 const double wt = lane_bcast(wr, t);
-                if (act) a[t] = __dsub_rn(a[t], __dadd_rn(__dmul_rn(vt, wr), __dmul_rn(wt, vr)));
+                if (act) a[t] = rn_sub(a[t], rn_add(rn_mul(vt, wr), rn_mul(wt, vr)));
                 q[t] = fma(-yr, vt, q[t]);
             }
         }

@@ -207,13 +207,22 @@ class L96Letkf(_Handle):
             raise _lib.LetkfError(st.value, "lekf_l96_create")
         self.nij, self.nlev, self.ne, self.nv3d, self.nv2d = nx, 1, ne, 1, 0
 
-    def set_obs(self, gidx, y, err, hxf):
+    def set_obs(self, gidx, y, err, hxf=None):
+        """Obs at grid points gidx; hxf (ny, ne) or None for H(x) of the resident
+        background computed on the GPU."""
         g = np.ascontiguousarray(gidx, dtype=np.int32)
         y = np.ascontiguousarray(y, dtype=np.float64)
         e = np.ascontiguousarray(err, dtype=np.float64)
-        hx = _f(hxf)
+        hx = _f(hxf) if hxf is not None else None
         check(_lib.lib().lekf_set_obs_l96(self.h, g.size, iptr(g), dptr(y), dptr(e),
-                                          hx.ctypes.data_as(_lib._dp)), "set_obs_l96")
+                                          hx.ctypes.data_as(_lib._dp) if hx is not None else None),
+              "set_obs_l96")
+
+    def forecast(self, kt, dt=0.005, force=8.0):
+        """kt RK4 steps of the Lorenz-96 model from the latest analysis; the
+        forecast becomes the next cycle's background (lorenz96/model/lorenz96.f90:93-128)."""
+        check(_lib.lib().lekf_l96_forecast(self.h, int(kt), float(dt), float(force)),
+              "l96_forecast")
 
 
 class RegionalLetkf(_Handle):

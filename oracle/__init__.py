@@ -50,6 +50,8 @@ def _lib_c():
                                       _dp, c_int, _dp]
         _c.orc_mtx_eigen.restype = c_int
         _c.orc_mtx_eigen.argtypes = [c_int, _dp, _dp, _dp, _ip]
+        _c.orc_l96_rk4.restype = None
+        _c.orc_l96_rk4.argtypes = [c_int, c_int, c_double, c_double, _dp]
         _c.orc_relax_point.restype = None
         _c.orc_relax_point.argtypes = [c_int, c_int, _dp, _dp, _dp, _dp, c_int, c_double,
                                        c_double, c_double, c_int, _dp, _dp]
@@ -104,6 +106,10 @@ def _lib_ref():
                                         c_int, _dp, c_int, c_int, _dp, c_int]
         _ref.ref_mtx_eigen.restype = None
         _ref.ref_mtx_eigen.argtypes = [c_int, _dp, _dp, _dp, _ip]
+        _ref.ref_l96_rk4.restype = None
+        _ref.ref_l96_rk4.argtypes = [c_int, c_double, c_double, _dp, _dp]
+        _ref.ref_l96_nx.restype = c_int
+        _ref.ref_l96_nx.argtypes = []
         _ref.ref_solve_apply.restype = c_int
         _ref.ref_solve_apply.argtypes = [c_int, c_int, c_int, c_int, _lp, _ip, _dp, _dp, _dp,
                                          _dp, c_int, _dp, _dp, _dp, c_double]
@@ -162,6 +168,23 @@ def letkf_core(hdxb, rdiag, rloc, dep, parm_infl, *, transm=False, pao=False, rd
     # column-major (ne, ne) buffers: transpose the row-major view back
     return dict(trans=trans.T.copy(), transm=tm if transm else None,
                 pao=pa.T.copy() if pao else None, parm_infl=float(parm[0]), status=int(st))
+
+
+def l96_rk4(x, kt, dt=0.005, force=8.0, impl="c"):
+    """kt RK4 steps of Lorenz-96 (lorenz96/model/lorenz96.f90:93-128,290-309).
+
+    impl "c": the restatement, any nx; "ref": the reference model itself
+    (oracle/_ref, its nx = 40 only)."""
+    x = _f64(x).copy()
+    if impl == "c":
+        _lib_c().orc_l96_rk4(x.size, int(kt), float(dt), float(force), _d(x))
+        return x
+    lib = _lib_ref()
+    if x.size != lib.ref_l96_nx():
+        raise ValueError("the reference model is compiled for nx = %d" % lib.ref_l96_nx())
+    out = np.zeros_like(x)
+    lib.ref_l96_rk4(int(kt), float(dt), float(force), _d(x), _d(out))
+    return out
 
 
 RELAX_NONE, RELAX_RTPP, RELAX_RTPS = 0, 1, 2

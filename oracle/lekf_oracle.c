@@ -24,6 +24,9 @@
  *          the grid indices, the longitude wrap to column 1, latitude-table
  *          points mapping to integer rj, the obsope vertical and type rules
  *          (tests/test_oracle_obsope.py).
+ *   orc_l96_rk4 -> the reference model compiled from its sources
+ *          (oracle/_ref, nx = 40) bit for bit, the fixed point x_i = F, and
+ *          step-size convergence (tests/test_oracle_l96.py).
  *   orc_relax_point -> RTPP with alpha 0 / 1, RTPS restoring the background
  *          spread at alpha 1, beta = 0 giving the background, the spread/mean
  *          of the limited q row, and a numpy restatement
@@ -1168,5 +1171,37 @@ void orc_relax_point(int k, int nrow, const double *xb, const double *trans, con
     free(g);
     free(wr);
     free(tr);
+}
+
+/*
+ * Lorenz-96 model, lorenz96/model/lorenz96.f90:93-128 (tinteg_rk4) and
+ * :290-309 (lorenz96_core) for any nx: kt RK4 steps of x (nx values) in
+ * place.  Operations in the reference's order.
+ */
+static void orc_l96_core(int nx, const double *x, double *q, double dt, double force)
+{
+    for (int i = 0; i < nx; ++i) {
+        const int im1 = i == 0 ? nx - 1 : i - 1;
+        const int im2 = i < 2 ? i + nx - 2 : i - 2;
+        const int ip1 = i == nx - 1 ? 0 : i + 1;
+        q[i] = dt * (x[im1] * (x[ip1] - x[im2]) - x[i] + force);
+    }
+}
+
+void orc_l96_rk4(int nx, int kt, double dt, double force, double *x)
+{
+    double *xt = (double *)malloc(sizeof(double) * nx * 5);
+    double *q1 = xt + nx, *q2 = q1 + nx, *q3 = q2 + nx, *q4 = q3 + nx;
+    for (int s = 0; s < kt; ++s) {
+        orc_l96_core(nx, x, q1, dt, force);
+        for (int i = 0; i < nx; ++i) xt[i] = x[i] + 0.5 * q1[i];
+        orc_l96_core(nx, xt, q2, dt, force);
+        for (int i = 0; i < nx; ++i) xt[i] = x[i] + 0.5 * q2[i];
+        orc_l96_core(nx, xt, q3, dt, force);
+        for (int i = 0; i < nx; ++i) xt[i] = x[i] + q3[i];
+        orc_l96_core(nx, xt, q4, dt, force);
+        for (int i = 0; i < nx; ++i) x[i] = x[i] + (q1[i] + 2.0 * q2[i] + 2.0 * q3[i] + q4[i]) / 6.0;
+    }
+    free(xt);
 }
 

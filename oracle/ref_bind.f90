@@ -14,14 +14,31 @@
 !                     scale/letkf/letkf_tools.f90:175 OpenMP pattern):
 !                     gather the local rows, call the reference
 !                     letkf_core, apply Xa = xmean + Xb' * trans.
+! ref_l96_rk4      -> lorenz96/model/lorenz96.f90:93-128 (tinteg_rk4, the
+!                     reference model with its nx = 40), dt and force set
 !=======================================================================
 MODULE ref_bind
   USE ISO_C_BINDING
   USE common, ONLY: r_size
   USE common_mtx, ONLY: mtx_eigen
   USE common_letkf, ONLY: letkf_core
+  USE lorenz96, ONLY: tinteg_rk4, l96_nx => nx, l96_dt => dt, l96_force => force
   IMPLICIT NONE
 CONTAINS
+
+SUBROUTINE ref_l96_rk4(kt, dt, force, xin, xout) BIND(C, NAME="ref_l96_rk4")
+  INTEGER(C_INT), VALUE :: kt
+  REAL(C_DOUBLE), VALUE :: dt, force
+  REAL(C_DOUBLE), INTENT(IN) :: xin(l96_nx)
+  REAL(C_DOUBLE), INTENT(OUT) :: xout(l96_nx)
+  l96_dt = dt
+  l96_force = force
+  CALL tinteg_rk4(kt, xin, xout)
+END SUBROUTINE ref_l96_rk4
+
+INTEGER(C_INT) FUNCTION ref_l96_nx() BIND(C, NAME="ref_l96_nx")
+  ref_l96_nx = l96_nx
+END FUNCTION ref_l96_nx
 
 SUBROUTINE ref_letkf_core(ne, nobs, nobsl, hdxb, rdiag, rloc, dep, parm, trans, &
      & transm, has_transm, pao, has_pao, rdiag_wloc, minfl, has_minfl) BIND(C, NAME="ref_letkf_core")
