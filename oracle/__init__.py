@@ -115,6 +115,18 @@ def _lib_ref():
                                          _dp, c_int, _dp, _dp, _dp, c_double]
         _ref.ref_max_threads.restype = c_int
         _ref.ref_max_threads.argtypes = []
+        _ref.ref_spd_grid.restype = None
+        _ref.ref_spd_grid.argtypes = [_ip]
+        for f in (_ref.ref_spd_read_grd, _ref.ref_spd_write_grd):
+            f.restype = None
+            f.argtypes = [ctypes.c_char_p, c_int, _dp, _dp]
+        _ref.ref_spd_get_nobs.restype = c_int
+        _ref.ref_spd_get_nobs.argtypes = [ctypes.c_char_p, c_int, c_int]
+        _ref.ref_spd_read_obs.restype = None
+        _ref.ref_spd_read_obs.argtypes = [ctypes.c_char_p, c_int, c_int] + [_dp] * 6
+        for f in (_ref.ref_spd_read_obs2, _ref.ref_spd_write_obs2):
+            f.restype = None
+            f.argtypes = [ctypes.c_char_p, c_int, c_int] + [_dp] * 7 + [_ip]
     return _ref
 
 
@@ -468,3 +480,71 @@ def ref_solve_apply(loc_off, loc_idx, loc_rho, hdxf, rdiag, dep, xb, infl, fixed
 
 def ref_threads() -> int:
     return int(_lib_ref().ref_max_threads())
+
+
+# ---------------------------------------------------------------------------
+# SPEEDY files through the reference's own routines (native byte order)
+# ---------------------------------------------------------------------------
+def ref_speedy_grid():
+    """(nlon, nlat, nlev, nv3d, nv2d) of speedy/common/common_speedy.f90:18-22."""
+    d = np.zeros(5, dtype=np.int32)
+    _lib_ref().ref_spd_grid(_i(d))
+    return tuple(int(v) for v in d)
+
+
+def ref_read_grd(path):
+    """read_grd (speedy/common/common_speedy.f90:163-194): (v3d, v2d) in float64."""
+    nlon, nlat, nlev, nv3d, nv2d = ref_speedy_grid()
+    v3 = np.zeros((nlon, nlat, nlev, nv3d), order="F")
+    v2 = np.zeros((nlon, nlat, nv2d), order="F")
+    b = os.fsencode(path)
+    _lib_ref().ref_spd_read_grd(b, len(b), _d(v3), _d(v2))
+    return v3, v2
+
+
+def ref_write_grd(path, v3d, v2d):
+    """write_grd (speedy/common/common_speedy.f90:226-257)."""
+    v3 = np.asfortranarray(v3d, dtype=np.float64)
+    v2 = np.asfortranarray(v2d, dtype=np.float64)
+    g = ref_speedy_grid()
+    if v3.shape != g[:4] or v2.shape != g[:2] + g[4:]:
+        raise ValueError("grid shape differs from the reference's SPEEDY grid")
+    b = os.fsencode(path)
+    _lib_ref().ref_spd_write_grd(b, len(b), _d(v3), _d(v2))
+
+
+def ref_get_nobs(path, nrec=8):
+    """get_nobs (speedy/common/common_obs_speedy.f90:400-456): the record count."""
+    b = os.fsencode(path)
+    return int(_lib_ref().ref_spd_get_nobs(b, len(b), int(nrec)))
+
+
+_OBS_KEYS = ("elem", "rlon", "rlat", "rlev", "odat", "oerr")
+
+
+def ref_read_obs(path, nn):
+    """read_obs (speedy/common/common_obs_speedy.f90:458-502), 6-value records."""
+    out = {k: np.zeros(nn) for k in _OBS_KEYS}
+    b = os.fsencode(path)
+    _lib_ref().ref_spd_read_obs(b, len(b), int(nn), *[_d(out[k]) for k in _OBS_KEYS])
+    return out
+
+
+def ref_read_obs2(path, nn):
+    """read_obs2 (speedy/common/common_obs_speedy.f90:504-552), 8-value records."""
+    out = {k: np.zeros(nn) for k in _OBS_KEYS + ("ohx",)}
+    out["oqc"] = np.zeros(nn, dtype=np.int32)
+    b = os.fsencode(path)
+    _lib_ref().ref_spd_read_obs2(b, len(b), int(nn), *[_d(out[k]) for k in _OBS_KEYS + ("ohx",)],
+                                 _i(out["oqc"]))
+    return out
+
+
+def ref_write_obs2(path, obs):
+    """write_obs2 (speedy/common/common_obs_speedy.f90:554-602); obs in the
+    in-memory units (Pa levels, Pa ps, fractional rh)."""
+    arr = {k: _f64(obs[k]) for k in _OBS_KEYS + ("ohx",)}
+    qc = np.ascontiguousarray(obs["oqc"], dtype=np.int32)
+    b = os.fsencode(path)
+    _lib_ref().ref_spd_write_obs2(b, len(b), int(qc.size), *[_d(arr[k]) for k in _OBS_KEYS + ("ohx",)],
+                                  _i(qc))
