@@ -217,6 +217,7 @@ def _obsin(p, d):
                    np.where(e == synth.ID_PS, dat * 0.01, dat),
                    np.where(e == synth.ID_PS, o["err"] * 0.01, o["err"])], axis=1)
     sio.write_records(os.path.join(d, "obsin.dat"), wk.astype(np.float32), big_endian=False)
+    return np.concatenate([idx3[:10], idx_ps[:3]])
 
 
 def test_cycle_equals_obsope_then_letkf(gpu, tmp_path):
@@ -229,7 +230,7 @@ def test_cycle_equals_obsope_then_letkf(gpu, tmp_path):
     for m in range(NE):                        # the same background, no member obs files
         os.link(fa.gues(m), fb.gues(m))
         os.remove(fa.obs(m))
-    _obsin(p, str(da))
+    planted = _obsin(p, str(da))
     _obsin(p, str(db))
     ro = sc.obsope(str(da), ne=NE, big_endian=False)
     assert 0 < ro["accepted"] <= ro["nobs"] - 13
@@ -240,10 +241,13 @@ def test_cycle_equals_obsope_then_letkf(gpu, tmp_path):
         assert open(fa.anal(m), "rb").read() == open(fb.anal(m), "rb").read(), m
     for name in ("infl_mul.grd", "anal_me.grd", "anal_sp.grd", "gues_me.grd", "gues_sp.grd"):
         assert open(fa.path(name), "rb").read() == open(fb.path(name), "rb").read(), name
-    # obsope's files read by the reference: H(x) 0 and qc 0 on the rejected records
-    r = oracle.ref_read_obs2(fa.obs(0), ro["nobs"])
-    assert np.all(r["ohx"][r["oqc"] == 0] == 0.0)
-    assert (r["oqc"] == 0).sum() >= 13
+    # obsope's files read by the reference: H(x) 0 and qc 0 on the records every
+    # member rejects, H(x) set on the records every member accepts
+    for m in (0, NE - 1):
+        r = oracle.ref_read_obs2(fa.obs(m), ro["nobs"])
+        assert np.all(r["oqc"][planted] == 0) and np.all(r["ohx"][planted] == 0.0)
+        assert np.all(r["ohx"][r["oqc"] == 1] != 0.0)
+        assert (r["oqc"] == 1).sum() == ro["accepted"]
 
 
 def test_fixed_inflation_and_no_obs(gpu, tmp_path):
