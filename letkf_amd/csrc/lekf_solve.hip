@@ -316,53 +316,73 @@ __device__ __forceinline__ double adaptive_alpha(double alpha, double p1, double
 }
 
 // ===========================================================================
-// k <= 32: fused kernel, one wave per point, register Jacobi
+// k <= 32: fused kernel, several points per wave, register Jacobi
 // ===========================================================================
 
 struct SolveLds {
-    double* A;    // n x lda   (C, later T)
-    double* V;    // k x lda   (eigenvectors, columns)
+    double* A;    // n x lda   C (the Gram), later V (eigenvectors, columns)
     double* B;    // k   (Y^T R^-1 d)
     double* L;    // n   eigenvalues
-    double* U;    // k
+    double* U;    // k   L^-1 V^T b, later sqrt((k-1)/L)
     double* Wb;   // k   w-bar
-    double* X;    // LEKF_NVMAX * k   state perturbation rows (aliases V)
-    double* M;    // LEKF_NVMAX       row means (aliases V)
-    double* sc;   // 8 scalars
+    double* X;    // nvmax * kmax  state perturbation rows of the apply
+    double* M;    // nvmax         row means
+    double* Z;    // nvmax * kmax  X' V diag(U)
+    double* D;    // nvmax         X' w-bar
+    double* sc;   // 8 scalars: sum w d^2, P2, P3, alpha, k, has obs, status
 };
 
-__host__ __device__ inline size_t solve_lds_doubles(int k)
+// LDS of the fused kernel for ensemble sizes up to kmax and up to nvmax state
+// rows per point: one region per point of the wave, then a region the points'
+// final stages share in turn.
+//   point region: C (the Gram; the Jacobi reads it and writes V over it), b,
+//                 the eigenvalues, 8 scalars
+//   shared:       U, w-bar, the apply rows X, their means M, X' V diag(U) and
+//                 X' w-bar
+struct FusedLayout {
+    int kmax, nvmax, n, lda, offB, offL, offS, per, offWb, offX, offM, offZ, offD;
+};
+
+__host__ __device__ inline FusedLayout fused_layout(int kmax, int nvmax)
 {
-    const int n = (k + 1) & ~1, lda = n + 1;
-    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;  // apply rows reuse the V region
-    size_t d = 0;
-    d += (size_t)n * lda;                                    // A
-    d += (size_t)k * lda > vx ? (size_t)k * lda : vx;        // V, later X and M
-    d += (size_t)k + n + 2 * (size_t)k;                      // B L U Wb
-    d += 8;                                                  // scalars
-    return d;
+    FusedLayout f;
+    f.kmax = kmax;
+    f.nvmax = nvmax;
+    f.n = (kmax + 1) & ~1;
+    f.lda = f.n + 1;
+    f.offB = f.n * f.lda;
+    f.offL = f.offB + kmax;
+    f.offS = f.offL + f.n;
+    f.per = f.offS + 8;
+    f.offWb = kmax;
+    f.offX = f.offWb + kmax;
+    f.offM = f.offX + nvmax * kmax;
+    f.offZ = f.offM + nvmax;
+    f.offD = f.offZ + nvmax * kmax;
+    return f;
 }
 
-__device__ inline SolveLds carve(double* s, int k)
+__host__ __device__ inline size_t solve_lds_doubles(int kmax, int nvmax, int ppw)
 {
-    const int n = (k + 1) & ~1, lda = n + 1;
+    const FusedLayout f = fused_layout(kmax, nvmax);
+    return (size_t)ppw * f.per + (size_t)f.offD + (size_t)nvmax;
+}
+
+// Views of point region base: C and later V, b, eigenvalues and scalars in the
+// point's own region; the final stage's vectors in the shared region.
+__device__ inline SolveLds carve_point(double* base, double* shared, const FusedLayout& f)
+{
     SolveLds L;
-    const size_t vx = (size_t)LEKF_NVMAX * k + LEKF_NVMAX;
-    L.A = s;
-    s += (size_t)n * lda;
-    L.V = s;
-    L.X = s;  // used only after T has been formed
-    L.M = s + (size_t)LEKF_NVMAX * k;
-    s += (size_t)k * lda > vx ? (size_t)k * lda : vx;
-    L.B = s;
-    s += k;
-    L.L = s;
-    s += n;
-    L.U = s;
-    s += k;
-    L.Wb = s;
-    s += k;
-    L.sc = s;
+    L.A = base;
+    L.B = base + f.offB;
+    L.L = base + f.offL;
+    L.sc = base + f.offS;
+    L.U = shared;
+    L.Wb = shared + f.offWb;
+    L.X = shared + f.offX;
+    L.M = shared + f.offM;
+    L.Z = shared + f.offZ;
+    L.D = shared + f.offD;
     return L;
 }
 
@@ -377,108 +397,16 @@ __device__ __forceinline__ int rr_partner(int n, int r, int j)
     return x == 0 ? 0 : 1 + ((m - x) + r) % m;
 }
 
-// One-sided (Hestenes) cyclic Jacobi with the matrix in registers, for k <= 32.
-// Lanes 0..31 hold the columns of A (initially C, so A V -> U S with S = the
-// eigenvalues of the SPD C); lanes 32..63 hold the columns of V (initially I).
-// Each round pairs every column with its tournament partner; a lane fetches
-// its partner's column with a cross-lane shuffle, the A lanes form
-// [alpha gamma; gamma beta] = the 2x2 Gram of the pair and the rotation that
-// zeroes gamma (the same rotation two-sided Jacobi would apply to C'C), and
-// both halves apply it to their own column.  KR = rows carried (k rounded up
-// to even), rows and columns >= k are zero and never rotate.
-// Writes the eigenvalues to L.L and V to L.V (k x k, lda).  Returns the
-// number of sweeps or -1 if the sweep limit was reached.
-template <int KR>
-__device__ int jacobi_reg32(SolveLds& L, int k, int lane, int lda)
+// Gram stage of one point: C + (k-1)/alpha I into the point region, b, the
+// scalars of the inflation update, and the point's k and obs flag in sc.
+template <int NT, int MODE>
+__device__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
 {
-    const int half = lane >> 5, j = lane & 31;
-    double x[KR];
-#pragma unroll
-    for (int i = 0; i < KR; ++i) {
-        if (half == 0)
-            x[i] = (j < k && i < k) ? L.A[i * lda + j] : 0.0;
-        else
-            x[i] = (i == j && j < k) ? 1.0 : 0.0;
-    }
-    double alpha = 0.0;
-#pragma unroll
-    for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
-    int sweep = 0;
-    for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
-        bool rotated = false;
-        for (int r = 0; r < KR - 1; ++r) {
-            const int pj = rr_partner(KR, r, j);
-            const int src = (half << 5) | pj;
-            double y[KR];
-#pragma unroll
-            for (int i = 0; i < KR; ++i) y[i] = __shfl(x[i], src, 64);
-            double gamma = 0.0;
-#pragma unroll
-            for (int i = 0; i < KR; ++i) gamma += x[i] * y[i];
-            const double beta = __shfl(alpha, pj, 64);
-            const bool lo = j < pj;
-            const double app = lo ? alpha : beta, aqq = lo ? beta : alpha;
-            double c = 1.0, s = 0.0, t = 0.0;
-            const bool rot =
-                gamma * gamma > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * (app * aqq);
-            if (rot) {
-                const double th = (aqq - app) * rcp_nr(2.0 * gamma);
-                const double q = fma(th, th, 1.0);
-                const double at = fabs(th) + q * rsq_nr(q);
-                t = th >= 0.0 ? rcp_nr(at) : -rcp_nr(at);
-                c = rsq_nr(fma(t, t, 1.0));
-                s = t * c;
-            }
-            // the V lanes take the rotation of their column from the A lane
-            c = __shfl(c, j, 64);
-            s = __shfl(s, j, 64);
-            const double sg = lo ? -s : s;
-#pragma unroll
-            for (int i = 0; i < KR; ++i) x[i] = c * x[i] + sg * y[i];
-            if (rot) {
-                alpha = lo ? app - t * gamma : aqq + t * gamma;
-                rotated = true;
-            }
-        }
-        if (__ballot(rotated && half == 0) == 0ull) break;
-        // refresh the column norms once per sweep
-        alpha = 0.0;
-#pragma unroll
-        for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
-    }
-    double nrm = 0.0;
-#pragma unroll
-    for (int i = 0; i < KR; ++i) nrm += x[i] * x[i];
-    if (half == 0 && j < k) L.L[j] = sqrt(nrm);
-    if (half == 1 && j < k) {
-#pragma unroll
-        for (int i = 0; i < KR; ++i)
-            if (i < k) L.V[i * lda + j] = x[i];
-    }
-    return sweep < LEKF_JACOBI_MAX_SWEEPS ? sweep : -1;
-}
-
-// NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32);
-// MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3.
-template <int NT, int KR, int MODE>
-__global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restrict__ jobs, int njobs)
-{
-    extern __shared__ double smem[];
-    const long gp = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int ji = lekf_job_of(jobs, njobs, gp);
-    const LekfJob& J = jobs[ji];
-    const int b = (int)(gp - J.pt_begin + J.pt0);
     const int k = J.k;
     const int n = (k + 1) & ~1, lda = n + 1;
-    SolveLds L = carve(smem, k);
-
     const long off = J.loc_off[b];
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - off);
     double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
-    const bool transm_mode = (J.flags & LEKF_F_TRANSM) != 0;
-    int status = 0;
-
     if (p > 0) {
         double p3 = 0.0;
         if (MODE == 0) {
@@ -510,15 +438,125 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
             L.sc[1] = tr / (double)(k - 1);  // P2
             L.sc[2] = p3;                    // P3
             if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
-            L.sc[3] = alpha;
             const double rho = 1.0 / alpha;
             for (int i = 0; i < k; ++i) L.A[i * lda + i] += (double)(k - 1) * rho;
         }
-        __syncthreads();
-        alpha = L.sc[3];
-        const int sweeps = jacobi_reg32<KR>(L, k, lane, lda);
-        if (sweeps < 0) status |= LEKF_PT_NOCONV;
-        __syncthreads();
+    }
+    if (lane == 0) {
+        L.sc[3] = alpha;
+        L.sc[4] = (double)k;
+        L.sc[5] = p > 0 ? 1.0 : 0.0;
+        L.sc[6] = 0.0;
+    }
+}
+
+// One-sided (Hestenes) cyclic Jacobi of PPW points at once, for KR <= 32.
+// Lane group g (lanes g*KR .. g*KR+KR-1) holds point g; lane j of a group holds
+// column j of A (initially C, so A V -> U S with S = the eigenvalues of the
+// SPD C) and column j of V (initially I).  Each round pairs every column with
+// its tournament partner; the lane fetches the partner's columns with
+// cross-lane shuffles, forms [alpha gamma; gamma beta] = the 2x2 Gram of the
+// pair and the rotation that zeroes gamma (the same rotation two-sided Jacobi
+// would apply to C'C), and applies it to both of its columns.  Rows and
+// columns >= k are zero and never rotate; so does a point with no obs, and a
+// point that has converged while others of the wave still rotate, so a
+// point's result does not depend on the other points of its wave.  Writes the
+// eigenvalues and V over C in each point region, and LEKF_PT_NOCONV into
+// sc[6] of a point still rotating at the sweep limit.
+template <int KR, int PPW>
+__device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
+{
+    const int g = lane / KR, j = lane - g * KR;
+    const bool in = g < PPW;
+    double* base = smem + (size_t)(in ? g : 0) * f.per;
+    const int k = in ? (int)base[f.offS + 4] : 0;
+    const int kk = (in && base[f.offS + 5] != 0.0) ? k : 0;
+    const int lda = ((k + 1) & ~1) + 1;
+    double x[KR], v[KR];
+#pragma unroll
+    for (int i = 0; i < KR; ++i) {
+        x[i] = (j < kk && i < kk) ? base[i * lda + j] : 0.0;
+        v[i] = (i == j && j < kk) ? 1.0 : 0.0;
+    }
+    double alpha = 0.0;
+#pragma unroll
+    for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
+    int sweep = 0;
+    bool rotated = false;
+    for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
+        rotated = false;
+        for (int r = 0; r < KR - 1; ++r) {
+            const int pj = rr_partner(KR, r, j);
+            const int src = in ? g * KR + pj : lane;
+            double y[KR];
+#pragma unroll
+            for (int i = 0; i < KR; ++i) y[i] = __shfl(x[i], src, 64);
+            double gamma = 0.0;
+#pragma unroll
+            for (int i = 0; i < KR; ++i) gamma += x[i] * y[i];
+            const double beta = __shfl(alpha, src, 64);
+            const bool lo = j < pj;
+            const double app = lo ? alpha : beta, aqq = lo ? beta : alpha;
+            double c = 1.0, s = 0.0, t = 0.0;
+            const bool rot =
+                gamma * gamma > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * (app * aqq);
+            if (rot) {
+                const double th = (aqq - app) * rcp_nr(2.0 * gamma);
+                const double q = fma(th, th, 1.0);
+                const double at = fabs(th) + q * rsq_nr(q);
+                t = th >= 0.0 ? rcp_nr(at) : -rcp_nr(at);
+                c = rsq_nr(fma(t, t, 1.0));
+                s = t * c;
+            }
+            const double sg = lo ? -s : s;
+#pragma unroll
+            for (int i = 0; i < KR; ++i) x[i] = c * x[i] + sg * y[i];
+#pragma unroll
+            for (int i = 0; i < KR; ++i) {
+                const double vy = __shfl(v[i], src, 64);
+                v[i] = c * v[i] + sg * vy;
+            }
+            if (rot) {
+                alpha = lo ? app - t * gamma : aqq + t * gamma;
+                rotated = true;
+            }
+        }
+        if (__ballot(rotated) == 0ull) break;
+        // refresh the column norms once per sweep
+        alpha = 0.0;
+#pragma unroll
+        for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
+    }
+    const unsigned long long still = __ballot(in && rotated);
+    double nrm = 0.0;
+#pragma unroll
+    for (int i = 0; i < KR; ++i) nrm += x[i] * x[i];
+    __syncthreads();  // every lane has read its column of C before V goes over it
+    if (in && j < kk) {
+        base[f.offL + j] = sqrt(nrm);
+#pragma unroll
+        for (int i = 0; i < KR; ++i)
+            if (i < kk) base[i * lda + j] = v[i];
+    }
+    if (in && j == 0 && sweep == LEKF_JACOBI_MAX_SWEEPS &&
+        ((still >> (g * KR)) & ((KR < 64 ? (1ull << KR) : 0ull) - 1ull)) != 0ull)
+        base[f.offS + 6] = (double)LEKF_PT_NOCONV;
+}
+
+// Final stage of one point: eigenvalue test, w-bar, the inflation update,
+// the optional outputs and the fused apply.  With T = V diag(U) V^T + w-bar 1^T
+// (U = sqrt((k-1)/lambda)), the apply row is x-mean + (X' V) diag(U) V^T +
+// (X' . w-bar) 1^T, so T itself is formed only when the caller asks for it.
+__device__ void fused_finish(const LekfJob& J, int b, SolveLds& L, int lane)
+{
+    const int k = J.k;
+    const int n = (k + 1) & ~1, lda = n + 1;
+    const double* V = L.A;
+    const double alpha = L.sc[3];
+    const bool has_obs = L.sc[5] != 0.0;
+    const bool transm_mode = (J.flags & LEKF_F_TRANSM) != 0;
+    int status = (int)L.sc[6];
+    if (has_obs) {
         // eigenvalues; reference truncation test (common_mtx.f90:66-74)
         double lmax = 0.0, lmin = 1.0e308;
         for (int j = 0; j < k; ++j) {
@@ -530,33 +568,35 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
         // u = L^-1 V^T b ; w-bar = V u
         for (int j = lane; j < k; j += LEKF_WAVE) {
             double s = 0.0;
-            for (int i = 0; i < k; ++i) s += L.V[i * lda + j] * L.B[i];
+            for (int i = 0; i < k; ++i) s += V[i * lda + j] * L.B[i];
             L.U[j] = s / L.L[j];
         }
         __syncthreads();
         for (int i = lane; i < k; i += LEKF_WAVE) {
             double s = 0.0;
-            for (int j = 0; j < k; ++j) s += L.V[i * lda + j] * L.U[j];
+            for (int j = 0; j < k; ++j) s += V[i * lda + j] * L.U[j];
             L.Wb[i] = s;
         }
-        // U <- sqrt((k-1)/lambda) (U no longer needed)
+        // U <- sqrt((k-1)/lambda)
         __syncthreads();
         for (int j = lane; j < k; j += LEKF_WAVE) L.U[j] = sqrt((double)(k - 1) / L.L[j]);
         __syncthreads();
-        // T (into A) = V diag(U) V^T (+ w-bar)
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
-            const int i = w / k, m = w - i * k;
-            double s = 0.0;
-            for (int l = 0; l < k; ++l) s += L.V[i * lda + l] * L.U[l] * L.V[m * lda + l];
-            if (!transm_mode) s += L.Wb[i];
-            L.A[i * lda + m] = s;
+        if (J.trans) {
+            double* tr = J.trans + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += LEKF_WAVE) {
+                const int m = w / k, i = w - m * k;  // column-major: coalesced over i
+                double s = 0.0;
+                for (int l = 0; l < k; ++l) s += V[i * lda + l] * L.U[l] * V[m * lda + l];
+                if (!transm_mode) s += L.Wb[i];
+                tr[i + (size_t)m * k] = s;
+            }
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
             for (int w = lane; w < k * k; w += LEKF_WAVE) {
                 const int i = w / k, m = w - i * k;
                 double s = 0.0;
-                for (int l = 0; l < k; ++l) s += L.V[i * lda + l] / L.L[l] * L.V[m * lda + l];
+                for (int l = 0; l < k; ++l) s += V[i * lda + l] / L.L[l] * V[m * lda + l];
                 pa[i + (size_t)m * k] = s;
             }
         }
@@ -567,11 +607,14 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
                                 ? J.infl_fixed
                                 : adaptive_alpha(alpha, L.sc[0], L.sc[1], L.sc[2]);
     } else {
-        // no local obs (common_letkf.f90:82-96)
+        // no local obs (common_letkf.f90:82-96): T = sqrt(alpha) I
         const double sa = sqrt(alpha);
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
-            const int i = w / k, m = w - i * k;
-            L.A[i * lda + m] = (i == m) ? sa : 0.0;
+        if (J.trans) {
+            double* tr = J.trans + (size_t)b * k * k;
+            for (int w = lane; w < k * k; w += LEKF_WAVE) {
+                const int m = w / k, i = w - m * k;
+                tr[i + (size_t)m * k] = (i == m) ? sa : 0.0;
+            }
         }
         if (J.pao) {
             double* pa = J.pao + (size_t)b * k * k;
@@ -586,14 +629,6 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
             J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
     }
     if (lane == 0 && J.status) J.status[b] = status;
-    __syncthreads();
-    if (J.trans) {
-        double* tr = J.trans + (size_t)b * k * k;
-        for (int w = lane; w < k * k; w += LEKF_WAVE) {
-            const int m = w / k, i = w - m * k;  // column-major: coalesced over i
-            tr[i + (size_t)m * k] = L.A[i * lda + m];
-        }
-    }
     if (!(J.flags & LEKF_F_APPLY)) return;
     // fused apply: rows of the point (3-D at every level, 2-D at level 0)
     const int lev = b / J.nij, ij = b - lev * J.nij;
@@ -616,18 +651,83 @@ __global__ __launch_bounds__(64) void lekf_solve_kernel(const LekfJob* __restric
     __syncthreads();
     for (int w = lane; w < nrow * k; w += LEKF_WAVE) L.X[w] -= L.M[w / k];
     __syncthreads();
+    if (has_obs) {
+        // Z = X' V diag(U) and D = X' w-bar, row by row
+        for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
+            const int r = w / k, l = w - r * k;
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += L.X[r * k + j] * V[j * lda + l];
+            L.Z[r * k + l] = s * L.U[l];
+        }
+        for (int r = lane; r < nrow; r += LEKF_WAVE) {
+            double d = 0.0;
+            if (!transm_mode)
+                for (int j = 0; j < k; ++j) d += L.X[r * k + j] * L.Wb[j];
+            L.D[r] = d;
+        }
+        __syncthreads();
+    }
+    const double sa = has_obs ? 0.0 : sqrt(alpha);
     for (int w = lane; w < nrow * k; w += LEKF_WAVE) {
         const int r = w / k, m = w - r * k;
         const bool upd = (r >= n3) || ((mask >> r) & 1u);
         double a = L.M[r];
-        if (upd) {
-            for (int j = 0; j < k; ++j) a += L.X[r * k + j] * L.A[j * lda + m];
-        } else {
+        if (!upd) {
             a += L.X[r * k + m];
+        } else if (has_obs) {
+            double s = 0.0;
+            for (int l = 0; l < k; ++l) s += L.Z[r * k + l] * V[m * lda + l];
+            a += s + L.D[r];
+        } else {
+            a += sa * L.X[r * k + m];
         }
         double* dst = (r < n3) ? J.x3a + ((size_t)b * n3 + r) * k
                                : J.x2a + ((size_t)ij * J.nv2d + (r - n3)) * k;
         dst[m] = a;
+    }
+}
+
+// NT: 16-wide Gram tiles; KR: rows of the register Jacobi (k <= KR <= 32);
+// MODE 0: FP64 Gram, 1: FP32, 2: split-BF16x3; PPW points per wave (lane
+// groups of KR for the Jacobi).  Points gp = blockIdx.x * PPW + g of the flat
+// point range [0, npt); every job of the launch has k <= kmax.
+template <int NT, int KR, int MODE, int PPW>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(KR <= 20 ? 3 : 2)))
+void lekf_solve_kernel(const LekfJob* __restrict__ jobs, int njobs, long npt, int kmax, int nvmax)
+{
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x;
+    const FusedLayout f = fused_layout(kmax, nvmax);
+    double* const shared = smem + (size_t)PPW * f.per;
+    const long gp0 = (long)blockIdx.x * PPW;
+
+    for (int g = 0; g < PPW; ++g) {
+        double* base = smem + (size_t)g * f.per;
+        const long gp = gp0 + g;
+        if (gp >= npt) {
+            if (lane == 0) {
+                base[f.offS + 4] = 0.0;
+                base[f.offS + 5] = 0.0;
+            }
+            continue;
+        }
+        const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+        const int b = (int)(gp - J.pt_begin + J.pt0);
+        SolveLds L = carve_point(base, shared, f);
+        fused_gram<NT, MODE>(J, b, L, lane);
+    }
+    __syncthreads();
+    jacobi_multi<KR, PPW>(smem, f, lane);
+    __syncthreads();
+    for (int g = 0; g < PPW; ++g) {
+        const long gp = gp0 + g;
+        if (gp >= npt) break;
+        const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+        const int b = (int)(gp - J.pt_begin + J.pt0);
+        SolveLds L = carve_point(smem + (size_t)g * f.per, shared, f);
+        fused_finish(J, b, L, lane);
+        __syncthreads();
     }
 }
 
@@ -2653,9 +2753,13 @@ int lekf_solve_class(int k)
     return lekf_solve_nt(k);
 }
 
+// points per wave of the fused kernel for Jacobi rows KR: lane groups of KR,
+// at most 4 points (the LDS holds a region per point)
+static constexpr int fused_ppw(int kr) { return 64 / kr < 4 ? 64 / kr : 4; }
+
 size_t lekf_solve_lds_bytes(int k, int nvmax)
 {
-    if (k <= 32) return solve_lds_doubles(k) * sizeof(double);
+    if (k <= 32) return solve_lds_doubles(k, nvmax, fused_ppw((k + 3) & ~3)) * sizeof(double);
     if (k > LEKF_KSPLIT) return eig_lds_doubles_big(k, nvmax) * sizeof(double);
     return eig_lds_doubles(k, nvmax) * sizeof(double);
 }
@@ -2674,11 +2778,15 @@ size_t lekf_solve_ws_bytes(long npt, int kmax)
 }
 
 template <int NT, int KR, int MODE>
-static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int kmax, hipStream_t st)
+static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
+                               hipStream_t st)
 {
-    const size_t lds = solve_lds_doubles(kmax) * sizeof(double);
-    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE>), dim3((unsigned)npt), dim3(LEKF_WAVE), lds,
-                       st, djobs, njobs);
+    constexpr int PPW = fused_ppw(KR);
+    if (kmax > KR) return hipErrorInvalidValue;
+    const size_t lds = solve_lds_doubles(kmax, nvmax, PPW) * sizeof(double);
+    const long nwg = (npt + PPW - 1) / PPW;
+    hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE, PPW>), dim3((unsigned)nwg), dim3(LEKF_WAVE),
+                       lds, st, djobs, njobs, npt, kmax, nvmax);
     return hipGetLastError();
 }
 
@@ -2768,14 +2876,14 @@ static hipError_t launch_mode(int cls, const LekfJob* djobs, int njobs, long npt
                               int nvmax, void* ws, size_t ws_bytes, hipStream_t st)
 {
     switch (cls) {
-    case 4 * 8 + 1: return launch_fused<1, 4, MODE>(djobs, njobs, npt, kmax, st);
-    case 8 * 8 + 1: return launch_fused<1, 8, MODE>(djobs, njobs, npt, kmax, st);
-    case 12 * 8 + 1: return launch_fused<1, 12, MODE>(djobs, njobs, npt, kmax, st);
-    case 16 * 8 + 2: return launch_fused<2, 16, MODE>(djobs, njobs, npt, kmax, st);
-    case 20 * 8 + 2: return launch_fused<2, 20, MODE>(djobs, njobs, npt, kmax, st);
-    case 24 * 8 + 2: return launch_fused<2, 24, MODE>(djobs, njobs, npt, kmax, st);
-    case 28 * 8 + 2: return launch_fused<2, 28, MODE>(djobs, njobs, npt, kmax, st);
-    case 32 * 8 + 3: return launch_fused<3, 32, MODE>(djobs, njobs, npt, kmax, st);
+    case 4 * 8 + 1: return launch_fused<1, 4, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 8 * 8 + 1: return launch_fused<1, 8, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 12 * 8 + 1: return launch_fused<1, 12, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 16 * 8 + 2: return launch_fused<2, 16, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 20 * 8 + 2: return launch_fused<2, 20, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 24 * 8 + 2: return launch_fused<2, 24, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 28 * 8 + 2: return launch_fused<2, 28, MODE>(djobs, njobs, npt, kmax, nvmax, st);
+    case 32 * 8 + 3: return launch_fused<3, 32, MODE>(djobs, njobs, npt, kmax, nvmax, st);
     case 3: return launch_split<3, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     case 4: return launch_split<4, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
     case 5: return launch_split<5, MODE>(djobs, njobs, npt, kmax, nvmax, ws, ws_bytes, st);
