@@ -73,50 +73,81 @@ __device__ __forceinline__ double rsq_nr(double x)
 
 // ---------------------------------------------------------------------------
 // Gram products.  Lane l holds A[i = l%16][kk = l/16] and B[kk][j = l%16] of
-// each 16x16x4 step (16x16x32 BF16: kk = 8*(l/16) .. +7).  Wave wv of nw
-// takes every nw-th group of obs.
+// each 16x16x4 step (16x16x32 BF16: kk = 8*(l/16) .. +7).
+//
+// The local list is read in chunks of 64 entries: lane l loads entry q0 + l
+// (obs index, weight rho/rdiag, departure) with one coalesced access, and the
+// MFMA steps of the chunk take them from the owning lanes by shuffles.  All
+// gathers of the chunk's Y rows are then independent loads, issued together,
+// instead of one dependent index -> row chain per step.  Wave wv of nw takes
+// every nw-th chunk.
 // ---------------------------------------------------------------------------
+struct ObsChunk {
+    int o;       // obs index (0 when past the end: a valid row, weighted 0)
+    double w;    // rho / rdiag (1 / rdiag with rdiag_wloc), 0 past the end
+    double d;    // departure
+};
+
+__device__ __forceinline__ ObsChunk stage_chunk(const LekfJob& J, long off, int p, int q0,
+                                                int lane, bool wloc, double& p3)
+{
+    ObsChunk c{0, 0.0, 0.0};
+    const int q = q0 + lane;
+    if (q < p) {
+        c.o = J.loc_idx[off + q];
+        const double rho = J.loc_rho[off + q];
+        const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[c.o];
+        c.w = (wloc ? 1.0 : rho) / rd;
+        c.d = J.dep[c.o];
+        p3 += rho;
+    }
+    return c;
+}
+
 template <int NT>
 __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int k, int lane,
                                          lekf_d4* acc, double& p3, int wv, int nw)
 {
+    // 4-obs steps whose rows are gathered at once (bounded by the registers
+    // the rows and the NT (NT + 1) / 2 accumulator tiles take)
+    constexpr int SUB = NT == 1 ? 16 : NT == 2 ? 8 : NT == 3 ? 4 : NT == 4 ? 2 : 1;
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
-#pragma unroll 2
-    for (int q0 = 4 * wv; q0 < p; q0 += 4 * nw) {
-        const int q = q0 + kq;
-        const bool valid = q < p;
-        double w = 0.0, dd = 0.0;
-        long o = 0;
-        if (valid) {
-            o = J.loc_idx[off + q];
-            const double rho = J.loc_rho[off + q];
-            const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
-            w = (wloc ? 1.0 : rho) / rd;
-            dd = J.dep[o];
-            if (lr == 0) p3 += rho;
-        }
-        double v[NT];
+    for (int q0 = 64 * wv; q0 < p; q0 += 64 * nw) {
+        const ObsChunk ch = stage_chunk(J, off, p, q0, lane, wloc, p3);
+#pragma unroll 1
+        for (int h = 0; h < 16 && q0 + 4 * h < p; h += SUB) {
+            double v[SUB][NT], wt[SUB];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int c = 16 * t + lr;
-            double x = 0.0;
-            if (valid) {
-                if (c < k)
-                    x = J.hdxf[o * J.y_so + (long)c * J.y_sm];
-                else if (c == k)
-                    x = dd;
+            for (int st = 0; st < SUB; ++st) {
+                const int src = 4 * (h + st) + kq;
+                const int o = __shfl(ch.o, src, 64);
+                const double w = __shfl(ch.w, src, 64);
+                const double dd = __shfl(ch.d, src, 64);
+                wt[st] = w;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int c = 16 * t + lr;
+                    double x = 0.0;
+                    if (c < k)
+                        x = J.hdxf[(long)o * J.y_so + (long)c * J.y_sm];
+                    else if (c == k)
+                        x = dd;
+                    v[st][t] = x;
+                }
             }
-            v[t] = x;
-        }
-        int a = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
-            const double ai = w * v[I];
+            for (int st = 0; st < SUB; ++st) {
+                int a = 0;
 #pragma unroll
-            for (int Jt = I; Jt < NT; ++Jt) {
-                acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, v[Jt], acc[a], 0, 0, 0);
-                ++a;
+                for (int I = 0; I < NT; ++I) {
+                    const double ai = wt[st] * v[st][I];
+#pragma unroll
+                    for (int Jt = I; Jt < NT; ++Jt) {
+                        acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, v[st][Jt], acc[a], 0, 0, 0);
+                        ++a;
+                    }
+                }
             }
         }
     }
@@ -126,14 +157,12 @@ __device__ __forceinline__ void gram_f64(const LekfJob& J, long off, int p, int 
 // FP32: v_mfma_f32_16x16x4f32 (operands rounded to FP32).
 // BF16x3: each FP32 operand split x = h + l (h = bf16(x), l = bf16(x - h));
 // A*B ~ Ah*Bh + Ah*Bl + Al*Bh on v_mfma_f32_16x16x32_bf16.
-// FP32 form: the FP32 accumulators are added into FP64 ones (accd) every
-// kMixFlush obs per wave, so no FP32 sum runs over more than kMixFlush
-// products and the rounding of long local lists (p ~ 10^3) does not build
-// up.  The BF16x3 form keeps FP32 accumulation: its two-term split carries
-// about 16 significant bits, which dominates its error, and the FP64
-// accumulators would cost it occupancy.
-constexpr int kMixFlush = 64;
-
+// FP32 form: the FP32 accumulators are added into FP64 ones (accd) after each
+// chunk of 64 obs, so no FP32 sum runs over more than 64 products and the
+// rounding of long local lists (p ~ 10^3) does not build up.  The BF16x3 form
+// keeps FP32 accumulation: its two-term split carries about 16 significant
+// bits, which dominates its error, and the FP64 accumulators would cost it
+// occupancy.
 template <int NT>
 __device__ __forceinline__ void mix_flush(lekf_f4* acc, lekf_d4* accd)
 {
@@ -149,117 +178,102 @@ template <int NT>
 __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int k, int lane,
                                          lekf_f4* acc, lekf_d4* accd, double& p3, int wv, int nw)
 {
-    int steps = 0;
+    constexpr int SUB = NT == 1 ? 16 : NT == 2 ? 8 : NT == 3 ? 4 : NT == 4 ? 2 : 1;
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
-#pragma unroll 2
-    for (int q0 = 4 * wv; q0 < p; q0 += 4 * nw) {
-        const int q = q0 + kq;
-        const bool valid = q < p;
-        float w = 0.0f, dd = 0.0f;
-        long o = 0;
-        if (valid) {
-            o = J.loc_idx[off + q];
-            const double rho = J.loc_rho[off + q];
-            const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
-            w = (float)((wloc ? 1.0 : rho) / rd);
-            dd = (float)J.dep[o];
-            if (lr == 0) p3 += rho;
-        }
-        float v[NT];
+    for (int q0 = 64 * wv; q0 < p; q0 += 64 * nw) {
+        const ObsChunk ch = stage_chunk(J, off, p, q0, lane, wloc, p3);
+#pragma unroll 1
+        for (int h = 0; h < 16 && q0 + 4 * h < p; h += SUB) {
+            float v[SUB][NT], wt[SUB];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int c = 16 * t + lr;
-            float x = 0.0f;
-            if (valid) {
-                if (c < k)
-                    x = J.hdxf32[o * (long)k + c];
-                else if (c == k)
-                    x = dd;
+            for (int st = 0; st < SUB; ++st) {
+                const int src = 4 * (h + st) + kq;
+                const int o = __shfl(ch.o, src, 64);
+                const float w = (float)__shfl(ch.w, src, 64);
+                const float dd = (float)__shfl(ch.d, src, 64);
+                wt[st] = w;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int c = 16 * t + lr;
+                    float x = 0.0f;
+                    if (c < k)
+                        x = J.hdxf32[(long)o * k + c];
+                    else if (c == k)
+                        x = dd;
+                    v[st][t] = x;
+                }
             }
-            v[t] = x;
-        }
-        int a = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
-            const float ai = w * v[I];
+            for (int st = 0; st < SUB; ++st) {
+                int a = 0;
 #pragma unroll
-            for (int Jt = I; Jt < NT; ++Jt) {
-                acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, v[Jt], acc[a], 0, 0, 0);
-                ++a;
+                for (int I = 0; I < NT; ++I) {
+                    const float ai = wt[st] * v[st][I];
+#pragma unroll
+                    for (int Jt = I; Jt < NT; ++Jt) {
+                        acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, v[st][Jt], acc[a], 0, 0, 0);
+                        ++a;
+                    }
+                }
             }
         }
-        if (++steps == kMixFlush / 4) {
-            mix_flush<NT>(acc, accd);
-            steps = 0;
-        }
+        mix_flush<NT>(acc, accd);
     }
-    mix_flush<NT>(acc, accd);
 }
 
-// Split-BF16x3: K = 32 obs per MFMA step.  Lane l supplies 8 consecutive k
-// values (kk = 8*(l/16) .. +7) of row/col l%16.
+// Split-BF16x3: K = 32 obs per MFMA step, two steps per chunk.  Lane l
+// supplies 8 consecutive k values (kk = 8*(l/16) .. +7) of row/col l%16.
 template <int NT>
 __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, int k, int lane,
                                             lekf_f4* acc, double& p3, int wv, int nw)
 {
     const int lr = lane & 15, kq = lane >> 4;
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
-    for (int q0 = 32 * wv; q0 < p; q0 += 32 * nw) {
-        lekf_bf8 ah[NT], al[NT], bh[NT], bl[NT];
-        float wv8[8];
-        long ov[8];
-        bool vv[8];
-        float dv[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int q = q0 + 8 * kq + e;
-            vv[e] = q < p;
-            wv8[e] = 0.0f;
-            dv[e] = 0.0f;
-            ov[e] = 0;
-            if (vv[e]) {
-                const long o = J.loc_idx[off + q];
-                const double rho = J.loc_rho[off + q];
-                const double rd = J.loc_rdiag ? J.loc_rdiag[off + q] : J.rdiag[o];
-                wv8[e] = (float)((wloc ? 1.0 : rho) / rd);
-                dv[e] = (float)J.dep[o];
-                ov[e] = o;
-                if (lr == 0) p3 += rho;
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int c = 16 * t + lr;
+    for (int q0 = 64 * wv; q0 < p; q0 += 64 * nw) {
+        const ObsChunk ch = stage_chunk(J, off, p, q0, lane, wloc, p3);
+        for (int h = 0; h < 64 && q0 + h < p; h += 32) {
+            lekf_bf8 ah[NT], al[NT], bh[NT], bl[NT];
+            float wv8[8], dv8[8];
+            int ov[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                float x = 0.0f;
-                if (vv[e]) {
-                    if (c < k)
-                        x = J.hdxf32[ov[e] * (long)k + c];
-                    else if (c == k)
-                        x = dv[e];
-                }
-                const float xa = wv8[e] * x;
-                const __bf16 bh1 = (__bf16)x;
-                const __bf16 bl1 = (__bf16)(x - (float)bh1);
-                const __bf16 ah1 = (__bf16)xa;
-                const __bf16 al1 = (__bf16)(xa - (float)ah1);
-                bh[t][e] = bh1;
-                bl[t][e] = bl1;
-                ah[t][e] = ah1;
-                al[t][e] = al1;
+                const int src = h + 8 * kq + e;
+                ov[e] = __shfl(ch.o, src, 64);
+                wv8[e] = (float)__shfl(ch.w, src, 64);
+                dv8[e] = (float)__shfl(ch.d, src, 64);
             }
-        }
-        int a = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
+            for (int t = 0; t < NT; ++t) {
+                const int c = 16 * t + lr;
 #pragma unroll
-            for (int Jt = I; Jt < NT; ++Jt) {
-                acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[I], bh[Jt], acc[a], 0, 0, 0);
-                acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bl[Jt], acc[a], 0, 0, 0);
-                acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bh[Jt], acc[a], 0, 0, 0);
-                ++a;
+                for (int e = 0; e < 8; ++e) {
+                    float x = 0.0f;
+                    if (c < k)
+                        x = J.hdxf32[(long)ov[e] * k + c];
+                    else if (c == k)
+                        x = dv8[e];
+                    const float xa = wv8[e] * x;
+                    const __bf16 bh1 = (__bf16)x;
+                    const __bf16 bl1 = (__bf16)(x - (float)bh1);
+                    const __bf16 ah1 = (__bf16)xa;
+                    const __bf16 al1 = (__bf16)(xa - (float)ah1);
+                    bh[t][e] = bh1;
+                    bl[t][e] = bl1;
+                    ah[t][e] = ah1;
+                    al[t][e] = al1;
+                }
+            }
+            int a = 0;
+#pragma unroll
+            for (int I = 0; I < NT; ++I) {
+#pragma unroll
+                for (int Jt = I; Jt < NT; ++Jt) {
+                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[I], bh[Jt], acc[a], 0, 0, 0);
+                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bl[Jt], acc[a], 0, 0, 0);
+                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[I], bh[Jt], acc[a], 0, 0, 0);
+                    ++a;
+                }
             }
         }
     }
@@ -481,10 +495,13 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
     double alpha = 0.0;
 #pragma unroll
     for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
+    const unsigned long long gmask = (KR < 64 ? (1ull << KR) : 0ull) - 1ull;
     int sweep = 0;
     bool rotated = false;
+    bool done = false;  // this lane's point has converged
     for (; sweep < LEKF_JACOBI_MAX_SWEEPS; ++sweep) {
         rotated = false;
+        bool big = false;  // a pair further than 1e-8 from orthogonal was rotated
         for (int r = 0; r < KR - 1; ++r) {
             const int pj = rr_partner(KR, r, j);
             const int src = in ? g * KR + pj : lane;
@@ -498,8 +515,10 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
             const bool lo = j < pj;
             const double app = lo ? alpha : beta, aqq = lo ? beta : alpha;
             double c = 1.0, s = 0.0, t = 0.0;
-            const bool rot =
-                gamma * gamma > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * (app * aqq);
+            const double g2 = done ? 0.0 : gamma * gamma;
+            const double aa = app * aqq;
+            const bool rot = g2 > (LEKF_JACOBI_TOL * LEKF_JACOBI_TOL) * aa;
+            big |= g2 > 1.0e-16 * aa;
             if (rot) {
                 const double th = (aqq - app) * rcp_nr(2.0 * gamma);
                 const double q = fma(th, th, 1.0);
@@ -521,6 +540,14 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
                 rotated = true;
             }
         }
+        // Quadratic convergence: a sweep that started with every pair of a
+        // point within 1e-8 of orthogonal leaves them within about 1e-16, so
+        // the point stops instead of running a sweep that would only confirm
+        // it.  The test is per point, so a point's sweeps do not depend on
+        // the other points of its wave.
+        const unsigned long long bigm = __ballot(big);
+        if (in && ((bigm >> (g * KR)) & gmask) == 0ull) done = true;
+        rotated = rotated && !done;
         if (__ballot(rotated) == 0ull) break;
         // refresh the column norms once per sweep
         alpha = 0.0;
@@ -538,8 +565,7 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
         for (int i = 0; i < KR; ++i)
             if (i < kk) base[i * lda + j] = v[i];
     }
-    if (in && j == 0 && sweep == LEKF_JACOBI_MAX_SWEEPS &&
-        ((still >> (g * KR)) & ((KR < 64 ? (1ull << KR) : 0ull) - 1ull)) != 0ull)
+    if (in && j == 0 && sweep == LEKF_JACOBI_MAX_SWEEPS && ((still >> (g * KR)) & gmask) != 0ull)
         base[f.offS + 6] = (double)LEKF_PT_NOCONV;
 }
 
@@ -741,8 +767,8 @@ __host__ __device__ inline long gram_stride(int kmax)
     return ((long)kmax * kmax + kmax + 2 + 1) & ~1L;
 }
 
-// W waves per point; wave w takes every W-th group of obs, and the partial
-// Grams are summed in LDS in wave order (deterministic for a given W).
+// W waves per point; wave w takes every W-th chunk of 64 local obs, and the
+// partial Grams are summed in LDS in wave order (deterministic for a given W).
 template <int NT, int MODE, int W>
 __global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
     const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride)
