@@ -212,6 +212,36 @@ int lekf_set_obs_global_hx(lekf_handle *h, int nobs, const double *elm, const do
 int lekf_obsope_global(lekf_handle *h, int nobs, const double *elm, const double *lon,
                        const double *lat, const double *lev, const double *dat, double *hxf,
                        int *qc, double *ri, double *rj, double *rk);
+/* SCALE's radar obs operator on a regional handle's resident ensemble.
+ * The 3-D variables var[0..10] of the handle hold u, v, w, T, p, q, qc, qr, qi,
+ * qs, qg (SCALE's iv3dd order, scale/common/common_scale.f90:103-113).
+ * method: METHOD_REF_CALC 1, 2 or 3; use_terminal_velocity, min_ref_dbz,
+ * low_ref_shift and zmax: USE_TERMINAL_VELOCITY, MIN_RADAR_REF_DBZ,
+ * LOW_REF_SHIFT and RADAR_ZMAX (scale/common/common_nml.f90:173-184; defaults
+ * 3, 0, 0.0, 0.0, 99e3). */
+typedef struct lekf_radar_cfg {
+    int var[11];
+    int method;
+    int use_terminal_velocity;
+    double min_ref_dbz, low_ref_shift, zmax;
+} lekf_radar_cfg;
+/* heights (m) of the model levels, hgt(nij, nlev) column-major, ascending with
+ * the level and the same for every member (SCALE's z_full, iv3dd_hgt) --
+ * regional handles of at least two levels only */
+int lekf_set_heights(lekf_handle *h, const double *hgt);
+/* H(x) of radar reflectivity (elm 4001, dBZ) and Doppler velocity (4002, m/s)
+ * for every member: the checks of SCALE's obsope (RADAR_ZMAX, phys2ijkz,
+ * scale/obs/obsope_tools.f90:353-380) and Trans_XtoY_radar / calc_ref_vr
+ * (scale/common/common_obs_scale.f90:338-476,609-960).  Per obs: element,
+ * lon, lat (deg), height z (m), 1-based grid coordinates ri, rj (the map
+ * projection's, 1 < ri <= nx, 1 < rj <= ny), the radar's lon, lat and height.
+ * Out: hxf(nobs,ne) and qc(nobs,ne) column-major (qc 0 good; 19 above zmax,
+ * 20 above the top, 21 below the lowest valid level, 22 outside the grid or at
+ * the radar, 90 not a radar element), rk(nobs) (may be NULL). */
+int lekf_obsope_radar(lekf_handle *h, const lekf_radar_cfg *cfg, int nobs, const double *elm,
+                      const double *lon, const double *lat, const double *z, const double *ri,
+                      const double *rj, const double *radar_lon, const double *radar_lat,
+                      const double *radar_z, double *hxf, int *qc, double *rk);
 /* Regional obs after the obs operator (SCALE's obsda): grid-unit position
  * (ri, rj), height z (m), error, departure dep = y - mean H(x), bucket
  * (the (element, obs type) index, 0..3), qc (NULL: all good) and the obs-space

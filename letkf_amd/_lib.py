@@ -95,6 +95,14 @@ class RegionalCfg(Structure):
     ]
 
 
+class RadarCfg(Structure):
+    _fields_ = [("var", c_int * 11), ("method", c_int), ("use_terminal_velocity", c_int),
+                ("min_ref_dbz", c_double), ("low_ref_shift", c_double), ("zmax", c_double)]
+
+
+# scale/common/common_obs_scale.f90:71-72
+ID_RADAR_REF, ID_RADAR_VR = 4001, 4002
+
 _lib = None
 
 
@@ -210,6 +218,10 @@ def lib():
     L.lekf_l96_forecast.argtypes = [c_void_p, c_int, c_double, c_double]
     L.lekf_set_relax.restype = c_int
     L.lekf_set_relax.argtypes = [c_void_p, c_int, c_double, c_double, c_int, c_double]
+    L.lekf_set_heights.restype = c_int
+    L.lekf_set_heights.argtypes = [c_void_p, _dp]
+    L.lekf_obsope_radar.restype = c_int
+    L.lekf_obsope_radar.argtypes = [c_void_p, POINTER(RadarCfg), c_int] + [_dp] * 10 + [_ip, _dp]
     L.lekf_tune_set.restype = c_int
     L.lekf_tune_set.argtypes = [c_int, c_long]
     L.lekf_tune_get.restype = c_long
@@ -231,7 +243,8 @@ EXPORTED = [
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
-    "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast",
+    "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
+    "lekf_obsope_radar",
 ]
 
 

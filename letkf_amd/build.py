@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "liblekf_hip.so")
 FLIB = os.path.join(FSRC, "liblekf_fortran.so")
 
 HIP_SOURCES = ["lekf_solve.hip", "lekf_cull.hip", "lekf_obsope.hip", "lekf_regional.hip",
-               "lekf_relax.hip", "lekf_l96.hip", "lekf_capi.hip"]
+               "lekf_relax.hip", "lekf_l96.hip", "lekf_radar.hip", "lekf_capi.hip"]
 F_SOURCES = ["common_letkf.f90", "letkf_tools_gpu.f90"]
 
 HIPCC_FLAGS = [

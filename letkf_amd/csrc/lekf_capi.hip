@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "lekf_cull.hpp"
+#include "lekf_radar.hpp"
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
 #include "lekf_obsope.hpp"
@@ -537,6 +538,7 @@ struct lekf_handle {
     int nbkt = 1;
     double dlon_zero = 0;  // regional rectangle half-width in grid units (dlat_zero above)
     DevBuf d_zlev, cellcum, cap_n, cap_off, o_ri, o_rj, o_z, o_bkt, r_dep, r_bkt, r_z;
+    DevBuf d_hgt;                    // level heights (lekf_set_heights), radar operator
     DevBuf gws;                      // Gram workspace of the k > 32 solve
     DevBuf sel_key, sel_ob, sel_rho;  // per-slot scratch of the capped selection
     DevBuf maxcand;
@@ -569,7 +571,7 @@ long handle_bytes(const lekf_handle* h)
                           &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
                           &h->r_dep, &h->r_bkt, &h->r_z, &h->gws, &h->sel_key,
                           &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off, &h->capl_cnt,
-                          &h->capl_idx, &h->capl_rho};
+                          &h->capl_idx, &h->capl_rho, &h->d_hgt};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -588,7 +590,7 @@ void free_all(lekf_handle* h)
                     &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
                     &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
                     &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off,
-                    &h->capl_cnt, &h->capl_idx, &h->capl_rho};
+                    &h->capl_cnt, &h->capl_idx, &h->capl_rho, &h->d_hgt};
     for (auto* b : bs) b->release();
 }
 
@@ -1113,6 +1115,95 @@ int lekf_obsope_global(lekf_handle* h, int nobs, const double* elm, const double
     dri.release();
     drj.release();
     drk.release();
+    return LEKF_OK;
+}
+
+int lekf_set_heights(lekf_handle* h, const double* hgt)
+{
+    if (!h || h->kind != 2 || !hgt) return LEKF_ERR_ARG;
+    const int rc = upload(h->d_hgt, hgt, sizeof(double) * (size_t)h->nij * h->nlev, h->st);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return LEKF_OK;
+}
+
+int lekf_obsope_radar(lekf_handle* h, const lekf_radar_cfg* cfg, int nobs, const double* elm,
+                      const double* lon, const double* lat, const double* z, const double* ri,
+                      const double* rj, const double* radar_lon, const double* radar_lat,
+                      const double* radar_z, double* hxf, int* qc, double* rk)
+{
+    if (!h || !cfg || h->kind != 2 || nobs < 0 || !h->have_ens || !h->d_hgt.p || !hxf || !qc ||
+        h->nlev < 2)
+        return LEKF_ERR_ARG;
+    if (cfg->method < 1 || cfg->method > 3) return LEKF_ERR_ARG;
+    for (int q = 0; q < 11; ++q)
+        if (cfg->var[q] < 0 || cfg->var[q] >= h->nv3d) return LEKF_ERR_ARG;
+    if (nobs == 0) return LEKF_OK;
+    const double* in[9] = {elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z};
+    for (auto* p : in)
+        if (!p) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const size_t n = (size_t)nobs, ne = (size_t)h->ne;
+    DevBuf rec, dhx, dqc, drk;
+    struct Release {  // the call's buffers go on every return path
+        DevBuf* b[4];
+        ~Release()
+        {
+            for (DevBuf* x : b) x->release();
+        }
+    } rel{{&rec, &dhx, &dqc, &drk}};
+    HIPCHK(rec.ensure(sizeof(double) * n * 9));
+    for (int f = 0; f < 9; ++f)
+        HIPCHK(hipMemcpyAsync(rec.as<double>() + n * f, in[f], sizeof(double) * n,
+                              hipMemcpyHostToDevice, st));
+    HIPCHK(dhx.ensure(sizeof(double) * n * ne));
+    HIPCHK(dqc.ensure(sizeof(int) * n * ne));
+    HIPCHK(drk.ensure(sizeof(double) * n));
+    RadarArgs A{};
+    A.nobs = nobs;
+    A.nx = h->r.nx;
+    A.ny = h->r.ny;
+    A.nlev = h->nlev;
+    A.ne = h->ne;
+    A.nv3d = h->nv3d;
+    for (int q = 0; q < 11; ++q) A.var[q] = cfg->var[q];
+    A.method = cfg->method;
+    A.use_wt = cfg->use_terminal_velocity ? 1 : 0;
+    A.min_ref_dbz = cfg->min_ref_dbz;
+    A.low_ref_shift = cfg->low_ref_shift;
+    A.zmax = cfg->zmax;
+    // com_gamma of 4 + b, 4 + d and 4.5 in calc_ref_vr
+    A.gam[0] = std::tgamma(4.8);
+    A.gam[1] = std::tgamma(4.25);
+    A.gam[2] = std::tgamma(4.5);
+    const double* r = rec.as<double>();
+    A.elm = r;
+    A.lon = r + n;
+    A.lat = r + 2 * n;
+    A.z = r + 3 * n;
+    A.ri = r + 4 * n;
+    A.rj = r + 5 * n;
+    A.rlon = r + 6 * n;
+    A.rlat = r + 7 * n;
+    A.rz = r + 8 * n;
+    A.x3 = h->x3b.as<double>();
+    A.hgt = h->d_hgt.as<double>();
+    A.hxf = dhx.as<double>();
+    A.qc = dqc.as<int>();
+    A.rk = drk.as<double>();
+    HIPCHK(lekf_radar_launch(A, st));
+    std::vector<double> t(n * ne);
+    std::vector<int> tq(n * ne);
+    HIPCHK(hipMemcpyAsync(t.data(), dhx.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tq.data(), dqc.p, sizeof(int) * tq.size(), hipMemcpyDeviceToHost, st));
+    if (rk) HIPCHK(hipMemcpyAsync(rk, drk.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // device rows (obs, members) -> column-major (nobs, ne)
+    for (size_t i = 0; i < n; ++i)
+        for (size_t m = 0; m < ne; ++m) {
+            hxf[i + m * n] = t[i * ne + m];
+            qc[i + m * n] = tq[i * ne + m];
+        }
     return LEKF_OK;
 }
 

@@ -256,6 +256,34 @@ class RegionalLetkf(_Handle):
                                                iptr(q) if q is not None else None,
                                                hx.ctypes.data_as(_lib._dp)), "set_obs_regional")
 
+    def set_heights(self, hgt):
+        """Heights (m) of the model levels, hgt (nij, nlev), for the radar operator."""
+        self._hgt = _f(hgt)
+        check(_lib.lib().lekf_set_heights(self.h, self._hgt.ctypes.data_as(_lib._dp)), "set_heights")
+
+    def radar_operator(self, elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z,
+                       var=tuple(range(11)), method=3, use_terminal_velocity=False,
+                       min_ref_dbz=0.0, low_ref_shift=0.0, zmax=99.0e3):
+        """SCALE's radar operator on the resident ensemble: (hxf (nobs, ne) in dBZ or
+        m/s, qc (nobs, ne), rk (nobs)).  var: the 3-D variables holding u, v, w, T,
+        p, q, qc, qr, qi, qs, qg."""
+        arrs = [np.ascontiguousarray(np.broadcast_to(a, np.shape(elm)), dtype=np.float64)
+                for a in (elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z)]
+        n = arrs[0].size
+        c = _lib.RadarCfg()
+        for q in range(11):
+            c.var[q] = int(var[q])
+        c.method, c.use_terminal_velocity = int(method), int(bool(use_terminal_velocity))
+        c.min_ref_dbz, c.low_ref_shift, c.zmax = min_ref_dbz, low_ref_shift, zmax
+        hxf = np.zeros((n, self.ne), order="F")
+        qc = np.zeros((n, self.ne), dtype=np.int32, order="F")
+        rk = np.zeros(n)
+        check(_lib.lib().lekf_obsope_radar(self.h, ctypes.byref(c), n, *[dptr(a) for a in arrs],
+                                           hxf.ctypes.data_as(_lib._dp),
+                                           qc.ctypes.data_as(_lib._ip), dptr(rk)),
+              "obsope_radar")
+        return hxf, qc, rk
+
     def obs_index(self):
         """(perm, cum): input index of each binned obs and the cumulative obs count
         through each cell, shape (ny, nx) (cell (i, j) at [j-1, i-1])."""

@@ -115,6 +115,10 @@ def _lib_ref():
                                          _dp, c_int, _dp, _dp, _dp, c_double]
         _ref.ref_max_threads.restype = c_int
         _ref.ref_max_threads.argtypes = []
+        _ref.ref_com_gamma.restype = c_double
+        _ref.ref_com_gamma.argtypes = [c_double]
+        _ref.ref_com_distll_1.restype = c_double
+        _ref.ref_com_distll_1.argtypes = [c_double] * 4
         _ref.ref_spd_grid.restype = None
         _ref.ref_spd_grid.argtypes = [_ip]
         for f in (_ref.ref_spd_read_grd, _ref.ref_spd_write_grd):
@@ -548,3 +552,101 @@ def ref_write_obs2(path, obs):
     b = os.fsencode(path)
     _lib_ref().ref_spd_write_obs2(b, len(b), int(qc.size), *[_d(arr[k]) for k in _OBS_KEYS + ("ohx",)],
                                   _i(qc))
+
+
+# ---------------------------------------------------------------------------
+# SCALE radar obs operator (orc_phys2ijkz, orc_itpl_radar, orc_calc_ref_vr,
+# orc_radar_xtoy)
+# ---------------------------------------------------------------------------
+RADAR_FIELDS = ("u", "v", "w", "t", "p", "q", "qc", "qr", "qi", "qs", "qg")
+ID_RADAR_REF, ID_RADAR_VR = 4001, 4002
+IQC = dict(good=0, ref_low=11, radar_vhi=19, out_vhi=20, out_vlo=21, out_h=22, otype=90)
+
+
+def radar_gammas():
+    """Gamma(4.8), Gamma(4.25), Gamma(4.5): com_gamma of 4 + b, 4 + d and 4.5 in
+    calc_ref_vr (scale/common/common_obs_scale.f90)."""
+    import math
+    return np.array([math.gamma(4.8), math.gamma(4.25), math.gamma(4.5)])
+
+
+def _radar_lib():
+    lib = _lib_c()
+    if not getattr(lib, "_radar_bound", False):
+        lib.orc_phys2ijkz.restype = c_int
+        lib.orc_phys2ijkz.argtypes = [_dp, c_int, c_int, c_int, c_double, c_double, c_double, _dp]
+        lib.orc_itpl_radar.restype = c_double
+        lib.orc_itpl_radar.argtypes = [_dp, c_long, c_int, c_double, c_double, c_double]
+        lib.orc_calc_ref_vr.restype = c_int
+        lib.orc_calc_ref_vr.argtypes = [c_double] * 13 + [c_int, c_int, _dp, _dp, _dp]
+        lib.orc_radar_xtoy.restype = c_int
+        lib.orc_radar_xtoy.argtypes = ([c_int] + [c_double] * 8 + [_dp, c_long, _dp, c_int, c_int,
+                                        c_int, c_int, c_int, _dp, _dp, _dp, _dp])
+        lib._radar_bound = True
+    return lib
+
+
+def phys2ijkz(hgt, nx, ny, ri, rj, rlev):
+    """(qc, rk) of SCALE's phys2ijkz; hgt (nij, nlev) column-major."""
+    h = np.asfortranarray(hgt, dtype=np.float64)
+    rk = np.zeros(1)
+    qc = _radar_lib().orc_phys2ijkz(_d(h), int(nx), int(ny), int(h.shape[1]), float(ri), float(rj),
+                                    float(rlev), _d(rk))
+    return int(qc), float(rk[0])
+
+
+def itpl_radar(var, nx, ri, rj, rk):
+    """Trilinear value of var (nij, nlev) at 1-based (ri, rj, rk)."""
+    v = np.asfortranarray(var, dtype=np.float64)
+    return float(_radar_lib().orc_itpl_radar(_d(v), v.shape[0], int(nx), float(ri), float(rj),
+                                             float(rk)))
+
+
+def calc_ref_vr(fields, az, elev, method=3, use_wt=False, gam=None):
+    """(ref, vr) of calc_ref_vr from a dict of the point values of RADAR_FIELDS."""
+    g = radar_gammas() if gam is None else np.ascontiguousarray(gam, dtype=np.float64)
+    ref = np.zeros(1)
+    vr = np.zeros(1)
+    f = fields
+    st = _radar_lib().orc_calc_ref_vr(f["q"], f["qc"], f["qr"], f["qi"], f["qs"], f["qg"], f["u"],
+                                      f["v"], f["w"], f["t"], f["p"], float(az), float(elev),
+                                      int(method), int(bool(use_wt)), _d(g), _d(ref), _d(vr))
+    if st != 0:
+        raise ValueError(f"unknown reflectivity method {method}")
+    return float(ref[0]), float(vr[0])
+
+
+def radar_xtoy(obs, radar, x3, hgt, nx, ny, method=3, use_wt=False, min_ref_dbz=0.0,
+               low_ref_shift=0.0, zmax=99.0e3, gam=None):
+    """SCALE's radar obsope on every (obs, member).
+
+    obs: dict elm, lon, lat, z, ri, rj (nobs); radar: dict lon, lat, z (nobs);
+    x3: (nij, nlev, ne, 11) fields in RADAR_FIELDS order, column-major;
+    hgt (nij, nlev).  Returns hxf (nobs, ne), qc (nobs, ne), rk (nobs)."""
+    lib = _radar_lib()
+    x = np.asfortranarray(x3, dtype=np.float64)
+    h = np.asfortranarray(hgt, dtype=np.float64)
+    nij, nlev, ne, nf = x.shape
+    assert nf == 11 and nij == nx * ny and h.shape == (nij, nlev)
+    g = radar_gammas() if gam is None else np.ascontiguousarray(gam, dtype=np.float64)
+    opt = np.array([min_ref_dbz, low_ref_shift, zmax])
+    n = np.asarray(obs["elm"]).size
+    hxf = np.zeros((n, ne))
+    qc = np.zeros((n, ne), dtype=np.int32)
+    rkv = np.zeros(n)
+    y = np.zeros(1)
+    rk = np.zeros(1)
+    members = [np.asfortranarray(x[:, :, m, :]) for m in range(ne)]
+    for i in range(n):
+        for m in range(ne):
+            xm = members[m]
+            q = lib.orc_radar_xtoy(int(obs["elm"][i]), float(obs["lon"][i]), float(obs["lat"][i]),
+                                   float(obs["z"][i]), float(obs["ri"][i]), float(obs["rj"][i]),
+                                   float(radar["lon"][i]), float(radar["lat"][i]),
+                                   float(radar["z"][i]), _d(xm), nij * nlev, _d(h), int(nx), int(ny),
+                                   int(nlev), int(method), int(bool(use_wt)), _d(opt), _d(g), _d(y),
+                                   _d(rk))
+            hxf[i, m] = y[0]
+            qc[i, m] = q
+        rkv[i] = rk[0]
+    return hxf, qc, rkv

@@ -1205,3 +1205,275 @@ void orc_l96_rk4(int nx, int kt, double dt, double force, double *x)
     free(xt);
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* SCALE radar obs operator                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* scale/common/common_obs_scale.f90:71-72,141-151 */
+#define ID_RADAR_REF 4001
+#define ID_RADAR_VR 4002
+#define IQC_GOOD 0
+#define IQC_REF_LOW 11
+#define IQC_RADAR_VHI 19
+#define IQC_OUT_VHI 20
+#define IQC_OUT_VLO 21
+#define IQC_OUT_H 22
+#define IQC_OTYPE 90
+/* common/common.f90:29-30,38-40 */
+#define ORC_GG 9.81
+#define ORC_RD 287.05
+#define ORC_UNDEF (-9.99e33)
+
+/* Default-real (single precision) literals of calc_ref_vr, promoted to double
+ * where they meet double operands, as the Fortran compiler does. */
+#define F4(x) ((double)(float)(x))
+
+/*
+ * phys2ijkz (scale/common/common_obs_scale.f90 phys2ijkz) on a grid of nx x ny
+ * columns and nlev levels without halo.  hgt(ij, lev) column-major with
+ * ij = (i-1) + nx (j-1); ri, rj, rk are 1-based grid coordinates.  The
+ * reference's halo lets it accept ri = 1 and a height equal to the top level's;
+ * here the corners must lie inside the grid: 1 < ri <= nx, 1 < rj <= ny, and a
+ * height equal to the top level's takes the top layer.  Returns the qc.
+ */
+int orc_phys2ijkz(const double *hgt, int nx, int ny, int nlev, double ri, double rj,
+                  double rlev, double *rk)
+{
+    const long nij = (long)nx * ny;
+    *rk = ORC_UNDEF;
+    if (!(ri > 1.0 && ri <= (double)nx && rj > 1.0 && rj <= (double)ny)) return IQC_OUT_H;
+    const int i = (int)ceil(ri), j = (int)ceil(rj);
+    const double ai = ri - (double)(i - 1), aj = rj - (double)(j - 1);
+    int ks = 1;
+    for (int jj = j - 1; jj <= j; ++jj)
+        for (int ii = i - 1; ii <= i; ++ii) {
+            int k = 1;
+            for (; k <= nlev; ++k) {
+                const double z = hgt[(ii - 1) + (long)nx * (jj - 1) + nij * (k - 1)];
+                if (z > -300.0 && z < 10000.0) break;
+            }
+            if (k > ks) ks = k;
+        }
+    if (ks > nlev) return IQC_OUT_VLO;
+    /* itpl_2d_column at (ri, rj), term order of the reference */
+    double zl[512];
+    if (nlev > 512) return IQC_OUT_VHI;
+    for (int k = 1; k <= nlev; ++k) {
+        const double *z = hgt + nij * (k - 1);
+        const long c00 = (i - 2) + (long)nx * (j - 2), c10 = c00 + 1, c01 = c00 + nx,
+                   c11 = c01 + 1;
+        zl[k - 1] = z[c00] * (1.0 - ai) * (1.0 - aj) + z[c10] * ai * (1.0 - aj) +
+                    z[c01] * (1.0 - ai) * aj + z[c11] * ai * aj;
+    }
+    if (rlev > zl[nlev - 1]) return IQC_OUT_VHI;
+    if (rlev < zl[ks - 1]) return IQC_OUT_VLO;
+    int k = ks + 1;
+    for (; k <= nlev; ++k)
+        if (zl[k - 1] > rlev) break;
+    if (k > nlev) k = nlev;
+    const double ak = (rlev - zl[k - 2]) / (zl[k - 1] - zl[k - 2]);
+    *rk = (double)(k - 1) + ak;
+    if (!(*rk >= 1.0 && *rk <= (double)nlev)) {  /* heights not ascending */
+        *rk = ORC_UNDEF;
+        return IQC_OUT_VLO;
+    }
+    return IQC_GOOD;
+}
+
+/*
+ * itpl_3d as Trans_XtoY_radar calls it (scale/common/common_obs_scale.f90:1298-1325
+ * with the level index first): trilinear in (level rk, x ri, y rj), 1-based,
+ * over var(ij, lev) with stride sl between levels.  A level coordinate of
+ * exactly 1 takes layer (1, 2) with weight 0 on level 2, the same terms the
+ * reference sums with its halo level weighted 0.
+ */
+double orc_itpl_radar(const double *var, long sl, int nx, double ri, double rj, double rk)
+{
+    int kl = (int)ceil(rk);
+    if (kl < 2) kl = 2;
+    const int ix = (int)ceil(ri), jy = (int)ceil(rj);
+    const double al = rk - (double)(kl - 1), ax = ri - (double)(ix - 1), ay = rj - (double)(jy - 1);
+    const long c00 = (ix - 2) + (long)nx * (jy - 2), c10 = c00 + 1, c01 = c00 + nx, c11 = c01 + 1;
+    const double *lo = var + sl * (kl - 2), *hi = var + sl * (kl - 1);
+    return lo[c00] * (1.0 - al) * (1.0 - ax) * (1.0 - ay) + hi[c00] * al * (1.0 - ax) * (1.0 - ay) +
+           lo[c10] * (1.0 - al) * ax * (1.0 - ay) + hi[c10] * al * ax * (1.0 - ay) +
+           lo[c01] * (1.0 - al) * (1.0 - ax) * ay + hi[c01] * al * (1.0 - ax) * ay +
+           lo[c11] * (1.0 - al) * ax * ay + hi[c11] * al * ax * ay;
+}
+
+/*
+ * calc_ref_vr (scale/common/common_obs_scale.f90:609-960): reflectivity factor
+ * ref (mm^6 m^-3) and radial velocity vr from the mixing ratios, wind,
+ * temperature and pressure at the obs, for METHOD_REF_CALC 1, 2 or 3.
+ * gam[0..2] = Gamma(4.8), Gamma(4.25), Gamma(4.5) (com_gamma of 4 + b,
+ * 4 + d and 4.5).  Returns 0, or -1 for an unknown method.
+ */
+int orc_calc_ref_vr(double qv, double qc, double qr, double qci, double qs, double qg, double u,
+                    double v, double w, double t, double p, double az, double elev, int method,
+                    int use_wt, const double *gam, double *ref_out, double *vr_out)
+{
+    (void)qv;
+    (void)qc;
+    (void)qci;
+    const double pi = ORC_PI, deg2rad = ORC_PI / 180.0;
+    double zr = 0.0, zs = 0.0, zg = 0.0, zms = 0.0, zmg = 0.0, ref = 0.0, wt = 0.0;
+    double ro = p / (ORC_RD * t);
+    if (method == 1) {
+        const double nor = 8.0e6, ror = 1000.0, pip = pow(pi, 1.75), cf = 10.0e18 * 72, p0 = 1.0e5;
+        const double qt = qr + qs + qg;
+        if (qt > 0.0) {
+            ref = cf * pow(ro * qt, 1.75);
+            ref = ref / (pip * pow(nor, 0.75) * pow(ror, 1.75));
+            const double a = pow(p0 / p, F4(0.4));
+            wt = 5.40 * a * pow(qt, 0.125);
+        }
+    } else if (method == 2) {
+        double nor = 8.0e6, nos = 3.0e6, nog = 4.0e4, ror = 1000.0, ros = 100.0, rog = 913.0,
+               roi = 917.0, roo = 1.0;
+        const double ki2 = 0.176, kr2 = 0.930, pip = pow(pi, 1.75), cf = 1.0e18 * 720;
+        if (qr > 0.0) {
+            zr = cf * pow(ro * qr, 1.75);
+            zr = zr / (pip * pow(nor, 0.75) * pow(ror, 1.75));
+        }
+        if (qs > 0.0) {
+            if (t <= F4(273.16)) {
+                zs = cf * ki2 * pow(ros, 0.25) * pow(ro * qs, 1.75);
+                zs = zs / (pip * kr2 * pow(nos, 0.75) * (roi * roi));
+            } else {
+                zs = cf * pow(ro * qs, 1.75);
+                zs = zs / (pip * pow(nos, 0.75) * pow(roi, 1.75));
+            }
+        }
+        if (qg > 0.0) {
+            zg = pow(cf / (pip * pow(nog, 0.75) * pow(rog, 1.75)), F4(0.95));
+            zg = zg * pow(ro * qg, F4(1.6625));
+        }
+        ref = zr + zs + zg;
+        if (ref > 0.0) {
+            nor = nor * F4(1e-3);
+            nos = nos * F4(1e-3);
+            nog = nog * F4(1e-3);
+            ror = ror * F4(1e-3);
+            ros = ros * F4(1e-3);
+            rog = rog * F4(1e-3);
+            roo = roo * F4(1e-3);
+            ro = ro * F4(1e-3);
+            const double a = 2115.0, b = 0.8, c = 152.93, d = 0.25, Cd = 0.6;
+            const double lr = pow(pi * ror * nor / (ro * qr), 0.25);
+            const double ls = pow(pi * ros * nos / (ro * qs), 0.25);
+            const double lg = pow(pi * rog * nog / (ro * qg), 0.25);
+            const double rofactor = pow(roo / ro, 0.25);
+            double wr = a * gam[0] / (6.0 * pow(lr, b));
+            wr = 1.0e-2 * wr * rofactor;
+            double ws = c * gam[1] / (6.0 * pow(ls, d));
+            ws = 1.0e-2 * ws * rofactor;
+            double wg = gam[2] * pow((4.0 * ORC_GG * 100.0 * rog) / (3.0 * Cd * ro), 0.5);
+            wg = 1.0e-2 * wg / (6.0 * pow(lg, 0.5));
+            wt = (wr * zr + ws * zs + wg * zg) / (zr + zs + zg);
+        }
+    } else if (method == 3) {
+        const double maxf = 0.5;
+        double Fg = 0.0, Fs = 0.0, fwg = 0.0, fws = 0.0;
+        if (qr > 0.0 && qg > 0.0) {
+            Fg = maxf * pow(fmin(qr / qg, qg / qr), 1.0 / 3.0);
+            fwg = qr / (qr + qg);
+        }
+        if (qr > 0.0 && qs > 0.0) {
+            Fs = maxf * pow(fmin(qr / qs, qs / qr), 1.0 / 3.0);
+            fws = qr / (qr + qs);
+        }
+        const double qrp = (1.0 - Fs - Fg) * qr, qsp = (1.0 - Fs) * qs, qgp = (1.0 - Fg) * qg;
+        const double qms = Fs * (qr + qs), qmg = Fg * (qr + qg);
+        if (qrp > 0.0) zr = 2.53e4 * pow(ro * qrp * 1.0e3, F4(1.84));
+        if (qsp > 0.0) zs = 3.48e3 * pow(ro * qsp * 1.0e3, F4(1.66));
+        if (qgp > 0.0) zg = 8.18e4 * pow(ro * qgp * 1.0e3, 1.50);
+        if (qms > 0.0) {
+            zms = (F4(0.00491) + 5.75 * fws - F4(5.588) * (fws * fws)) * 1.0e5;
+            zms = zms * pow(ro * qms * 1.0e3, F4(1.67) - F4(0.202) * fws + F4(0.398) * (fws * fws));
+        }
+        if (qmg > 0.0) {
+            zmg = (F4(0.809) + F4(10.13) * fwg - F4(5.98) * (fwg * fwg)) * 1.0e5;
+            zmg = zmg * pow(ro * qmg * 1.0e3, F4(1.48) + F4(0.0448) * fwg - F4(0.0313) * (fwg * fwg));
+        }
+        ref = zr + zg + zs + zms + zmg;
+        if (ref > 0.0) {
+            const double nor = 8.0e-2, nos = 3.0e-2, nog = 4.0e-4, ror = 1.0, ros = 0.1, rog = 0.917,
+                         roo = 0.001;
+            ro = 1.0e-3 * ro;
+            const double a = 2115.0, b = 0.8, c = 152.93, d = 0.25, Cd = 0.6;
+            const double rofactor = pow(roo / ro, 0.5);
+            double wr = 0.0, ws = 0.0, wg = 0.0;
+            if (qr > 0.0) {
+                const double lr = pow(pi * ror * nor / (ro * qr), 0.25);
+                wr = a * gam[0] / (6.0 * pow(lr, b));
+                wr = 1.0e-2 * wr * rofactor;
+            }
+            if (qs > 0.0) {
+                const double ls = pow(pi * ros * nos / (ro * qs), 0.25);
+                ws = c * gam[1] / (6.0 * pow(ls, d));
+                ws = 1.0e-2 * ws * rofactor;
+            }
+            if (qg > 0.0) {
+                const double lg = pow(pi * rog * nog / (ro * qg), 0.25);
+                wg = gam[2] * pow((4.0 * ORC_GG * 100.0 * rog) / (3.0 * Cd * ro), 0.5);
+                wg = 1.0e-2 * wg / (6.0 * pow(lg, 0.5));
+            }
+            wt = (wr * zr + ws * zs + ws * zms + wg * zg + wg * zmg) / (zr + zs + zg + zms + zmg);
+        }
+    } else {
+        return -1;
+    }
+    double vr = u * cos(elev * deg2rad) * sin(az * deg2rad);
+    vr = vr + v * cos(elev * deg2rad) * cos(az * deg2rad);
+    if (use_wt)
+        vr = vr + (w - wt) * sin(elev * deg2rad);
+    else
+        vr = vr + w * sin(elev * deg2rad);
+    *ref_out = ref;
+    *vr_out = vr;
+    return 0;
+}
+
+/*
+ * The radar operator of SCALE's obsope for one (obs, member): the height and
+ * range checks of obsope_tools.f90:353-361 (RADAR_ZMAX, phys2ijkz), then
+ * Trans_XtoY_radar (common_obs_scale.f90:338-476, stggrd = 0) and obsope's
+ * reset of iqc_ref_low to good (obsope_tools.f90:380).
+ * x3: the member's 3-D fields var(ij, lev) for u, v, w, t, p, q, qc, qr, qi,
+ * qs, qg at x3 + f * sv (sv = the stride between fields, nij*nlev for a
+ * contiguous member); hgt(ij, lev) the level heights.  opt = {min_ref_dbz,
+ * low_ref_shift, zmax}.  Writes y and rk; returns the qc.
+ */
+int orc_radar_xtoy(int elm, double lon, double lat, double lev, double ri, double rj,
+                   double radar_lon, double radar_lat, double radar_z, const double *x3, long sv,
+                   const double *hgt, int nx, int ny, int nlev, int method, int use_wt,
+                   const double *opt, const double *gam, double *y, double *rk)
+{
+    const double deg2rad = ORC_PI / 180.0, rad2deg = 180.0 / ORC_PI;
+    const long sl = (long)nx * ny;
+    *y = ORC_UNDEF;
+    *rk = ORC_UNDEF;
+    if (elm != ID_RADAR_REF && elm != ID_RADAR_VR) return IQC_OTYPE;
+    if (lev > opt[2]) return IQC_RADAR_VHI;
+    int qc = orc_phys2ijkz(hgt, nx, ny, nlev, ri, rj, lev, rk);
+    if (qc != IQC_GOOD) return qc;
+    double f[11];
+    for (int n = 0; n < 11; ++n) f[n] = orc_itpl_radar(x3 + sv * n, sl, nx, ri, rj, *rk);
+    const double dlon = lon - radar_lon, dlat = lat - radar_lat;
+    if (dlon == 0.0 && dlat == 0.0) return IQC_OUT_H;
+    double az = rad2deg * atan2(dlon * cos(radar_lat * deg2rad), dlat);
+    if (az < 0.0) az = 360.0 + az;
+    const double dist = orc_distll_1(lon, lat, radar_lon, radar_lat);
+    const double elev = rad2deg * atan2(lev - radar_z, dist);
+    double ref = 0.0, vr = 0.0;
+    if (orc_calc_ref_vr(f[5], f[6], f[7], f[8], f[9], f[10], f[0], f[1], f[2], f[3], f[4], az, elev,
+                        method, use_wt, gam, &ref, &vr) != 0)
+        return IQC_OTYPE;
+    const double min_ref = pow(10.0, opt[0] / 10.0);
+    if (elm == ID_RADAR_REF)
+        *y = ref < min_ref ? opt[0] + opt[1] : 10.0 * log10(ref);
+    else
+        *y = vr;
+    return IQC_GOOD;  /* iqc_ref_low is reset to good by obsope */
+}

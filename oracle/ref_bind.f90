@@ -16,10 +16,12 @@
 !                     letkf_core, apply Xa = xmean + Xb' * trans.
 ! ref_l96_rk4      -> lorenz96/model/lorenz96.f90:93-128 (tinteg_rk4, the
 !                     reference model with its nx = 40), dt and force set
+! ref_com_gamma    -> common/common.f90:861-912 (com_gamma)
+! ref_com_distll_1 -> common/common.f90:401-424 (com_distll_1)
 !=======================================================================
 MODULE ref_bind
   USE ISO_C_BINDING
-  USE common, ONLY: r_size
+  USE common, ONLY: r_size, com_gamma, com_distll_1
   USE common_mtx, ONLY: mtx_eigen
   USE common_letkf, ONLY: letkf_core
   USE lorenz96, ONLY: tinteg_rk4, l96_nx => nx, l96_dt => dt, l96_force => force
@@ -35,6 +37,21 @@ SUBROUTINE ref_l96_rk4(kt, dt, force, xin, xout) BIND(C, NAME="ref_l96_rk4")
   l96_force = force
   CALL tinteg_rk4(kt, xin, xout)
 END SUBROUTINE ref_l96_rk4
+
+REAL(C_DOUBLE) FUNCTION ref_com_gamma(x) BIND(C, NAME="ref_com_gamma")
+  REAL(C_DOUBLE), VALUE :: x
+  REAL(r_size) :: xx, ga
+  xx = x
+  CALL com_gamma(xx, ga)
+  ref_com_gamma = ga
+END FUNCTION ref_com_gamma
+
+REAL(C_DOUBLE) FUNCTION ref_com_distll_1(alon, alat, blon, blat) BIND(C, NAME="ref_com_distll_1")
+  REAL(C_DOUBLE), VALUE :: alon, alat, blon, blat
+  REAL(r_size) :: d
+  CALL com_distll_1(alon, alat, blon, blat, d)
+  ref_com_distll_1 = d
+END FUNCTION ref_com_distll_1
 
 INTEGER(C_INT) FUNCTION ref_l96_nx() BIND(C, NAME="ref_l96_nx")
   ref_l96_nx = l96_nx
