@@ -15,6 +15,9 @@
   values and errors hPa -> Pa, rh values and errors percent -> fraction, in
   single precision as the reference does.
 
+* GrADS descriptors (``write_ctl``) of the grid files, laid out as
+  speedy/common/yyyymmddhh.ctl.
+
 All arrays are numpy; nothing here touches the GPU.
 """
 from __future__ import annotations
@@ -154,3 +157,26 @@ def count_obs(path, nval=8):
     if not os.path.exists(path):
         return 0
     return os.path.getsize(path) // (4 * nval + 8)
+
+
+SPEEDY_VARS = (("U", 7, "U-wind [m/s]"), ("V", 7, "V-wind [m/s]"), ("T", 7, "Temperature [K]"),
+               ("Q", 7, "Specific Humidity [kg/kg]"), ("PS", 0, "Surface Pressure [Pa]"),
+               ("RAIN", 0, "Precipitation [mm/6hr]"))
+
+
+def write_ctl(path, lat, sig, dset="^%y4%m2%d2%h2.grd", title="SPEEDY MODEL OUTPUT",
+              nlon=NLON, tdef=(10000, "00Z01JAN1982", "6HR"), variables=SPEEDY_VARS,
+              big_endian=True):
+    """GrADS descriptor of the grid files (the layout of speedy/common/yyyymmddhh.ctl):
+    longitudes 0, 360/nlon, ..., the given latitude table and sigma levels, one
+    record per (variable, level) in the order read_grd / write_grd use."""
+    opts = "template big_endian" if big_endian else "template"
+    lines = [f"DSET {dset}", f"TITLE {title}", "UNDEF -9.99E33", f"OPTIONS {opts}",
+             f"XDEF {nlon} LINEAR 0.0 {360.0 / nlon:.2f}",
+             f"YDEF {len(lat)} LEVELS " + " ".join(f"{v:7.3f}" for v in lat),
+             f"ZDEF {len(sig)} LEVELS " + " ".join(f"{v:5.3f}" for v in sig),
+             f"TDEF {tdef[0]} LINEAR {tdef[1]} {tdef[2]}", f"VARS {len(variables)}"]
+    lines += [f"{name} {nlev} 99 {desc}" for name, nlev, desc in variables]
+    lines.append("ENDVARS")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

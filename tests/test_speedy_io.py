@@ -155,3 +155,24 @@ def test_rejects_bad_files(tmp_path):
     with pytest.raises(ValueError):
         sio.read_grd(path)
     assert sio.count_obs(tmp_path / "missing.dat") == 0
+
+
+def test_grads_descriptor(tmp_path):
+    """The .ctl describes the records write_grd lays down: 96 longitudes from 0
+    by 3.75, the T30 latitude table, the 7 sigma levels, 4 x 7 + 2 records."""
+    from letkf_amd import speedy_cycle as sc
+    path = tmp_path / "yyyymmddhh.ctl"
+    sio.write_ctl(path, sc.LAT, sc.SIG)
+    lines = path.read_text().splitlines()
+    kv = {ln.split()[0]: ln.split()[1:] for ln in lines}
+    assert kv["OPTIONS"] == ["template", "big_endian"]
+    assert kv["XDEF"] == ["96", "LINEAR", "0.0", "3.75"]
+    lat = np.array(kv["YDEF"][2:], dtype=float)
+    np.testing.assert_array_equal(lat, sc.LAT)
+    np.testing.assert_array_equal(np.array(kv["ZDEF"][2:], dtype=float), sc.SIG)
+    i = lines.index("VARS 6")
+    rows = [ln.split()[:2] for ln in lines[i + 1:i + 7]]
+    assert [r[0] for r in rows] == ["U", "V", "T", "Q", "PS", "RAIN"]
+    nrec = sum(max(int(r[1]), 1) for r in rows)
+    assert nrec == sio.NV3D * sio.NLEV + sio.NV2D
+    assert lines[-1] == "ENDVARS"
