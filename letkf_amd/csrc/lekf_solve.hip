@@ -464,19 +464,26 @@ __device__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
     }
 }
 
-// One-sided (Hestenes) cyclic Jacobi of PPW points at once, for KR <= 32.
-// Lane group g (lanes g*KR .. g*KR+KR-1) holds point g; lane j of a group holds
-// column j of A (initially C, so A V -> U S with S = the eigenvalues of the
-// SPD C) and column j of V (initially I).  Each round pairs every column with
-// its tournament partner; the lane fetches the partner's columns with
-// cross-lane shuffles, forms [alpha gamma; gamma beta] = the 2x2 Gram of the
-// pair and the rotation that zeroes gamma (the same rotation two-sided Jacobi
-// would apply to C'C), and applies it to both of its columns.  Rows and
-// columns >= k are zero and never rotate; so does a point with no obs, and a
-// point that has converged while others of the wave still rotate, so a
-// point's result does not depend on the other points of its wave.  Writes the
-// eigenvalues and V over C in each point region, and LEKF_PT_NOCONV into
-// sc[6] of a point still rotating at the sweep limit.
+// Eigendecomposition of PPW points at once, for KR <= 32.  Lane group g (lanes
+// g*KR .. g*KR+KR-1) holds point g, lane j of a group column j of its matrix.
+//   1. Cholesky C = R^T R in registers: step s takes R[s][s] and row s of R
+//      from the owning lanes by shuffles.  R is kept in the point region.
+//   2. One-sided (Hestenes) cyclic Jacobi on the columns of R, tournament
+//      ordering: each round a lane fetches its partner's column, forms
+//      [alpha gamma; gamma beta] (the 2x2 Gram of the pair) and the rotation
+//      that zeroes gamma, and applies it to its column.  At convergence
+//      R J = U S with orthogonal columns, so C = R^T R = J S^2 J^T: the
+//      eigenvalues are the squared column norms.
+//   3. V = J = R^-1 (R J), one back substitution per lane with R read from
+//      the point region.
+// Only the columns of R travel between lanes (not V as well), and the
+// eigenvectors come from a triangular solve with R, whose condition number is
+// the square root of C's.  Rows and columns >= k are zero and never rotate;
+// so does a point with no obs, and a point that has converged while others of
+// the wave still rotate, so a point's result does not depend on the other
+// points of its wave.  Writes the eigenvalues and V over C in each point
+// region, and LEKF_PT_NOCONV into sc[6] of a point still rotating at the
+// sweep limit.
 template <int KR, int PPW>
 __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
 {
@@ -486,12 +493,31 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
     const int k = in ? (int)base[f.offS + 4] : 0;
     const int kk = (in && base[f.offS + 5] != 0.0) ? k : 0;
     const int lda = ((k + 1) & ~1) + 1;
-    double x[KR], v[KR];
+    const int gb = in ? g * KR : lane;  // lane of column 0 of this group (idle lanes: themselves)
+    double x[KR];
 #pragma unroll
-    for (int i = 0; i < KR; ++i) {
-        x[i] = (j < kk && i < kk) ? base[i * lda + j] : 0.0;
-        v[i] = (i == j && j < kk) ? 1.0 : 0.0;
+    for (int i = 0; i < KR; ++i) x[i] = (j < kk && i < kk) ? base[i * lda + j] : 0.0;
+    // 1. Cholesky, right-looking: after step s lane j (j >= s) holds R[s][j] in
+    // x[s] and has taken R[s][i] R[s][j] off its entries i in (s, j]
+#pragma unroll
+    for (int s0 = 0; s0 < KR; ++s0) {
+        const double d = __shfl(x[s0], in ? gb + s0 : lane, 64);
+        const double rss = d > 0.0 ? sqrt(d) : 0.0;
+        const double inv = rss > 0.0 ? 1.0 / rss : 0.0;
+        x[s0] = j > s0 ? x[s0] * inv : (j == s0 ? rss : 0.0);
+#pragma unroll
+        for (int i = s0 + 1; i < KR; ++i) {
+            const double ri = __shfl(x[s0], in ? gb + i : lane, 64);  // R[s0][i]
+            if (j >= i) x[i] = x[i] - ri * x[s0];
+        }
     }
+    __syncthreads();  // every lane has read its column of C before R goes over it
+    if (in && j < kk) {
+#pragma unroll
+        for (int i = 0; i < KR; ++i)
+            if (i < kk) base[i * lda + j] = x[i];
+    }
+    // 2. Jacobi on the columns of R
     double alpha = 0.0;
 #pragma unroll
     for (int i = 0; i < KR; ++i) alpha += x[i] * x[i];
@@ -504,7 +530,7 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
         bool big = false;  // a pair further than 1e-8 from orthogonal was rotated
         for (int r = 0; r < KR - 1; ++r) {
             const int pj = rr_partner(KR, r, j);
-            const int src = in ? g * KR + pj : lane;
+            const int src = in ? gb + pj : lane;
             double y[KR];
 #pragma unroll
             for (int i = 0; i < KR; ++i) y[i] = __shfl(x[i], src, 64);
@@ -530,11 +556,6 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
             const double sg = lo ? -s : s;
 #pragma unroll
             for (int i = 0; i < KR; ++i) x[i] = c * x[i] + sg * y[i];
-#pragma unroll
-            for (int i = 0; i < KR; ++i) {
-                const double vy = __shfl(v[i], src, 64);
-                v[i] = c * v[i] + sg * vy;
-            }
             if (rot) {
                 alpha = lo ? app - t * gamma : aqq + t * gamma;
                 rotated = true;
@@ -558,9 +579,21 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
     double nrm = 0.0;
 #pragma unroll
     for (int i = 0; i < KR; ++i) nrm += x[i] * x[i];
-    __syncthreads();  // every lane has read its column of C before V goes over it
+    // 3. V = R^-1 (R J): back substitution with R from the point region
+    __syncthreads();
+    double v[KR];
+#pragma unroll
+    for (int ii = KR - 1; ii >= 0; --ii) {
+        double sacc = x[ii];
+#pragma unroll
+        for (int l = ii + 1; l < KR; ++l)
+            if (l < kk) sacc = sacc - base[ii * lda + l] * v[l];
+        const double rii = ii < kk ? base[ii * lda + ii] : 0.0;
+        v[ii] = rii > 0.0 ? sacc / rii : 0.0;
+    }
+    __syncthreads();  // every lane has read R before V goes over it
     if (in && j < kk) {
-        base[f.offL + j] = sqrt(nrm);
+        base[f.offL + j] = nrm;
 #pragma unroll
         for (int i = 0; i < KR; ++i)
             if (i < kk) base[i * lda + j] = v[i];
