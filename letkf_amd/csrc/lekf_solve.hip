@@ -414,7 +414,7 @@ __device__ __forceinline__ int rr_partner(int n, int r, int j)
 // Gram stage of one point: C + (k-1)/alpha I into the point region, b, the
 // scalars of the inflation update, and the point's k and obs flag in sc.
 template <int NT, int MODE>
-__device__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
+__device__ __forceinline__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
 {
     const int k = J.k;
     const int n = (k + 1) & ~1, lda = n + 1;
@@ -466,8 +466,9 @@ __device__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
 
 // Eigendecomposition of PPW points at once, for KR <= 32.  Lane group g (lanes
 // g*KR .. g*KR+KR-1) holds point g, lane j of a group column j of its matrix.
-//   1. Cholesky C = R^T R in registers: step s takes R[s][s] and row s of R
-//      from the owning lanes by shuffles.  R is kept in the point region.
+//   1. Cholesky C = R^T R, column j in lane j: step s takes R[s][s] from its
+//      lane by a shuffle and row s of R through the point region, where R
+//      stays.
 //   2. One-sided (Hestenes) cyclic Jacobi on the columns of R, tournament
 //      ordering: each round a lane fetches its partner's column, forms
 //      [alpha gamma; gamma beta] (the 2x2 Gram of the pair) and the rotation
@@ -485,7 +486,7 @@ __device__ void fused_gram(const LekfJob& J, int b, SolveLds& L, int lane)
 // region, and LEKF_PT_NOCONV into sc[6] of a point still rotating at the
 // sweep limit.
 template <int KR, int PPW>
-__device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
+__device__ __forceinline__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
 {
     const int g = lane / KR, j = lane - g * KR;
     const bool in = g < PPW;
@@ -497,25 +498,22 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
     double x[KR];
 #pragma unroll
     for (int i = 0; i < KR; ++i) x[i] = (j < kk && i < kk) ? base[i * lda + j] : 0.0;
-    // 1. Cholesky, right-looking: after step s lane j (j >= s) holds R[s][j] in
-    // x[s] and has taken R[s][i] R[s][j] off its entries i in (s, j]
+    // 1. Cholesky, right-looking: at step s lane j (j >= s) forms R[s][j] in
+    // x[s] and writes it to row s of the point region (which C no longer
+    // needs: every lane holds its column), then takes R[s][i] R[s][j] off its
+    // entries i in (s, j] with row s read back from the region
+    __syncthreads();  // every lane has read its column of C
 #pragma unroll
     for (int s0 = 0; s0 < KR; ++s0) {
         const double d = __shfl(x[s0], in ? gb + s0 : lane, 64);
         const double rss = d > 0.0 ? sqrt(d) : 0.0;
         const double inv = rss > 0.0 ? 1.0 / rss : 0.0;
         x[s0] = j > s0 ? x[s0] * inv : (j == s0 ? rss : 0.0);
+        if (in && j < kk && s0 < kk) base[s0 * lda + j] = x[s0];
+        __syncthreads();
 #pragma unroll
-        for (int i = s0 + 1; i < KR; ++i) {
-            const double ri = __shfl(x[s0], in ? gb + i : lane, 64);  // R[s0][i]
-            if (j >= i) x[i] = x[i] - ri * x[s0];
-        }
-    }
-    __syncthreads();  // every lane has read its column of C before R goes over it
-    if (in && j < kk) {
-#pragma unroll
-        for (int i = 0; i < KR; ++i)
-            if (i < kk) base[i * lda + j] = x[i];
+        for (int i = 0; i < KR; ++i)  // constant trip count: unrolls fully
+            if (i > s0 && j >= i && i < kk) x[i] = x[i] - base[s0 * lda + i] * x[s0];
     }
     // 2. Jacobi on the columns of R
     double alpha = 0.0;
@@ -579,24 +577,24 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
     double nrm = 0.0;
 #pragma unroll
     for (int i = 0; i < KR; ++i) nrm += x[i] * x[i];
-    // 3. V = R^-1 (R J): back substitution with R from the point region
+    // 3. V = R^-1 (R J): back substitution with R from the point region, in
+    // place (x[l], l > ii, already holds V's entry l when row ii is solved)
     __syncthreads();
-    double v[KR];
 #pragma unroll
     for (int ii = KR - 1; ii >= 0; --ii) {
         double sacc = x[ii];
 #pragma unroll
-        for (int l = ii + 1; l < KR; ++l)
-            if (l < kk) sacc = sacc - base[ii * lda + l] * v[l];
+        for (int l = 0; l < KR; ++l)  // constant trip count: unrolls fully
+            if (l > ii && l < kk) sacc = sacc - base[ii * lda + l] * x[l];
         const double rii = ii < kk ? base[ii * lda + ii] : 0.0;
-        v[ii] = rii > 0.0 ? sacc / rii : 0.0;
+        x[ii] = rii > 0.0 ? sacc / rii : 0.0;
     }
     __syncthreads();  // every lane has read R before V goes over it
     if (in && j < kk) {
         base[f.offL + j] = nrm;
 #pragma unroll
         for (int i = 0; i < KR; ++i)
-            if (i < kk) base[i * lda + j] = v[i];
+            if (i < kk) base[i * lda + j] = x[i];
     }
     if (in && j == 0 && sweep == LEKF_JACOBI_MAX_SWEEPS && ((still >> (g * KR)) & gmask) != 0ull)
         base[f.offS + 6] = (double)LEKF_PT_NOCONV;
@@ -606,7 +604,7 @@ __device__ void jacobi_multi(double* smem, const FusedLayout& f, int lane)
 // the optional outputs and the fused apply.  With T = V diag(U) V^T + w-bar 1^T
 // (U = sqrt((k-1)/lambda)), the apply row is x-mean + (X' V) diag(U) V^T +
 // (X' . w-bar) 1^T, so T itself is formed only when the caller asks for it.
-__device__ void fused_finish(const LekfJob& J, int b, SolveLds& L, int lane)
+__device__ __forceinline__ void fused_finish(const LekfJob& J, int b, SolveLds& L, int lane)
 {
     const int k = J.k;
     const int n = (k + 1) & ~1, lda = n + 1;
