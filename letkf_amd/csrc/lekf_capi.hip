@@ -594,18 +594,6 @@ void free_all(lekf_handle* h)
     for (auto* b : bs) b->release();
 }
 
-// ln(ps sig) along the levels: -1 strictly falling, +1 strictly rising, 0 neither
-int sig_direction(const std::vector<double>& sig)
-{
-    if (sig.size() < 2) return 0;
-    bool dn = true, up = true;
-    for (size_t l = 1; l < sig.size(); ++l) {
-        dn = dn && sig[l] < sig[l - 1];
-        up = up && sig[l] > sig[l - 1];
-    }
-    return dn ? -1 : (up ? 1 : 0);
-}
-
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
 {
     HIPCHK(b.ensure(bytes));
@@ -792,7 +780,6 @@ int run_select(lekf_handle* h)
         A.loc_cnt = h->loc_cnt.as<int>();
         A.loc_idx = h->loc_idx.as<int>();
         A.loc_rho = h->loc_rho.as<double>();
-        A.lev_dir = sig_direction(h->sig);
         if (h->g.nobs_max <= 0) {
             HIPCHK(lekf_cull_global_fill_launch(A, h->nij, st));
         } else {

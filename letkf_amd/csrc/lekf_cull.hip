@@ -165,6 +165,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
 {
     const int ij = A.col_begin + blockIdx.x;
     const int lane = threadIdx.x;
+    __shared__ int s_cnt[LEKF_MAX_LEV];
     __shared__ int s_seg_beg[2 * LEKF_MAX_BANDS];
     __shared__ int s_seg_pre[2 * LEKF_MAX_BANDS + 1];
     const int nlev = A.nlev;
@@ -185,10 +186,8 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
             }
         s_seg_pre[nsegs] = acc;
     }
+    for (int l = lane; l < nlev; l += 64) s_cnt[l] = 0;
     __syncthreads();
-    // kept counts per level: lane l holds level l's in cnt0 and level l + 64's
-    // in cnt1 (nlev <= LEKF_MAX_LEV = 128), so no level needs a barrier
-    int cnt0 = 0, cnt1 = 0;
     const int ncand = s_seg_pre[nsegs];
     const int cap = A.cand_cnt[ij];
     const long base = A.cand_off[ij] * (long)nlev - A.list_base;
@@ -213,61 +212,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
         const int typ = valid ? A.o_typ[o] : 0;
         const double vl = (valid && typ >= 0) ? A.var_local[typ] : 0.0;
         hok = hok && (vl >= 2.2250738585072014e-308);
-        // Levels this candidate can be selected at.  With ln p monotone along
-        // the levels they form one run: two searches bracket the levels whose
-        // |ln p - ln p_obs| is within a slightly widened cutoff, one more level
-        // is added each side, and the wave loops over the union of the lanes'
-        // runs.  The exact test below still decides each level, so the skipped
-        // levels are ones where no lane is selected.
-        int l0 = nlev, l1 = -1;
-        if (hok) {
-            const bool ps = typ == LEKF_TYP_PS;
-            const double a = ps ? A.o_lndat[o] : A.o_lnlev[o];
-            const double f0 = A.logpfm[ij];
-            if (A.lev_dir == 0 || a != a || f0 != f0) {  // not monotone, or NaN: every level
-                l0 = 0;
-                l1 = nlev - 1;
-            } else {
-                const double dzw = A.dist_zerov * (1.0 + 1.0e-9) + 1.0e-300;
-                const double* f = A.logpfm + ij;
-                const long st = A.nij;
-                // first level in search order whose value passes a bound: f
-                // falling (lev_dir < 0): first l with f <= hiv, first with f < lov
-                auto first = [&](bool below, double v) {
-                    int lo = 0, hi = nlev;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        const double fm = f[(long)mid * st];
-                        const bool pass = below ? (A.lev_dir < 0 ? fm <= v : fm > v)
-                                                : (A.lev_dir < 0 ? fm < v : fm >= v);
-                        if (pass)
-                            hi = mid;
-                        else
-                            lo = mid + 1;
-                    }
-                    return lo;
-                };
-                int p, q;
-                if (A.lev_dir < 0) {
-                    p = first(true, a + dzw);       // f(l) <= a + dzw from p on
-                    q = first(false, a - dzw) - 1;  // f(l) >= a - dzw up to q
-                } else {
-                    p = first(false, a - dzw);      // f(l) >= a - dzw from p on
-                    q = first(true, a + dzw) - 1;   // f(l) <= a + dzw up to q
-                }
-                l0 = p - 1 > 0 ? p - 1 : 0;
-                l1 = q + 1 < nlev - 1 ? q + 1 : nlev - 1;
-                if (ps) l0 = 0;  // ps obs are selected at the first level
-                if (l1 < l0) l1 = ps ? 0 : l0 - 1;
-            }
-        }
-#pragma unroll
-        for (int o2 = 32; o2 > 0; o2 >>= 1) {
-            const int a0 = __shfl_xor(l0, o2, 64), a1 = __shfl_xor(l1, o2, 64);
-            l0 = a0 < l0 ? a0 : l0;
-            l1 = a1 > l1 ? a1 : l1;
-        }
-        for (int l = l0; l <= l1; ++l) {
+        for (int l = 0; l < nlev; ++l) {
             bool sel = false;
             double dlev = 0.0;
             if (hok) {
@@ -281,7 +226,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
             }
             const unsigned long long m = __ballot(sel);
             if (m == 0ull) continue;
-            const int c0 = __shfl(l < 64 ? cnt0 : cnt1, l & 63, 64);
+            const int c0 = s_cnt[l];
             if (sel) {
                 const int pos = c0 + __popcll(m & lane_mask_lt(lane));
                 const long w = base + (long)l * cap + pos;
@@ -289,18 +234,16 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
                 A.loc_idx[w] = o;
                 A.loc_rho[w] = exp(-0.5 * (a * a + bb * bb)) * vl;
             }
-            if (lane == (l & 63)) {
-                if (l < 64)
-                    cnt0 += __popcll(m);
-                else
-                    cnt1 += __popcll(m);
-            }
+            __syncthreads();
+            if (lane == 0) s_cnt[l] = c0 + __popcll(m);
+            __syncthreads();
         }
     }
+    __syncthreads();
     for (int l = lane; l < nlev; l += 64) {
         const long b = (long)l * A.nij + ij;
         A.loc_off[b] = base + (long)l * cap;
-        A.loc_cnt[b] = l < 64 ? cnt0 : cnt1;
+        A.loc_cnt[b] = s_cnt[l];
     }
 }
 

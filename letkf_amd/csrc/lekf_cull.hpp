@@ -31,7 +31,6 @@ struct GlobalCullArgs {
     double* loc_rho;
     int col_begin;               // first column of a fill launch
     long list_base;              // cand_off[col_begin] * nlev
-    int lev_dir;                 // logpfm along the levels: -1 falling, +1 rising, 0 neither
 };
 
 struct L96CullArgs {
