@@ -242,6 +242,37 @@ int lekf_obsope_radar(lekf_handle *h, const lekf_radar_cfg *cfg, int nobs, const
                       const double *lon, const double *lat, const double *z, const double *ri,
                       const double *rj, const double *radar_lon, const double *radar_lat,
                       const double *radar_z, double *hxf, int *qc, double *rk);
+/* Options of SCALE's departure and QC step: the PARAM_LETKF_OBS and
+ * PARAM_LETKF_RADAR namelist values of scale/common/common_nml.f90:83-91,
+ * 140-146,165-174 (defaults 1, 1, 0, 0, 1, 1, 15.0, 0.0, 0.0, 5.0, 3.0, 5.0 and
+ * -1.0 for every element's gross-error factor; a factor below zero stands for
+ * gross_error, common_nml.f90:348-371). */
+typedef struct lekf_scale_qc_cfg {
+    int use_radar_ref, use_radar_vr;
+    int use_obserr_radar_ref, use_obserr_radar_vr;
+    int min_radar_ref_member, min_radar_ref_member_obsref;
+    double radar_ref_thres_dbz, min_radar_ref_dbz, low_ref_shift;
+    double obserr_radar_ref, obserr_radar_vr;
+    double gross_error, gross_error_rain, gross_error_radar_ref, gross_error_radar_vr,
+        gross_error_radar_prh, gross_error_tcx, gross_error_tcy, gross_error_tcp;
+} lekf_scale_qc_cfg;
+/* SCALE's departure and QC step of set_letkf_obs on the GPU
+ * (scale/letkf/letkf_obs.f90:320-343,357,476-508,560-645; replaces the loop
+ * between the obs preprocessing and the sorting).  Per obs: element, value as
+ * observed (reflectivity in mm^6 m^-3; dat_out gives it in dBZ, undef -9.99e33
+ * outside [0, 1e10)), error (err_out: replaced by OBSERR_RADAR_REF / _VR when
+ * selected); qc_member(nobs,ne) the operator's qc of each member (NULL: all
+ * good), merged by the maximum; hx(nobs,ne) H(x) of each member, replaced by
+ * the perturbations H(x) - mean where the obs's qc comes out good (rows of
+ * other obs are left as they were).  dep = y - mean H(x) (0 unless good), qc
+ * 0 good, 5 gross error, 12 too few members with echo, 50 bad obs, 90 element
+ * not used, else the operator's.  Column-major arrays, ne = the handle's
+ * members.  Himawari-8 obs (elm 8800) are rejected with LEKF_ERR_ARG: their
+ * QC needs the RTTOV operator's cloud flags. */
+int lekf_obs_departure_scale(lekf_handle *h, const lekf_scale_qc_cfg *cfg, int nobs,
+                             const double *elm, const double *dat, const double *err,
+                             const int *qc_member, double *hx, double *dat_out, double *err_out,
+                             double *dep, int *qc);
 /* Regional obs after the obs operator (SCALE's obsda): grid-unit position
  * (ri, rj), height z (m), error, departure dep = y - mean H(x), bucket
  * (the (element, obs type) index, 0..3), qc (NULL: all good) and the obs-space

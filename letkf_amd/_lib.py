@@ -100,6 +100,29 @@ class RadarCfg(Structure):
                 ("min_ref_dbz", c_double), ("low_ref_shift", c_double), ("zmax", c_double)]
 
 
+class ScaleQcCfg(Structure):
+    _fields_ = ([(k, c_int) for k in ("use_radar_ref", "use_radar_vr", "use_obserr_radar_ref",
+                                      "use_obserr_radar_vr", "min_radar_ref_member",
+                                      "min_radar_ref_member_obsref")] +
+                [(k, c_double) for k in ("radar_ref_thres_dbz", "min_radar_ref_dbz",
+                                         "low_ref_shift", "obserr_radar_ref", "obserr_radar_vr",
+                                         "gross_error", "gross_error_rain",
+                                         "gross_error_radar_ref", "gross_error_radar_vr",
+                                         "gross_error_radar_prh", "gross_error_tcx",
+                                         "gross_error_tcy", "gross_error_tcp")])
+
+
+# namelist defaults of the departure and QC step (scale/common/common_nml.f90:83-91,
+# 140-146,165-174)
+SCALE_QC_DEFAULTS = dict(use_radar_ref=1, use_radar_vr=1, use_obserr_radar_ref=0,
+                         use_obserr_radar_vr=0, min_radar_ref_member=1,
+                         min_radar_ref_member_obsref=1, radar_ref_thres_dbz=15.0,
+                         min_radar_ref_dbz=0.0, low_ref_shift=0.0, obserr_radar_ref=5.0,
+                         obserr_radar_vr=3.0, gross_error=5.0, gross_error_rain=-1.0,
+                         gross_error_radar_ref=-1.0, gross_error_radar_vr=-1.0,
+                         gross_error_radar_prh=-1.0, gross_error_tcx=-1.0, gross_error_tcy=-1.0,
+                         gross_error_tcp=-1.0)
+
 # scale/common/common_obs_scale.f90:71-72
 ID_RADAR_REF, ID_RADAR_VR = 4001, 4002
 
@@ -222,6 +245,9 @@ def lib():
     L.lekf_set_heights.argtypes = [c_void_p, _dp]
     L.lekf_obsope_radar.restype = c_int
     L.lekf_obsope_radar.argtypes = [c_void_p, POINTER(RadarCfg), c_int] + [_dp] * 10 + [_ip, _dp]
+    L.lekf_obs_departure_scale.restype = c_int
+    L.lekf_obs_departure_scale.argtypes = ([c_void_p, POINTER(ScaleQcCfg), c_int] + [_dp] * 3 +
+                                           [_ip] + [_dp] * 4 + [_ip])
     L.lekf_tune_set.restype = c_int
     L.lekf_tune_set.argtypes = [c_int, c_long]
     L.lekf_tune_get.restype = c_long
@@ -244,7 +270,7 @@ EXPORTED = [
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
     "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
-    "lekf_obsope_radar",
+    "lekf_obsope_radar", "lekf_obs_departure_scale",
 ]
 
 

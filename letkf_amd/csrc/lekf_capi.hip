@@ -1127,6 +1127,83 @@ int lekf_set_heights(lekf_handle* h, const double* hgt)
     return LEKF_OK;
 }
 
+int lekf_obs_departure_scale(lekf_handle* h, const lekf_scale_qc_cfg* cfg, int nobs,
+                             const double* elm, const double* dat, const double* err,
+                             const int* qc_member, double* hx, double* dat_out, double* err_out,
+                             double* dep, int* qc)
+{
+    if (!h || !cfg || nobs < 0 || h->ne < 1) return LEKF_ERR_ARG;
+    if (nobs == 0) return LEKF_OK;
+    if (!elm || !dat || !err || !hx || !dat_out || !err_out || !dep || !qc) return LEKF_ERR_ARG;
+    for (int n = 0; n < nobs; ++n)
+        if ((int)elm[n] == LEKF_ID_H08IR) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const size_t n = (size_t)nobs, ne = (size_t)h->ne;
+    DevBuf rec, dhx, dqm, dq;
+    struct Release {  // the call's buffers go on every return path
+        DevBuf* b[4];
+        ~Release()
+        {
+            for (DevBuf* x : b) x->release();
+        }
+    } rel{{&rec, &dhx, &dqm, &dq}};
+    HIPCHK(rec.ensure(sizeof(double) * n * 6));
+    const double* in[3] = {elm, dat, err};
+    for (int f = 0; f < 3; ++f)
+        HIPCHK(hipMemcpyAsync(rec.as<double>() + n * f, in[f], sizeof(double) * n,
+                              hipMemcpyHostToDevice, st));
+    HIPCHK(dhx.ensure(sizeof(double) * n * ne));
+    HIPCHK(hipMemcpyAsync(dhx.p, hx, sizeof(double) * n * ne, hipMemcpyHostToDevice, st));
+    if (qc_member) {
+        HIPCHK(dqm.ensure(sizeof(int) * n * ne));
+        HIPCHK(hipMemcpyAsync(dqm.p, qc_member, sizeof(int) * n * ne, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(dq.ensure(sizeof(int) * n));
+    auto factor = [&](double f) { return f < 0.0 ? cfg->gross_error : f; };
+    ScaleQcArgs A{};
+    A.nobs = nobs;
+    A.ne = h->ne;
+    A.use_ref = cfg->use_radar_ref != 0;
+    A.use_vr = cfg->use_radar_vr != 0;
+    A.use_obserr_ref = cfg->use_obserr_radar_ref != 0;
+    A.use_obserr_vr = cfg->use_obserr_radar_vr != 0;
+    A.min_member = cfg->min_radar_ref_member;
+    A.min_member_obsref = cfg->min_radar_ref_member_obsref;
+    // common_nml.f90:596-598 and common_obs_scale.f90:249
+    A.thres = std::max(cfg->radar_ref_thres_dbz, cfg->min_radar_ref_dbz);
+    A.min_ref_dbz = cfg->min_radar_ref_dbz;
+    A.low_ref_shift = cfg->low_ref_shift;
+    A.min_ref = std::pow(10.0, cfg->min_radar_ref_dbz / 10.0);
+    A.obserr_ref = cfg->obserr_radar_ref;
+    A.obserr_vr = cfg->obserr_radar_vr;
+    A.gross = cfg->gross_error;
+    A.gross_rain = factor(cfg->gross_error_rain);
+    A.gross_ref = factor(cfg->gross_error_radar_ref);
+    A.gross_vr = factor(cfg->gross_error_radar_vr);
+    A.gross_prh = factor(cfg->gross_error_radar_prh);
+    A.gross_tcx = factor(cfg->gross_error_tcx);
+    A.gross_tcy = factor(cfg->gross_error_tcy);
+    A.gross_tcp = factor(cfg->gross_error_tcp);
+    double* r = rec.as<double>();
+    A.elm = r;
+    A.dat_in = r + n;
+    A.err_in = r + 2 * n;
+    A.dat = r + 3 * n;
+    A.err = r + 4 * n;
+    A.dep = r + 5 * n;
+    A.qc_mem = qc_member ? dqm.as<int>() : nullptr;
+    A.hx = dhx.as<double>();
+    A.qc = dq.as<int>();
+    HIPCHK(lekf_scale_qc_launch(A, st));
+    HIPCHK(hipMemcpyAsync(dat_out, r + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(err_out, r + 4 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dep, r + 5 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hx, dhx.p, sizeof(double) * n * ne, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(qc, dq.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LEKF_OK;
+}
+
 int lekf_obsope_radar(lekf_handle* h, const lekf_radar_cfg* cfg, int nobs, const double* elm,
                       const double* lon, const double* lat, const double* z, const double* ri,
                       const double* rj, const double* radar_lon, const double* radar_lat,

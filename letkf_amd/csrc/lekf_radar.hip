@@ -302,11 +302,82 @@ __global__ __launch_bounds__(64) void radar_obsope(RadarArgs A)
     }
 }
 
+// SCALE's departure and QC step (scale/letkf/letkf_obs.f90:320-343 obs
+// preprocessing, :357 the merged operator qc, :476-508 radar checks, :560-576
+// mean / perturbations / departure, :590-645 gross error).  One thread per
+// obs; the members of an obs are a column-major row hx[m * nobs + n], so a
+// wave reads 64 consecutive obs of one member per load.  The mean is summed in
+// member order as the reference does.
+__global__ __launch_bounds__(256) void scale_obs_qc(ScaleQcArgs A)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= A.nobs) return;
+    const long N = A.nobs;
+    const int elm = (int)A.elm[n];
+    double y = A.dat_in[n], e = A.err_in[n];
+    if (elm == LEKF_ID_RADAR_REF) {
+        if (y >= 0.0 && y < 1.0e10)
+            y = y < A.min_ref ? A.min_ref_dbz + A.low_ref_shift : 10.0 * log10(y);
+        else
+            y = LEKF_UNDEF;
+        if (A.use_obserr_ref) e = A.obserr_ref;
+    }
+    if (elm == LEKF_ID_RADAR_VR && A.use_obserr_vr) e = A.obserr_vr;
+    A.dat[n] = y;
+    A.err[n] = e;
+    A.dep[n] = 0.0;
+    int qc = LEKF_IQC_GOOD;
+    if (A.qc_mem)
+        for (int m = 0; m < A.ne; ++m) qc = max(qc, A.qc_mem[m * N + n]);
+    if (qc == LEKF_IQC_GOOD && elm == LEKF_ID_RADAR_REF) {
+        if (!A.use_ref) {
+            qc = LEKF_IQC_OTYPE;
+        } else if (y == LEKF_UNDEF) {
+            qc = LEKF_IQC_OBS_BAD;
+        } else {
+            const double t = A.thres + 1.0e-6;
+            int mem_ref = 0;
+            for (int m = 0; m < A.ne; ++m) mem_ref += A.hx[m * N + n] > t ? 1 : 0;
+            if (mem_ref < (y > t ? A.min_member_obsref : A.min_member)) qc = LEKF_IQC_REF_MEM;
+        }
+    }
+    if (qc == LEKF_IQC_GOOD && elm == LEKF_ID_RADAR_VR && !A.use_vr) qc = LEKF_IQC_OTYPE;
+    if (qc != LEKF_IQC_GOOD) {
+        A.qc[n] = qc;
+        return;
+    }
+    double mean = A.hx[n];
+    for (int m = 1; m < A.ne; ++m) mean = mean + A.hx[m * N + n];
+    mean = mean / (double)A.ne;
+    for (int m = 0; m < A.ne; ++m) A.hx[m * N + n] = A.hx[m * N + n] - mean;
+    const double d = y - mean;
+    A.dep[n] = d;
+    double fac;
+    switch (elm) {
+    case LEKF_ID_SCALE_RAIN: fac = A.gross_rain; break;
+    case LEKF_ID_RADAR_REF: fac = A.gross_ref; break;
+    case LEKF_ID_RADAR_VR: fac = A.gross_vr; break;
+    case LEKF_ID_RADAR_PRH: fac = A.gross_prh; break;
+    case LEKF_ID_TCLON: fac = A.gross_tcx; break;
+    case LEKF_ID_TCLAT: fac = A.gross_tcy; break;
+    case LEKF_ID_TCMIP: fac = A.gross_tcp; break;
+    default: fac = A.gross; break;
+    }
+    A.qc[n] = fabs(d) > fac * e ? LEKF_IQC_GROSS_ERR : LEKF_IQC_GOOD;
+}
+
 }  // namespace
 
 hipError_t lekf_radar_launch(const RadarArgs& A, hipStream_t st)
 {
     if (A.nobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(radar_obsope, dim3((unsigned)A.nobs), dim3(LEKF_WAVE), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t lekf_scale_qc_launch(const ScaleQcArgs& A, hipStream_t st)
+{
+    if (A.nobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_obs_qc, dim3((unsigned)((A.nobs + 255) / 256)), dim3(256), 0, st, A);
     return hipGetLastError();
 }

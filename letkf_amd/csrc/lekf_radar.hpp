@@ -6,11 +6,21 @@
 #define LEKF_ID_RADAR_REF 4001
 #define LEKF_ID_RADAR_VR 4002
 #define LEKF_IQC_GOOD 0
+#define LEKF_IQC_GROSS_ERR 5
+#define LEKF_IQC_REF_MEM 12
 #define LEKF_IQC_RADAR_VHI 19
 #define LEKF_IQC_OUT_VHI 20
 #define LEKF_IQC_OUT_VLO 21
 #define LEKF_IQC_OUT_H 22
+#define LEKF_IQC_OBS_BAD 50
 #define LEKF_IQC_OTYPE 90
+#define LEKF_ID_RADAR_PRH 4003
+#define LEKF_ID_H08IR 8800
+#define LEKF_ID_SCALE_RAIN 19999
+#define LEKF_ID_TCLON 99991
+#define LEKF_ID_TCLAT 99992
+#define LEKF_ID_TCMIP 99993
+#define LEKF_UNDEF (-9.99e33)
 
 struct RadarArgs {
     int nobs, nx, ny, nlev, ne, nv3d;
@@ -27,3 +37,22 @@ struct RadarArgs {
 };
 
 hipError_t lekf_radar_launch(const RadarArgs& A, hipStream_t st);
+
+// SCALE's departure and QC step of set_letkf_obs, one thread per obs.
+struct ScaleQcArgs {
+    int nobs, ne;
+    int use_ref, use_vr, use_obserr_ref, use_obserr_vr;
+    int min_member, min_member_obsref;
+    double thres;          // RADAR_REF_THRES_DBZ, raised to MIN_RADAR_REF_DBZ
+    double min_ref_dbz, low_ref_shift;
+    double min_ref;        // 10^(MIN_RADAR_REF_DBZ / 10)
+    double obserr_ref, obserr_vr;
+    double gross, gross_rain, gross_ref, gross_vr, gross_prh, gross_tcx, gross_tcy, gross_tcp;
+    const double *elm, *dat_in, *err_in;   // nobs each
+    const int* qc_mem;     // [m * nobs + n], or null
+    double* hx;            // [m * nobs + n]: H(x) in, perturbations out where qc is good
+    double *dat, *err, *dep;
+    int* qc;
+};
+
+hipError_t lekf_scale_qc_launch(const ScaleQcArgs& A, hipStream_t st);

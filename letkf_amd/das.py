@@ -284,6 +284,42 @@ class RegionalLetkf(_Handle):
               "obsope_radar")
         return hxf, qc, rk
 
+    def obs_departure(self, elm, dat, err, hxf, qc_member=None, **cfg):
+        """SCALE's departure and QC step of set_letkf_obs on the GPU.
+
+        elm, dat, err (nobs): obs as observed (reflectivity in mm^6 m^-3);
+        hxf (nobs, ne): H(x) of each member (the radar operator's output);
+        qc_member (nobs, ne): its qc.  cfg: the namelist options of
+        _lib.SCALE_QC_DEFAULTS.  Returns (dat, err, dep, hdxf, qc): the obs in dBZ
+        where it is a reflectivity, its error, y - mean H(x), the perturbations
+        (H(x) where qc is not good) and the qc."""
+        c = _lib.ScaleQcCfg()
+        opts = dict(_lib.SCALE_QC_DEFAULTS)
+        for k, v in cfg.items():
+            if k not in opts:
+                raise TypeError(f"unknown option {k}")
+            opts[k] = v
+        for k, v in opts.items():
+            setattr(c, k, int(v) if isinstance(getattr(c, k), int) else float(v))
+        a = [np.ascontiguousarray(np.broadcast_to(v, np.shape(elm)), dtype=np.float64)
+             for v in (elm, dat, err)]
+        n = a[0].size
+        hx = np.array(hxf, dtype=np.float64, order="F")
+        if hx.shape != (n, self.ne):
+            raise ValueError(f"hxf must be ({n}, {self.ne})")
+        q = None
+        if qc_member is not None:
+            q = np.asfortranarray(qc_member, dtype=np.int32)
+            if q.shape != (n, self.ne):
+                raise ValueError(f"qc_member must be ({n}, {self.ne})")
+        out = [np.zeros(n) for _ in range(3)]
+        qc = np.zeros(n, dtype=np.int32)
+        check(_lib.lib().lekf_obs_departure_scale(
+            self.h, ctypes.byref(c), n, *[dptr(x) for x in a],
+            q.ctypes.data_as(_lib._ip) if q is not None else None, hx.ctypes.data_as(_lib._dp),
+            *[dptr(x) for x in out], iptr(qc)), "obs_departure_scale")
+        return out[0], out[1], out[2], hx, qc
+
     def obs_index(self):
         """(perm, cum): input index of each binned obs and the cumulative obs count
         through each cell, shape (ny, nx) (cell (i, j) at [j-1, i-1])."""

@@ -560,7 +560,8 @@ def ref_write_obs2(path, obs):
 # ---------------------------------------------------------------------------
 RADAR_FIELDS = ("u", "v", "w", "t", "p", "q", "qc", "qr", "qi", "qs", "qg")
 ID_RADAR_REF, ID_RADAR_VR = 4001, 4002
-IQC = dict(good=0, ref_low=11, radar_vhi=19, out_vhi=20, out_vlo=21, out_h=22, otype=90)
+IQC = dict(good=0, gross_err=5, ref_low=11, ref_mem=12, radar_vhi=19, out_vhi=20, out_vlo=21,
+           out_h=22, obs_bad=50, otype=90)
 
 
 def radar_gammas():
@@ -650,3 +651,63 @@ def radar_xtoy(obs, radar, x3, hgt, nx, ny, method=3, use_wt=False, min_ref_dbz=
             qc[i, m] = q
         rkv[i] = rk[0]
     return hxf, qc, rkv
+
+
+# ---------------------------------------------------------------------------
+# SCALE departure / QC step of set_letkf_obs (orc_obs_departure_scale)
+# ---------------------------------------------------------------------------
+ID_RAIN, ID_RADAR_PRH, ID_H08IR = 19999, 4003, 8800
+ID_TCLON, ID_TCLAT, ID_TCMIP = 99991, 99992, 99993
+UNDEF = -9.99e33
+# namelist defaults (scale/common/common_nml.f90:83-91,140-146,165-174)
+SCALE_QC_DEFAULTS = dict(use_radar_ref=True, use_radar_vr=True, use_obserr_radar_ref=False,
+                         use_obserr_radar_vr=False, min_radar_ref_member=1,
+                         min_radar_ref_member_obsref=1, radar_ref_thres_dbz=15.0,
+                         min_radar_ref_dbz=0.0, low_ref_shift=0.0, obserr_radar_ref=5.0,
+                         obserr_radar_vr=3.0, gross_error=5.0, gross_error_rain=-1.0,
+                         gross_error_radar_ref=-1.0, gross_error_radar_vr=-1.0,
+                         gross_error_radar_prh=-1.0, gross_error_tcx=-1.0, gross_error_tcy=-1.0,
+                         gross_error_tcp=-1.0)
+
+
+def _scale_qc_cfg(**kw):
+    c = dict(SCALE_QC_DEFAULTS)
+    for k, v in kw.items():
+        if k not in c:
+            raise TypeError(f"unknown option {k}")
+        c[k] = v
+    return np.array([float(c[k]) for k in SCALE_QC_DEFAULTS])
+
+
+def obs_departure_scale(elm, dat, err, hxf, qc_member=None, **cfg):
+    """SCALE's departure and QC step (orc_obs_departure_scale) over nobs obs.
+
+    hxf (nobs, ne): H(x) of each member; qc_member (nobs, ne) or None.  Returns
+    (dat, err, dep, hdxf, qc): the preprocessed obs value and error, y - mean
+    H(x) (0 unless qc is good), the perturbations (H(x) where qc is not good)
+    and the qc."""
+    lib = _lib_c()
+    if not getattr(lib, "_dep_bound", False):
+        lib.orc_obs_departure_scale.restype = c_int
+        lib.orc_obs_departure_scale.argtypes = [c_int, c_double, c_double, _ip, c_int, _dp, _dp,
+                                                _dp, _dp, _dp]
+        lib._dep_bound = True
+    c = _scale_qc_cfg(**cfg)
+    hx = np.array(hxf, dtype=np.float64, order="F")
+    n, ne = hx.shape
+    qm = None if qc_member is None else np.asarray(qc_member, dtype=np.int32)
+    elm = np.asarray(elm)
+    out = [np.zeros(n) for _ in range(3)]
+    qc = np.zeros(n, dtype=np.int32)
+    y, e, d = np.zeros(1), np.zeros(1), np.zeros(1)
+    for i in range(n):
+        row = np.ascontiguousarray(hx[i])
+        qrow = None if qm is None else np.ascontiguousarray(qm[i])
+        q = lib.orc_obs_departure_scale(int(elm[i]), float(dat[i]), float(err[i]),
+                                        _i(qrow) if qrow is not None else None, ne, _d(row), _d(c),
+                                        _d(y), _d(e), _d(d))
+        if q < 0:
+            raise ValueError("H08 obs need the RTTOV operator's cloud flags")
+        hx[i] = row
+        out[0][i], out[1][i], out[2][i], qc[i] = y[0], e[0], d[0], q
+    return out[0], out[1], out[2], hx, qc

@@ -31,6 +31,11 @@
  *          spread at alpha 1, beta = 0 giving the background, the spread/mean
  *          of the limited q row, and a numpy restatement
  *          (tests/test_oracle_relax.py).
+ *   orc_radar_xtoy and its parts -> tests/test_oracle_radar.py.
+ *   orc_obs_departure_scale -> known answers of each rule at and beside its
+ *          threshold, the dBZ conversion at powers of ten, mean-free
+ *          perturbations, the departure against a direct mean
+ *          (tests/test_oracle_departure.py).
  */
 #include <float.h>
 #include <math.h>
@@ -1476,4 +1481,104 @@ int orc_radar_xtoy(int elm, double lon, double lat, double lev, double ri, doubl
     else
         *y = vr;
     return IQC_GOOD;  /* iqc_ref_low is reset to good by obsope */
+}
+
+/* scale/common/common_obs_scale.f90:63-78,141-151 */
+#define ID_SCALE_RAIN 19999
+#define ID_RADAR_PRH 4003
+#define ID_H08IR 8800
+#define ID_TCLON 99991
+#define ID_TCLAT 99992
+#define ID_TCMIP 99993
+#define IQC_GROSS_ERR 5
+#define IQC_REF_MEM 12
+#define IQC_OBS_BAD 50
+
+/* cfg indices of orc_obs_departure_scale (the namelist values of
+ * scale/common/common_nml.f90:83-91,140-146,165-174) */
+enum {
+    RQ_USE_REF, RQ_USE_VR, RQ_USE_OBSERR_REF, RQ_USE_OBSERR_VR, RQ_MIN_MEMBER,
+    RQ_MIN_MEMBER_OBSREF, RQ_THRES_DBZ, RQ_MIN_REF_DBZ, RQ_LOW_REF_SHIFT, RQ_OBSERR_REF,
+    RQ_OBSERR_VR, RQ_GROSS, RQ_GROSS_RAIN, RQ_GROSS_REF, RQ_GROSS_VR, RQ_GROSS_PRH,
+    RQ_GROSS_TCX, RQ_GROSS_TCY, RQ_GROSS_TCP, RQ_N
+};
+
+/*
+ * SCALE's departure and QC step of set_letkf_obs for one obs
+ * (scale/letkf/letkf_obs.f90):
+ *  - preprocessing (:320-343): a reflectivity obs in mm^6 m^-3 in [0, 1e10)
+ *    becomes 10 log10(dat) dBZ, or MIN_RADAR_REF_DBZ + LOW_REF_SHIFT below
+ *    MIN_RADAR_REF = 10^(MIN_RADAR_REF_DBZ/10) (common_obs_scale.f90:249);
+ *    outside that range undef.  USE_OBSERR_RADAR_REF / _VR replace the error;
+ *  - the qc of the obs is the maximum of the members' operator qc (the
+ *    MPI_MAX all-reduce of :266-272); a positive qc ends the step (:357);
+ *  - radar checks (:476-508): USE_RADAR_REF / USE_RADAR_VR false -> otype;
+ *    an undef reflectivity -> obs_bad; mem_ref = members with H(x) above
+ *    RADAR_REF_THRES_DBZ + 1e-6; an obs above that value needs
+ *    MIN_RADAR_REF_MEMBER_OBSREF such members, any other
+ *    MIN_RADAR_REF_MEMBER, else ref_mem.  RADAR_REF_THRES_DBZ below
+ *    MIN_RADAR_REF_DBZ is raised to it (common_nml.f90:596-598);
+ *  - mean of H(x) summed in member order, perturbations, departure
+ *    y - mean (:560-576);
+ *  - the gross-error check of each element (:590-645); a factor below zero
+ *    stands for GROSS_ERROR (common_nml.f90:348-371).
+ * qc_mem (ne) the operator's qc of each member (NULL: all good); hx (ne) in
+ * H(x), out the perturbations when the returned qc is good (else unchanged).
+ * Writes dat, err (preprocessed) and dep (0 unless good).  Returns the qc, or
+ * -1 for an H08 obs (its cloud flags come from the RTTOV operator, not built).
+ */
+int orc_obs_departure_scale(int elm, double dat_in, double err_in, const int *qc_mem, int ne,
+                            double *hx, const double *cfg, double *dat, double *err, double *dep)
+{
+    double thres = cfg[RQ_THRES_DBZ];
+    if (thres < cfg[RQ_MIN_REF_DBZ]) thres = cfg[RQ_MIN_REF_DBZ];
+    const double min_ref = pow(10.0, cfg[RQ_MIN_REF_DBZ] / 10.0);
+    double y = dat_in, e = err_in;
+    *dep = 0.0;
+    if (elm == ID_H08IR) return -1;
+    if (elm == ID_RADAR_REF) {
+        if (y >= 0.0 && y < 1.0e10)
+            y = y < min_ref ? cfg[RQ_MIN_REF_DBZ] + cfg[RQ_LOW_REF_SHIFT] : 10.0 * log10(y);
+        else
+            y = ORC_UNDEF;
+        if (cfg[RQ_USE_OBSERR_REF] != 0.0) e = cfg[RQ_OBSERR_REF];
+    }
+    if (elm == ID_RADAR_VR && cfg[RQ_USE_OBSERR_VR] != 0.0) e = cfg[RQ_OBSERR_VR];
+    *dat = y;
+    *err = e;
+    int qc = IQC_GOOD;
+    if (qc_mem)
+        for (int m = 0; m < ne; ++m)
+            if (qc_mem[m] > qc) qc = qc_mem[m];
+    if (qc > 0) return qc;
+    if (elm == ID_RADAR_REF) {
+        if (cfg[RQ_USE_REF] == 0.0) return IQC_OTYPE;
+        if (y == ORC_UNDEF) return IQC_OBS_BAD;
+        int mem_ref = 0;
+        for (int m = 0; m < ne; ++m)
+            if (hx[m] > thres + 1.0e-6) ++mem_ref;
+        const double need = y > thres + 1.0e-6 ? cfg[RQ_MIN_MEMBER_OBSREF] : cfg[RQ_MIN_MEMBER];
+        if (mem_ref < (int)need) return IQC_REF_MEM;
+    }
+    if (elm == ID_RADAR_VR && cfg[RQ_USE_VR] == 0.0) return IQC_OTYPE;
+    double mean = hx[0];
+    for (int m = 1; m < ne; ++m) mean = mean + hx[m];
+    mean = mean / (double)ne;
+    for (int m = 0; m < ne; ++m) hx[m] = hx[m] - mean;
+    const double d = y - mean;
+    *dep = d;
+    int g;
+    switch (elm) {
+    case ID_SCALE_RAIN: g = RQ_GROSS_RAIN; break;
+    case ID_RADAR_REF: g = RQ_GROSS_REF; break;
+    case ID_RADAR_VR: g = RQ_GROSS_VR; break;
+    case ID_RADAR_PRH: g = RQ_GROSS_PRH; break;
+    case ID_TCLON: g = RQ_GROSS_TCX; break;
+    case ID_TCLAT: g = RQ_GROSS_TCY; break;
+    case ID_TCMIP: g = RQ_GROSS_TCP; break;
+    default: g = RQ_GROSS; break;
+    }
+    const double fac = cfg[g] < 0.0 ? cfg[RQ_GROSS] : cfg[g];
+    if (fabs(d) > fac * e) return IQC_GROSS_ERR;
+    return IQC_GOOD;
 }
