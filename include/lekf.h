@@ -273,9 +273,25 @@ int lekf_obs_departure_scale(lekf_handle *h, const lekf_scale_qc_cfg *cfg, int n
                              const double *elm, const double *dat, const double *err,
                              const int *qc_member, double *hx, double *dat_out, double *err_out,
                              double *dep, int *qc);
+/* Radar obs of a regional handle set up without leaving the device: the
+ * radar operator (as lekf_obsope_radar), the departure and QC step (as
+ * lekf_obs_departure_scale, with the members' qc and H(x) of the operator)
+ * and the regional obs index (as lekf_set_obs_regional), in place of SCALE's
+ * obsope + set_letkf_obs for radar files (scale/obs/obsope_tools.f90:353-380,
+ * scale/letkf/letkf_obs.f90:320-830).  Per obs: the operator's inputs, the
+ * value as observed (reflectivity in mm^6 m^-3), its error and bucket (the
+ * (element, obs type) index of the cap, 0..3).  qc_out(nobs) (may be NULL):
+ * the qc of each obs. */
+int lekf_set_obs_radar(lekf_handle *h, const lekf_radar_cfg *rcfg, const lekf_scale_qc_cfg *qcfg,
+                       int nobs, const double *elm, const double *lon, const double *lat,
+                       const double *z, const double *ri, const double *rj,
+                       const double *radar_lon, const double *radar_lat, const double *radar_z,
+                       const double *dat, const double *err, const int *bucket, int *qc_out);
 /* Regional obs after the obs operator (SCALE's obsda): grid-unit position
  * (ri, rj), height z (m), error, departure dep = y - mean H(x), bucket
- * (the (element, obs type) index, 0..3), qc (NULL: all good) and the obs-space
+ * (the (element, obs type) index, 0..3), qc (1: use the obs, any other value:
+ * skip it, as the obs files' flag; NULL: use all; SCALE's qc codes, where 0 is
+ * good, map to qc == 0 ? 1 : 0) and the obs-space
  * perturbations hdxf(nobs,ne) column-major (scale/letkf/letkf_obs.f90:733-830). */
 int lekf_set_obs_regional(lekf_handle *h, int nobs, const double *ri, const double *rj,
                           const double *z, const double *err, const double *dep, const int *bucket,

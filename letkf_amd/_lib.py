@@ -248,6 +248,9 @@ def lib():
     L.lekf_obs_departure_scale.restype = c_int
     L.lekf_obs_departure_scale.argtypes = ([c_void_p, POINTER(ScaleQcCfg), c_int] + [_dp] * 3 +
                                            [_ip] + [_dp] * 4 + [_ip])
+    L.lekf_set_obs_radar.restype = c_int
+    L.lekf_set_obs_radar.argtypes = ([c_void_p, POINTER(RadarCfg), POINTER(ScaleQcCfg), c_int] +
+                                     [_dp] * 11 + [_ip, _ip])
     L.lekf_tune_set.restype = c_int
     L.lekf_tune_set.argtypes = [c_int, c_long]
     L.lekf_tune_get.restype = c_long
@@ -270,7 +273,7 @@ EXPORTED = [
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
     "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
-    "lekf_obsope_radar", "lekf_obs_departure_scale",
+    "lekf_obsope_radar", "lekf_obs_departure_scale", "lekf_set_obs_radar",
 ]
 
 

@@ -1191,6 +1191,8 @@ int lekf_obs_departure_scale(lekf_handle* h, const lekf_scale_qc_cfg* cfg, int n
     A.dat = r + 3 * n;
     A.err = r + 4 * n;
     A.dep = r + 5 * n;
+    A.so = 1;
+    A.sm = nobs;
     A.qc_mem = qc_member ? dqm.as<int>() : nullptr;
     A.hx = dhx.as<double>();
     A.qc = dq.as<int>();
@@ -1656,33 +1658,17 @@ lekf_handle* lekf_regional_create(const lekf_regional_cfg* cfg, int* status)
     return h;
 }
 
-int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const double* rj,
-                          const double* z, const double* err, const double* dep, const int* bucket,
-                          const int* qc, const double* hdxf)
+// The regional obs index from the obs in the handle's r_* buffers (ri, rj, z,
+// err, dep, bucket, qc if has_qc (obs kept where it equals qc_keep),
+// perturbations of member m of obs n at r_hxf[n * h_so + m * h_sm]): SCALE's bin sort by grid cell
+// (scale/letkf/letkf_obs.f90:733-830), the per-point list capacities and the
+// chunking of the points by the list budget.
+static int regional_obs_index(lekf_handle* h, int nobs, bool has_qc, int qc_keep, long h_so,
+                              long h_sm)
 {
-    if (!h || h->kind != 2 || nobs < 0) return LEKF_ERR_ARG;
-    if (h) graph_drop(h);
-    if (nobs > 0 && (!ri || !rj || !z || !err || !dep || !bucket || !hdxf)) return LEKF_ERR_ARG;
     hipStream_t st = h->st;
     const int ne = h->ne, nx = h->r.nx, ny = h->r.ny;
-    int nb = 1;
-    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;  // obs index and bucket share 32 bits
-    for (int n = 0; n < nobs; ++n) {
-        if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
-        nb = std::max(nb, bucket[n] + 1);
-    }
-    h->nbkt = nb;
     const size_t n1 = (size_t)(nobs > 0 ? nobs : 1);
-    int rc;
-    if ((rc = upload(h->r_lon, ri, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_lat, rj, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_z, z, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_err, err, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_dep, dep, sizeof(double) * nobs, st)) ||
-        (rc = upload(h->r_bkt, bucket, sizeof(int) * nobs, st)) ||
-        (rc = upload(h->r_hxf, hdxf, sizeof(double) * nobs * (size_t)ne, st)))
-        return rc;
-    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
     long n2 = 2048;
     while (n2 < nobs) n2 <<= 1;
     const int ncell = nx * ny;
@@ -1713,10 +1699,11 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     A.err = h->r_err.as<double>();
     A.dep = h->r_dep.as<double>();
     A.bkt = h->r_bkt.as<int>();
-    A.qc = qc ? h->r_qc.as<int>() : nullptr;
+    A.qc = has_qc ? h->r_qc.as<int>() : nullptr;
+    A.qc_keep = qc_keep;
     A.hdxf = h->r_hxf.as<double>();
-    A.h_so = 1;
-    A.h_sm = nobs;
+    A.h_so = h_so;
+    A.h_sm = h_sm;
     A.keys = h->keys.as<unsigned long long>();
     A.cell_cnt = h->cell_cnt.as<int>();
     A.perm = h->perm.as<int>();
@@ -1781,6 +1768,162 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     }
     h->have_obs = true;
     return upload_jobs(h);
+}
+
+
+int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const double* rj,
+                          const double* z, const double* err, const double* dep, const int* bucket,
+                          const int* qc, const double* hdxf)
+{
+    if (!h || h->kind != 2 || nobs < 0) return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
+    if (nobs > 0 && (!ri || !rj || !z || !err || !dep || !bucket || !hdxf)) return LEKF_ERR_ARG;
+    hipStream_t st = h->st;
+    const int ne = h->ne;
+    int nb = 1;
+    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;  // obs index and bucket share 32 bits
+    for (int n = 0; n < nobs; ++n) {
+        if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
+        nb = std::max(nb, bucket[n] + 1);
+    }
+    h->nbkt = nb;
+    int rc;
+    if ((rc = upload(h->r_lon, ri, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_lat, rj, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_z, z, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_err, err, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_dep, dep, sizeof(double) * nobs, st)) ||
+        (rc = upload(h->r_bkt, bucket, sizeof(int) * nobs, st)) ||
+        (rc = upload(h->r_hxf, hdxf, sizeof(double) * nobs * (size_t)ne, st)))
+        return rc;
+    if (qc && (rc = upload(h->r_qc, qc, sizeof(int) * nobs, st))) return rc;
+    return regional_obs_index(h, nobs, qc != nullptr, 1, 1, nobs);
+}
+
+// Radar obs straight from the resident ensemble: the radar operator, SCALE's
+// departure and QC step and the regional obs index, with H(x), the
+// perturbations and the departures kept on the device between them.
+int lekf_set_obs_radar(lekf_handle* h, const lekf_radar_cfg* rcfg, const lekf_scale_qc_cfg* qcfg,
+                       int nobs, const double* elm, const double* lon, const double* lat,
+                       const double* z, const double* ri, const double* rj, const double* radar_lon,
+                       const double* radar_lat, const double* radar_z, const double* dat,
+                       const double* err, const int* bucket, int* qc_out)
+{
+    if (!h || !rcfg || !qcfg || h->kind != 2 || nobs < 0 || !h->have_ens || !h->d_hgt.p ||
+        h->nlev < 2)
+        return LEKF_ERR_ARG;
+    if (h) graph_drop(h);
+    if (rcfg->method < 1 || rcfg->method > 3) return LEKF_ERR_ARG;
+    for (int q = 0; q < 11; ++q)
+        if (rcfg->var[q] < 0 || rcfg->var[q] >= h->nv3d) return LEKF_ERR_ARG;
+    const double* in[11] = {elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z, dat, err};
+    if (nobs > 0) {
+        for (auto* p : in)
+            if (!p) return LEKF_ERR_ARG;
+        if (!bucket) return LEKF_ERR_ARG;
+    }
+    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;
+    int nb = 1;
+    for (int n = 0; n < nobs; ++n) {
+        if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
+        nb = std::max(nb, bucket[n] + 1);
+    }
+    h->nbkt = nb;
+    hipStream_t st = h->st;
+    const size_t n = (size_t)nobs, ne = (size_t)h->ne, n1 = n > 0 ? n : 1;
+    DevBuf rec, dqm;
+    struct Release {  // the call's buffers go on every return path
+        DevBuf* b[2];
+        ~Release()
+        {
+            for (DevBuf* x : b) x->release();
+        }
+    } rel{{&rec, &dqm}};
+    HIPCHK(rec.ensure(sizeof(double) * n1 * 12));
+    double* r = rec.as<double>();
+    for (int f = 0; f < 11 && n > 0; ++f)
+        HIPCHK(hipMemcpyAsync(r + n * f, in[f], sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(dqm.ensure(sizeof(int) * n1 * ne));
+    HIPCHK(h->r_hxf.ensure(sizeof(double) * n1 * ne));
+    int rc;
+    if ((rc = upload(h->r_lon, ri, sizeof(double) * n, st)) ||
+        (rc = upload(h->r_lat, rj, sizeof(double) * n, st)) ||
+        (rc = upload(h->r_z, z, sizeof(double) * n, st)) ||
+        (rc = upload(h->r_bkt, bucket, sizeof(int) * n, st)))
+        return rc;
+    HIPCHK(h->r_err.ensure(sizeof(double) * n1));
+    HIPCHK(h->r_dep.ensure(sizeof(double) * n1));
+    HIPCHK(h->r_qc.ensure(sizeof(int) * n1));
+    RadarArgs A{};
+    A.nobs = nobs;
+    A.nx = h->r.nx;
+    A.ny = h->r.ny;
+    A.nlev = h->nlev;
+    A.ne = h->ne;
+    A.nv3d = h->nv3d;
+    for (int q = 0; q < 11; ++q) A.var[q] = rcfg->var[q];
+    A.method = rcfg->method;
+    A.use_wt = rcfg->use_terminal_velocity ? 1 : 0;
+    A.min_ref_dbz = rcfg->min_ref_dbz;
+    A.low_ref_shift = rcfg->low_ref_shift;
+    A.zmax = rcfg->zmax;
+    A.gam[0] = std::tgamma(4.8);
+    A.gam[1] = std::tgamma(4.25);
+    A.gam[2] = std::tgamma(4.5);
+    A.elm = r;
+    A.lon = r + n;
+    A.lat = r + 2 * n;
+    A.z = r + 3 * n;
+    A.ri = r + 4 * n;
+    A.rj = r + 5 * n;
+    A.rlon = r + 6 * n;
+    A.rlat = r + 7 * n;
+    A.rz = r + 8 * n;
+    A.x3 = h->x3b.as<double>();
+    A.hgt = h->d_hgt.as<double>();
+    A.hxf = h->r_hxf.as<double>();
+    A.qc = dqm.as<int>();
+    A.rk = r + 11 * n;
+    HIPCHK(lekf_radar_launch(A, st));
+    auto factor = [&](double f) { return f < 0.0 ? qcfg->gross_error : f; };
+    ScaleQcArgs Q{};
+    Q.nobs = nobs;
+    Q.ne = h->ne;
+    Q.use_ref = qcfg->use_radar_ref != 0;
+    Q.use_vr = qcfg->use_radar_vr != 0;
+    Q.use_obserr_ref = qcfg->use_obserr_radar_ref != 0;
+    Q.use_obserr_vr = qcfg->use_obserr_radar_vr != 0;
+    Q.min_member = qcfg->min_radar_ref_member;
+    Q.min_member_obsref = qcfg->min_radar_ref_member_obsref;
+    Q.thres = std::max(qcfg->radar_ref_thres_dbz, qcfg->min_radar_ref_dbz);
+    Q.min_ref_dbz = qcfg->min_radar_ref_dbz;
+    Q.low_ref_shift = qcfg->low_ref_shift;
+    Q.min_ref = std::pow(10.0, qcfg->min_radar_ref_dbz / 10.0);
+    Q.obserr_ref = qcfg->obserr_radar_ref;
+    Q.obserr_vr = qcfg->obserr_radar_vr;
+    Q.gross = qcfg->gross_error;
+    Q.gross_rain = factor(qcfg->gross_error_rain);
+    Q.gross_ref = factor(qcfg->gross_error_radar_ref);
+    Q.gross_vr = factor(qcfg->gross_error_radar_vr);
+    Q.gross_prh = factor(qcfg->gross_error_radar_prh);
+    Q.gross_tcx = factor(qcfg->gross_error_tcx);
+    Q.gross_tcy = factor(qcfg->gross_error_tcy);
+    Q.gross_tcp = factor(qcfg->gross_error_tcp);
+    Q.elm = r;
+    Q.dat_in = r + 9 * n;
+    Q.err_in = r + 10 * n;
+    Q.so = h->ne;  // the operator's rows: member m of obs n at n * ne + m
+    Q.sm = 1;
+    Q.qc_mem = dqm.as<int>();
+    Q.hx = h->r_hxf.as<double>();
+    Q.dat = r + 11 * n;  // the dBZ values are not kept: rk's slot is free again
+    Q.err = h->r_err.as<double>();
+    Q.dep = h->r_dep.as<double>();
+    Q.qc = h->r_qc.as<int>();
+    HIPCHK(lekf_scale_qc_launch(Q, st));
+    if (qc_out && n > 0)
+        HIPCHK(hipMemcpyAsync(qc_out, h->r_qc.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    return regional_obs_index(h, nobs, true, LEKF_IQC_GOOD, (long)ne, 1);
 }
 
 int lekf_set_stream(lekf_handle* h, void* stream)

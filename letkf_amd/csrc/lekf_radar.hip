@@ -305,14 +305,15 @@ __global__ __launch_bounds__(64) void radar_obsope(RadarArgs A)
 // SCALE's departure and QC step (scale/letkf/letkf_obs.f90:320-343 obs
 // preprocessing, :357 the merged operator qc, :476-508 radar checks, :560-576
 // mean / perturbations / departure, :590-645 gross error).  One thread per
-// obs; the members of an obs are a column-major row hx[m * nobs + n], so a
-// wave reads 64 consecutive obs of one member per load.  The mean is summed in
+// obs; member m of obs n at [n * so + m * sm]: column-major (so = 1, sm =
+// nobs) a wave reads 64 consecutive obs of one member per load, the radar
+// operator's rows (so = ne, sm = 1) are read in place.  The mean is summed in
 // member order as the reference does.
 __global__ __launch_bounds__(256) void scale_obs_qc(ScaleQcArgs A)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= A.nobs) return;
-    const long N = A.nobs;
+    const long so = A.so, sm = A.sm;
     const int elm = (int)A.elm[n];
     double y = A.dat_in[n], e = A.err_in[n];
     if (elm == LEKF_ID_RADAR_REF) {
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(256) void scale_obs_qc(ScaleQcArgs A)
     A.dep[n] = 0.0;
     int qc = LEKF_IQC_GOOD;
     if (A.qc_mem)
-        for (int m = 0; m < A.ne; ++m) qc = max(qc, A.qc_mem[m * N + n]);
+        for (int m = 0; m < A.ne; ++m) qc = max(qc, A.qc_mem[n * so + m * sm]);
     if (qc == LEKF_IQC_GOOD && elm == LEKF_ID_RADAR_REF) {
         if (!A.use_ref) {
             qc = LEKF_IQC_OTYPE;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void scale_obs_qc(ScaleQcArgs A)
         } else {
             const double t = A.thres + 1.0e-6;
             int mem_ref = 0;
-            for (int m = 0; m < A.ne; ++m) mem_ref += A.hx[m * N + n] > t ? 1 : 0;
+            for (int m = 0; m < A.ne; ++m) mem_ref += A.hx[n * so + m * sm] > t ? 1 : 0;
             if (mem_ref < (y > t ? A.min_member_obsref : A.min_member)) qc = LEKF_IQC_REF_MEM;
         }
     }
@@ -346,10 +347,10 @@ __global__ __launch_bounds__(256) void scale_obs_qc(ScaleQcArgs A)
         A.qc[n] = qc;
         return;
     }
-    double mean = A.hx[n];
-    for (int m = 1; m < A.ne; ++m) mean = mean + A.hx[m * N + n];
+    double mean = A.hx[n * so];
+    for (int m = 1; m < A.ne; ++m) mean = mean + A.hx[n * so + m * sm];
     mean = mean / (double)A.ne;
-    for (int m = 0; m < A.ne; ++m) A.hx[m * N + n] = A.hx[m * N + n] - mean;
+    for (int m = 0; m < A.ne; ++m) A.hx[n * so + m * sm] = A.hx[n * so + m * sm] - mean;
     const double d = y - mean;
     A.dep[n] = d;
     double fac;

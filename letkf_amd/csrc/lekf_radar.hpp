@@ -49,8 +49,9 @@ struct ScaleQcArgs {
     double obserr_ref, obserr_vr;
     double gross, gross_rain, gross_ref, gross_vr, gross_prh, gross_tcx, gross_tcy, gross_tcp;
     const double *elm, *dat_in, *err_in;   // nobs each
-    const int* qc_mem;     // [m * nobs + n], or null
-    double* hx;            // [m * nobs + n]: H(x) in, perturbations out where qc is good
+    long so, sm;           // member m of obs n at [n * so + m * sm] in qc_mem and hx
+    const int* qc_mem;     // the operator's qc of each member, or null
+    double* hx;            // H(x) in, perturbations out where qc is good
     double *dat, *err, *dep;
     int* qc;
 };

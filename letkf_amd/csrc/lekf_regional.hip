@@ -567,7 +567,7 @@ __global__ void reg_keys(RegObsArgs A)
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= A.nobs) return;
     unsigned long long key = ~0ull;
-    if (!A.qc || A.qc[n] == 1) {
+    if (!A.qc || A.qc[n] == A.qc_keep) {
         int i = (int)ceil(A.ri[n] - 0.5), j = (int)ceil(A.rj[n] - 0.5);
         i = i < 1 ? 1 : (i > A.nx ? A.nx : i);
         j = j < 1 ? 1 : (j > A.ny ? A.ny : j);

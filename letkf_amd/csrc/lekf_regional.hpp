@@ -12,7 +12,8 @@ struct RegObsArgs {
     int nobs, ne, nx, ny;
     const double *ri, *rj, *z, *err, *dep;  // ri, rj: 1-based grid units; z: metres
     const int* bkt;
-    const int* qc;                          // nullable (1 = good)
+    const int* qc;                          // nullable: keep the obs whose qc is qc_keep
+    int qc_keep;
     const double* hdxf;                     // obs perturbation of member i at hdxf[n*h_so + i*h_sm]
     long h_so, h_sm;
     unsigned long long* keys;               // padded to a power of two

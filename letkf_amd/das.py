@@ -225,6 +225,19 @@ class L96Letkf(_Handle):
               "l96_forecast")
 
 
+def _qc_cfg(cfg):
+    """lekf_scale_qc_cfg from the namelist defaults updated by cfg."""
+    c = _lib.ScaleQcCfg()
+    opts = dict(_lib.SCALE_QC_DEFAULTS)
+    for k, v in cfg.items():
+        if k not in opts:
+            raise TypeError(f"unknown option {k}")
+        opts[k] = v
+    for k, v in opts.items():
+        setattr(c, k, int(v) if isinstance(getattr(c, k), int) else float(v))
+    return c
+
+
 class RegionalLetkf(_Handle):
     def __init__(self, *, nx, ny, zlev, dx, dy, ne, nv3d, sigma_h, sigma_v, nobs_max=1000,
                  prec="f64", infl_mode="adaptive", infl_value=1.0, minfl=0.0, list_budget=0):
@@ -293,14 +306,7 @@ class RegionalLetkf(_Handle):
         _lib.SCALE_QC_DEFAULTS.  Returns (dat, err, dep, hdxf, qc): the obs in dBZ
         where it is a reflectivity, its error, y - mean H(x), the perturbations
         (H(x) where qc is not good) and the qc."""
-        c = _lib.ScaleQcCfg()
-        opts = dict(_lib.SCALE_QC_DEFAULTS)
-        for k, v in cfg.items():
-            if k not in opts:
-                raise TypeError(f"unknown option {k}")
-            opts[k] = v
-        for k, v in opts.items():
-            setattr(c, k, int(v) if isinstance(getattr(c, k), int) else float(v))
+        c = _qc_cfg(cfg)
         a = [np.ascontiguousarray(np.broadcast_to(v, np.shape(elm)), dtype=np.float64)
              for v in (elm, dat, err)]
         n = a[0].size
@@ -319,6 +325,31 @@ class RegionalLetkf(_Handle):
             q.ctypes.data_as(_lib._ip) if q is not None else None, hx.ctypes.data_as(_lib._dp),
             *[dptr(x) for x in out], iptr(qc)), "obs_departure_scale")
         return out[0], out[1], out[2], hx, qc
+
+    def set_obs_radar(self, elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z, dat, err,
+                      bucket, var=tuple(range(11)), method=3, use_terminal_velocity=False,
+                      zmax=99.0e3, **qc):
+        """Radar obs set up on the device: the radar operator, SCALE's departure and
+        QC step and the obs index, with nothing copied back but the qc (returned).
+        dat: the obs as observed (reflectivity in mm^6 m^-3); bucket: the (element,
+        obs type) index of the cap; qc: the namelist options of
+        _lib.SCALE_QC_DEFAULTS (min_radar_ref_dbz and low_ref_shift also go to the
+        operator)."""
+        arrs = [np.ascontiguousarray(np.broadcast_to(a, np.shape(elm)), dtype=np.float64)
+                for a in (elm, lon, lat, z, ri, rj, radar_lon, radar_lat, radar_z, dat, err)]
+        n = arrs[0].size
+        b = np.ascontiguousarray(np.broadcast_to(bucket, (n,)), dtype=np.int32)
+        qcfg = _qc_cfg(qc)
+        c = _lib.RadarCfg()
+        for q in range(11):
+            c.var[q] = int(var[q])
+        c.method, c.use_terminal_velocity = int(method), int(bool(use_terminal_velocity))
+        c.min_ref_dbz, c.low_ref_shift, c.zmax = qcfg.min_radar_ref_dbz, qcfg.low_ref_shift, zmax
+        out = np.zeros(n, dtype=np.int32)
+        check(_lib.lib().lekf_set_obs_radar(self.h, ctypes.byref(c), ctypes.byref(qcfg), n,
+                                            *[dptr(a) for a in arrs], iptr(b), iptr(out)),
+              "set_obs_radar")
+        return out
 
     def obs_index(self):
         """(perm, cum): input index of each binned obs and the cumulative obs count

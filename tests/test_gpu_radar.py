@@ -122,3 +122,50 @@ def test_radar_operator_arguments_checked(gpu, radar_case):
         h.radar_operator(*args, var=(12,) + VAR[1:])      # no such state variable
     hx, qc, rk = h.radar_operator(*args, var=VAR)
     assert hx.shape == (5, NE) and qc.shape == (5, NE) and rk.shape == (5,)
+
+
+def test_resident_radar_obs_equal_host_chain(gpu, radar_case):
+    """set_obs_radar (operator, departure/QC and obs index on the device) gives the
+    same qc, local lists and analysis, bit for bit, as the host-staged chain
+    radar_operator -> obs_departure -> set_obs."""
+    hgt, g3, _, obs, radar = radar_case
+    rng = np.random.default_rng(5)
+    elm = obs["elm"]
+    n = elm.size
+    ref, vr = elm == oracle.ID_RADAR_REF, elm == oracle.ID_RADAR_VR
+    dat = np.where(ref, 10.0 ** (rng.uniform(-5.0, 45.0, n) / 10.0), rng.normal(0.0, 8.0, n))
+    err = np.where(ref, 5.0, np.where(vr, 3.0, 1.0))
+    bucket = np.where(ref, 0, np.where(vr, 1, 2)).astype(np.int32)
+    kw = dict(var=VAR, method=3, zmax=4000.0)
+    qkw = dict(min_radar_ref_member=3, gross_error=4.0)
+
+    def handle():
+        h = das.RegionalLetkf(nx=NX, ny=NY, zlev=np.arange(NLEV) * 300.0, dx=1000.0, dy=1000.0,
+                              ne=NE, nv3d=12, sigma_h=3000.0, sigma_v=1000.0, nobs_max=50)
+        h.set_ensemble(g3)
+        h.set_heights(hgt)
+        return h
+
+    print(f"resident radar obs: {n} obs, k={NE}", flush=True)
+    h1 = handle()
+    hx, qcm, _ = h1.radar_operator(elm, obs["lon"], obs["lat"], obs["z"], obs["ri"], obs["rj"],
+                                   radar["lon"], radar["lat"], radar["z"], **kw)
+    _, e, dep, hd, qc = h1.obs_departure(elm, dat, err, hx, qcm, **qkw)
+    h1.set_obs(obs["ri"], obs["rj"], obs["z"], e, dep, bucket, hd, (qc == 0).astype(np.int32))
+    h1.run()
+    a1, _ = h1.analysis()
+    n1 = h1.nobsl()
+    h1.close()
+    h2 = handle()
+    qc2 = h2.set_obs_radar(elm, obs["lon"], obs["lat"], obs["z"], obs["ri"], obs["rj"],
+                           radar["lon"], radar["lat"], radar["z"], dat, err, bucket, **kw, **qkw)
+    h2.run()
+    a2, _ = h2.analysis()
+    n2 = h2.nobsl()
+    h2.close()
+    np.testing.assert_array_equal(qc2, qc)
+    assert (qc == 0).sum() > 50 and np.any(qc == oracle.IQC["gross_err"])
+    np.testing.assert_array_equal(n2, n1)
+    assert n1.max() > 0
+    np.testing.assert_array_equal(a2, a1)
+    assert not np.array_equal(a1, g3)                     # the obs did change the ensemble
