@@ -2750,10 +2750,17 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
     }
 }
 
+// Waves per SIMD asked of the register eigen kernel: at KR = 40..48 the
+// register QL fits two waves per SIMD with a few spills, which hides more of
+// its latency than the spills cost (measured on the GFS-sized problem: the
+// solve 28% faster at k = 40, 9.5% at 44, 2% at 48; at KR = 36 it is two
+// waves without the hint and the hint's spills made it 2% slower).
+constexpr int eig_waves_per_simd(int kr) { return (kr >= 40 && kr <= 48) ? 2 : 1; }
+
 template <int KR>
-__global__ __launch_bounds__(64) void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs,
-                                                      long gp0, double* __restrict__ ws,
-                                                      long gstride, int nvmax)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(eig_waves_per_simd(KR))))
+void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
+                     double* __restrict__ ws, long gstride, int nvmax)
 {
     extern __shared__ double smem[];
     const long gp = gp0 + blockIdx.x;
