@@ -2754,7 +2754,8 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
 // register QL fits two waves per SIMD with a few spills, which hides more of
 // its latency than the spills cost (measured on the GFS-sized problem: the
 // solve 28% faster at k = 40, 9.5% at 44, 2% at 48; at KR = 36 it is two
-// waves without the hint and the hint's spills made it 2% slower).
+// waves without the hint and the hint's spills made it 2% slower; at KR = 52
+// it spills 110 registers and the radar-sized cycle is 15% slower).
 constexpr int eig_waves_per_simd(int kr) { return (kr >= 40 && kr <= 48) ? 2 : 1; }
 
 template <int KR>
