@@ -188,6 +188,10 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
     }
     for (int l = lane; l < nlev; l += 64) s_cnt[l] = 0;
     __syncthreads();
+    // this workgroup's levels: group blockIdx.y of gridDim.y equal runs
+    const int lpg = (nlev + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int l_lo = (int)blockIdx.y * lpg;
+    const int l_hi = l_lo + lpg < nlev ? l_lo + lpg : nlev;
     const int ncand = s_seg_pre[nsegs];
     const int cap = A.cand_cnt[ij];
     const long base = A.cand_off[ij] * (long)nlev - A.list_base;
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
         const int typ = valid ? A.o_typ[o] : 0;
         const double vl = (valid && typ >= 0) ? A.var_local[typ] : 0.0;
         hok = hok && (vl >= 2.2250738585072014e-308);
-        for (int l = 0; l < nlev; ++l) {
+        for (int l = l_lo; l < l_hi; ++l) {
             bool sel = false;
             double dlev = 0.0;
             if (hok) {
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(64) void cull_global_fill(GlobalCullArgs A)
         }
     }
     __syncthreads();
-    for (int l = lane; l < nlev; l += 64) {
+    for (int l = l_lo + lane; l < l_hi; l += 64) {
         const long b = (long)l * A.nij + ij;
         A.loc_off[b] = base + (long)l * cap;
         A.loc_cnt[b] = s_cnt[l];
@@ -515,12 +519,25 @@ __global__ void logpfm_kernel(const double* x2b, int nij, int nv2d, int ips, int
 
 }  // namespace
 
+// Level groups of a fill launch over ncols columns: enough workgroups to give
+// every CU several waves (a column chunk of a capped global grid holds only a
+// few hundred columns), each group scanning the column's candidates for its
+// own run of levels.  A level's list is filled in scan order whatever the
+// grouping, so the lists do not depend on it.
+static unsigned fill_level_groups(int ncols, int nlev)
+{
+    constexpr int kFillBlocks = 2048;
+    const int g = (kFillBlocks + ncols - 1) / ncols;
+    return (unsigned)(g < 1 ? 1 : (g > nlev ? nlev : g));
+}
+
 hipError_t lekf_cull_global_launch(const GlobalCullArgs& A, hipStream_t st)
 {
     const int nb = (A.nij + 255) / 256;
     hipLaunchKernelGGL(cull_global_count, dim3(nb), dim3(256), 0, st, A);
     hipLaunchKernelGGL(scan_ex, dim3(1), dim3(1024), 0, st, A.cand_cnt, A.cand_off, A.nij);
-    hipLaunchKernelGGL(cull_global_fill, dim3(A.nij), dim3(64), 0, st, A);
+    hipLaunchKernelGGL(cull_global_fill, dim3(A.nij, fill_level_groups(A.nij, A.nlev)), dim3(64),
+                       0, st, A);
     return hipGetLastError();
 }
 
@@ -535,7 +552,8 @@ hipError_t lekf_cull_global_count_launch(const GlobalCullArgs& A, hipStream_t st
 hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, int ncols, hipStream_t st)
 {
     if (ncols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(cull_global_fill, dim3(ncols), dim3(64), 0, st, A);
+    hipLaunchKernelGGL(cull_global_fill, dim3(ncols, fill_level_groups(ncols, A.nlev)), dim3(64), 0,
+                       st, A);
     return hipGetLastError();
 }
 
