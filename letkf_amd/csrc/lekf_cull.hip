@@ -523,10 +523,12 @@ __global__ void logpfm_kernel(const double* x2b, int nij, int nv2d, int ips, int
 // every CU several waves (a column chunk of a capped global grid holds only a
 // few hundred columns), each group scanning the column's candidates for its
 // own run of levels.  A level's list is filled in scan order whatever the
-// grouping, so the lists do not depend on it.
+// grouping, so the lists do not depend on it.  4096 workgroups: on the
+// GFS-sized grid (chunks of ~270 columns, 64 levels) the selection took 615 ms
+// per cycle with 2048, 583 ms with 4096 and 615 ms with 8192.
 static unsigned fill_level_groups(int ncols, int nlev)
 {
-    constexpr int kFillBlocks = 2048;
+    constexpr int kFillBlocks = 4096;
     const int g = (kFillBlocks + ncols - 1) / ncols;
     return (unsigned)(g < 1 ? 1 : (g > nlev ? nlev : g));
 }
