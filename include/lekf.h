@@ -153,7 +153,10 @@ typedef struct lekf_global_cfg {
     int infl_mode;            /* LEKF_INFL_* */
     double infl_value;        /* fixed value, or the initial adaptive field value */
     int nobs_max;             /* > 0: keep at most nobs_max local obs per point, nearest
-                               * first by err^2/rho (ties: scan order); 0: no cap */
+                               * first (largest rho; ties: the earlier obs in scan order),
+                               * by the reference cap's insertion sort and storage-slot
+                               * order (scale/letkf/letkf_tools.f90:1041-1156, key -rho);
+                               * 0: no cap */
 } lekf_global_cfg;
 
 typedef struct lekf_l96_cfg {
@@ -175,7 +178,9 @@ typedef struct lekf_regional_cfg {
     int ne;
     int nv3d;                 /* state variables (all 3-D) */
     double sigma_h, sigma_v;  /* localisation scales (m) */
-    int nobs_max;             /* MAX_NOBS_PER_GRID per bucket; <= 0: no cap */
+    int nobs_max;             /* MAX_NOBS_PER_GRID per bucket (kept obs emitted bucket by
+                               * bucket in the reference's storage-slot order,
+                               * scale/letkf/letkf_tools.f90:1041-1156); <= 0: no cap */
     int prec;
     int infl_mode;
     double infl_value;
@@ -392,6 +397,9 @@ int lekf_event_sync(void *ev);
 #define LEKF_TUNE_GRAM_WAVES 0 /* waves per point of the k > 32 Gram kernel: 2, 4 or 8 */
 #define LEKF_TUNE_GRAM_WS_MB 1 /* Gram workspace per launch, MiB, 16 .. 65536 */
 #define LEKF_TUNE_SEL_PER_CU 2 /* capped-selection workgroups per CU: 1 .. 4 */
+#define LEKF_TUNE_EIG_SPLIT 3  /* 32 < k <= 64: 1 = tridiagonalisation, QL (one lane per point)
+                                  and eigenvector/apply kernels, 0 = one fused eigen kernel;
+                                  both give bit-identical results */
 int lekf_tune_set(int param, long value);
 long lekf_tune_get(int param);
 

@@ -47,6 +47,7 @@ RELAX = {"none": LEKF_RELAX_NONE, "rtpp": LEKF_RELAX_RTPP, "rtps": LEKF_RELAX_RT
 LEKF_TUNE_GRAM_WAVES = 0
 LEKF_TUNE_GRAM_WS_MB = 1
 LEKF_TUNE_SEL_PER_CU = 2
+LEKF_TUNE_EIG_SPLIT = 3
 LEKF_PT_NOCONV = 2
 LEKF_NVMAX = 16
 LEKF_KMAX = 256

@@ -13,6 +13,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -24,7 +25,7 @@ FLIB = os.path.join(FSRC, "liblekf_fortran.so")
 
 HIP_SOURCES = ["lekf_solve.hip", "lekf_cull.hip", "lekf_obsope.hip", "lekf_regional.hip",
                "lekf_relax.hip", "lekf_l96.hip", "lekf_radar.hip", "lekf_capi.hip"]
-F_SOURCES = ["common_letkf.f90", "letkf_tools_gpu.f90"]
+F_SOURCES = ["common_letkf.f90", "letkf_tools_gpu.f90", "letkf_speedy_gpu.f90"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -58,10 +59,18 @@ def _run(cmd, verbose):
 
 
 def build_hip(verbose: bool = False) -> str:
+    """Compile every HIP source to an object (in parallel) and link the library."""
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    cmds = [[_hipcc(), *HIPCC_FLAGS, "-I", INC, "-I", CSRC, "-c", s, "-o", o]
+            for s, o in zip(srcs, objs)]
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-shared", "-I", INC, "-I", CSRC, *srcs, "-o", tmp]
-    _run(cmd, verbose)
+    _run([_hipcc(), *HIPCC_FLAGS, "-shared", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 
@@ -80,13 +89,15 @@ def build_fortran(verbose: bool = False) -> str | None:
            "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
     _run(cmd, verbose)
     os.replace(tmp, FLIB)
-    # the Lorenz-96 driver program of the Fortran host side (used by the tests)
-    exe = os.path.join(FSRC, "l96_driver")
-    cmd = [fc, "-O2", "-I", mod, os.path.join(FSRC, "l96_driver.f90"), "-L", FSRC,
-           "-llekf_fortran", "-L", HERE, "-llekf_hip", "-Wl,-rpath,$ORIGIN",
-           "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
-    _run(cmd, verbose)
-    os.replace(exe + ".tmp", exe)
+    # the driver programs of the Fortran host side (used by the tests):
+    # the Lorenz-96 loop and the SPEEDY analysis sequence
+    for name in ("l96_driver", "speedy_driver"):
+        exe = os.path.join(FSRC, name)
+        cmd = [fc, "-O2", "-I", mod, os.path.join(FSRC, name + ".f90"), "-L", FSRC,
+               "-llekf_fortran", "-L", HERE, "-llekf_hip", "-Wl,-rpath,$ORIGIN",
+               "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
+        _run(cmd, verbose)
+        os.replace(exe + ".tmp", exe)
     return FLIB
 
 

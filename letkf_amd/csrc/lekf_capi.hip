@@ -347,6 +347,10 @@ int lekf_tune_set(int param, long value)
         if (value < 1 || value > 4) return LEKF_ERR_ARG;
         g_lekf_tune.sel_per_cu = (int)value;
         break;
+    case LEKF_TUNE_EIG_SPLIT:
+        if (value != 0 && value != 1) return LEKF_ERR_ARG;
+        g_lekf_tune.eig_split = (int)value;
+        break;
     default: return LEKF_ERR_ARG;
     }
     g_tune_epoch.fetch_add(1);
@@ -359,6 +363,7 @@ long lekf_tune_get(int param)
     case LEKF_TUNE_GRAM_WAVES: return g_lekf_tune.gram_waves;
     case LEKF_TUNE_GRAM_WS_MB: return (long)(g_lekf_tune.gram_ws_bytes >> 20);
     case LEKF_TUNE_SEL_PER_CU: return g_lekf_tune.sel_per_cu;
+    case LEKF_TUNE_EIG_SPLIT: return g_lekf_tune.eig_split;
     default: return -LEKF_ERR_ARG;
     }
 }
@@ -540,10 +545,12 @@ struct lekf_handle {
     DevBuf d_zlev, cellcum, cap_n, cap_off, o_ri, o_rj, o_z, o_bkt, r_dep, r_bkt, r_z;
     DevBuf d_hgt;                    // level heights (lekf_set_heights), radar operator
     DevBuf gws;                      // Gram workspace of the k > 32 solve
-    DevBuf sel_key, sel_ob, sel_rho;  // per-slot scratch of the capped selection
+    DevBuf sel_key, sel_ob, sel_rho;  // per-slot bucket streams of the capped regional selection
+    DevBuf sel_gkey, sel_gpos, sel_gslot;  // per-slot kept sets of caps above LEKF_CAP_LDS
     DevBuf maxcand;
     long slot_cap = 0;
     int sel_slots = 0;
+    int gcap = 0;  // global grid: the cap in effect (nobs_max, at most the longest raw list)
     // global grid with nobs_max: raw lists per column chunk, capped lists at b * nobs_max
     DevBuf capl_off, capl_cnt, capl_idx, capl_rho;
     std::vector<int> gchunk_cols;    // column chunk boundaries
@@ -570,7 +577,8 @@ long handle_bytes(const lekf_handle* h)
                           &h->loc_rho, &h->status, &h->djob, &h->d_zlev, &h->cellcum,
                           &h->cap_n, &h->cap_off, &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt,
                           &h->r_dep, &h->r_bkt, &h->r_z, &h->gws, &h->sel_key,
-                          &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off, &h->capl_cnt,
+                          &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos, &h->sel_gslot,
+                          &h->maxcand, &h->capl_off, &h->capl_cnt,
                           &h->capl_idx, &h->capl_rho, &h->d_hgt};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
@@ -589,9 +597,21 @@ void free_all(lekf_handle* h)
                     &h->cand_cnt, &h->cand_off, &h->loc_off, &h->loc_cnt, &h->loc_idx, &h->loc_rho,
                     &h->status, &h->djob, &h->d_zlev, &h->cellcum, &h->cap_n, &h->cap_off,
                     &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
-                    &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->maxcand, &h->capl_off,
+                    &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos,
+                    &h->sel_gslot, &h->maxcand, &h->capl_off,
                     &h->capl_cnt, &h->capl_idx, &h->capl_rho, &h->d_hgt};
     for (auto* b : bs) b->release();
+}
+
+// kept sets of a cap above LEKF_CAP_LDS: 2 * cap entries per selection slot
+hipError_t ensure_cap_sets(lekf_handle* h, int cap)
+{
+    if (cap <= LEKF_CAP_LDS) return hipSuccess;
+    const size_t n = (size_t)h->sel_slots * 2 * (size_t)cap;
+    hipError_t e;
+    if ((e = h->sel_gkey.ensure(sizeof(unsigned long long) * n)) != hipSuccess) return e;
+    if ((e = h->sel_gpos.ensure(sizeof(int) * n)) != hipSuccess) return e;
+    return h->sel_gslot.ensure(sizeof(int) * n);
 }
 
 int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
@@ -725,6 +745,9 @@ RegionalArgs make_reg_args(const lekf_handle* h)
     R.s_ob = h->sel_ob.as<int>();
     R.s_rho = h->sel_rho.as<double>();
     R.slot_cap = h->slot_cap;
+    R.g_key = h->sel_gkey.as<unsigned long long>();
+    R.g_pos = h->sel_gpos.as<int>();
+    R.g_slot = h->sel_gslot.as<int>();
     return R;
 }
 
@@ -788,20 +811,18 @@ int run_select(lekf_handle* h)
             memset(&K, 0, sizeof(K));
             K.nij = h->nij;
             K.nlev = h->nlev;
-            K.cap = h->g.nobs_max;
+            K.cap = h->gcap;
             K.in_off = h->loc_off.as<long>();
             K.in_cnt = h->loc_cnt.as<int>();
             K.in_idx = h->loc_idx.as<int>();
             K.in_rho = h->loc_rho.as<double>();
-            K.rdiag = h->o_rdiag.as<double>();
             K.out_off = h->capl_off.as<long>();
             K.out_cnt = h->capl_cnt.as<int>();
             K.out_idx = h->capl_idx.as<int>();
             K.out_rho = h->capl_rho.as<double>();
-            K.s_key = h->sel_key.as<unsigned long long>();
-            K.s_ob = h->sel_ob.as<int>();
-            K.s_rho = h->sel_rho.as<double>();
-            K.slot_cap = h->slot_cap;
+            K.g_key = h->sel_gkey.as<unsigned long long>();
+            K.g_pos = h->sel_gpos.as<int>();
+            K.g_slot = h->sel_gslot.as<int>();
             for (size_t c = 0; c + 1 < h->gchunk_cols.size(); ++c) {
                 A.col_begin = h->gchunk_cols[c];
                 A.list_base = h->gchunk_base[c];
@@ -1403,17 +1424,16 @@ int set_obs_global_dev(lekf_handle* h, int nobs, bool have_qc, long h_so, long h
         }
         for (int c = 0; c < h->nij; ++c) maxcol = std::max(maxcol, co[c + 1] - co[c]);
         h->loc_cap = maxraw;
-        const size_t npt = (size_t)h->nij * h->nlev, cap = (size_t)h->g.nobs_max;
+        // no list is longer than its column's candidate count: a larger cap
+        // never binds, and the capped lists need no more room than that
+        h->gcap = (int)std::min<long>(h->g.nobs_max, maxcol);
+        const size_t npt = (size_t)h->nij * h->nlev, cap = (size_t)h->gcap;
         HIPCHK(h->capl_off.ensure(sizeof(long) * npt));
         HIPCHK(h->capl_cnt.ensure(sizeof(int) * npt));
         HIPCHK(h->capl_idx.ensure(sizeof(int) * npt * cap));
         HIPCHK(h->capl_rho.ensure(sizeof(double) * npt * cap));
         h->sel_slots = lekf_reg_select_slots_max();
-        h->slot_cap = maxcol;
-        const size_t se = (size_t)h->sel_slots * (size_t)h->slot_cap;
-        HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * se));
-        HIPCHK(h->sel_ob.ensure(sizeof(int) * se));
-        HIPCHK(h->sel_rho.ensure(sizeof(double) * se));
+        HIPCHK(ensure_cap_sets(h, (int)cap));
     }
     HIPCHK(h->loc_idx.ensure(sizeof(int) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
     HIPCHK(h->loc_rho.ensure(sizeof(double) * (size_t)(h->loc_cap ? h->loc_cap : 1)));
@@ -1756,15 +1776,16 @@ static int regional_obs_index(lekf_handle* h, int nobs, bool has_qc, int qc_keep
     HIPCHK(h->loc_idx.ensure(sizeof(int) * maxlist));
     HIPCHK(h->loc_rho.ensure(sizeof(double) * maxlist));
     h->loc_cap = maxlist;
-    // capped selection: one scratch list per workgroup slot, as long as the
-    // largest rectangle's candidate count
+    // capped selection: per workgroup slot one stream per bucket, each as
+    // long as the largest rectangle's candidate count
     h->sel_slots = lekf_reg_select_slots_max();
     h->slot_cap = h->r.nobs_max > 0 ? std::max(maxcand, 1) : 0;
     if (h->slot_cap > 0) {
-        const size_t ne_ = (size_t)h->sel_slots * (size_t)h->slot_cap;
+        const size_t ne_ = (size_t)h->sel_slots * (size_t)h->nbkt * (size_t)h->slot_cap;
         HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * ne_));
         HIPCHK(h->sel_ob.ensure(sizeof(int) * ne_));
         HIPCHK(h->sel_rho.ensure(sizeof(double) * ne_));
+        HIPCHK(ensure_cap_sets(h, h->r.nobs_max));
     }
     h->have_obs = true;
     return upload_jobs(h);
@@ -1781,7 +1802,6 @@ int lekf_set_obs_regional(lekf_handle* h, int nobs, const double* ri, const doub
     hipStream_t st = h->st;
     const int ne = h->ne;
     int nb = 1;
-    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;  // obs index and bucket share 32 bits
     for (int n = 0; n < nobs; ++n) {
         if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;
         nb = std::max(nb, bucket[n] + 1);
@@ -1822,7 +1842,6 @@ int lekf_set_obs_radar(lekf_handle* h, const lekf_radar_cfg* rcfg, const lekf_sc
             if (!p) return LEKF_ERR_ARG;
         if (!bucket) return LEKF_ERR_ARG;
     }
-    if (nobs >= (1 << 30)) return LEKF_ERR_SIZE;
     int nb = 1;
     for (int n = 0; n < nobs; ++n) {
         if (bucket[n] < 0 || bucket[n] >= LEKF_REG_NB) return LEKF_ERR_ARG;

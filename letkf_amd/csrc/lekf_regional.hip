@@ -8,16 +8,29 @@
 // floor(ri -/+ dlon_zero) .. ceiling(...) is scanned row by row; a candidate
 // passes when ndist = (sqrt(rdx^2+rdy^2)/sigma_h)^2 + (|dz|/sigma_v)^2 is not
 // above the literal 13.33333333, with rloc = exp(-ndist/2) and
-// rdiag = err^2/rloc.  With MAX_NOBS_PER_GRID = nobs_max > 0 every bucket
-// (the reference's (element, obs type) pair) keeps its nobs_max smallest
-// (rdiag, scan position) pairs -- exactly the set the reference's insertion
-// sort keeps (strict < keeps earlier obs on ties).  Instead of inserting
-// serially, a workgroup per point evaluates every candidate once into a
-// scratch list in scan order, finds each bucket's threshold pair with
-// radix-digit histogram passes over the list's 64-bit keys (10-bit digits in
-// LDS), resolves the last digit bin by sorting at most LEKF_REG_LIST entries,
-// and emits the kept obs bucket by bucket in scan order.  Without a cap the
-// accepted obs are emitted in scan order.
+// rdiag = err^2/rloc.  Without a cap the accepted obs are emitted in scan
+// order.
+//
+// The cap MAX_NOBS_PER_GRID = nobs_max > 0 (:1041-1156).  Per bucket (the
+// reference's (element, obs type) pair) the reference keeps an
+// insertion-sorted set of at most nobs_max obs by rdiag in storage slots: an
+// obs that arrives while the set is not full takes the next free slot; later,
+// an obs with rdiag strictly below the set's last-priority entry evicts that
+// entry and takes over its slot (so an earlier obs wins a tie).  The kept obs
+// are emitted bucket by bucket in slot order.  cap_slots gives the same slots
+// without inserting serially:
+//   * a candidate enters when fewer than nobs_max candidates before it in scan
+//     order precede it in (rdiag, scan position) order, and the set after any
+//     prefix is the nobs_max first entries of that prefix in this order;
+//   * the set's last entry only moves down, so entries are evicted from the
+//     largest down: the i-th candidate that enters a full set evicts the i-th
+//     largest of the entries ever evicted, and takes its slot.
+// A block of 256 candidates at a time is merged into the set held in LDS
+// (ascending, double-buffered): each candidate's count of preceding entries
+// is a binary search in the set plus a count over the block's earlier
+// candidates, the new set and the block's evictions come from ranks in the
+// union, and a candidate evicted within its own block passes its slot on
+// (pointer jumping).
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
 #include "lekf_tune.hpp"
@@ -37,28 +50,42 @@ __device__ __forceinline__ unsigned long long lane_lt(int lane)
 
 constexpr int kSelT = 256;  // threads per workgroup of the selection (4 waves)
 constexpr int kSelW = kSelT / 64;
-constexpr int kSelU = 1;    // candidates / list entries per thread per round
+
+constexpr int kCapU = 1;               // candidates per thread in a block of the cap
+constexpr int kCapB = kSelT * kCapU;   // candidates per block
+
+// The kept set S of the cap (ascending (key, candidate), double-buffered) and
+// one block's potential entrants, entrants and evictions.
+struct CapLds {
+    unsigned long long skey[2][LEKF_CAP_LDS];
+    int spos[2][LEKF_CAP_LDS];
+    int sslot[2][LEKF_CAP_LDS];
+    unsigned long long pkey[kCapB];  // potential entrants, scan order
+    int ppos[kCapB];
+    unsigned long long nkey[kCapB];  // entrants, scan order
+    int npos[kCapB];
+    int nc1[kCapB];                  // entries of S before each entrant
+    int nU[kCapB];                   // entrant's rank in S + entrants
+    unsigned long long ekey[kCapB];  // entrants, ascending
+    int epos[kCapB];
+    int ev[kCapB];                   // evictions, ascending: slot (>= 0) or -(entrant + 1)
+    int slotn[kCapB];
+    int link[kCapB];
+    int wc[2][kCapU][kSelW];
+};
+
+// S in global memory (caps above LEKF_CAP_LDS): 2 * cap entries per array
+struct CapGlobal {
+    unsigned long long* key;
+    int* pos;
+    int* slot;
+};
 
 struct SelLds {
-    int hist[LEKF_REG_NB][1024];
-    unsigned long long lbits[LEKF_REG_NB][LEKF_REG_LIST];
-    int lpos[LEKF_REG_NB][LEKF_REG_LIST];
+    CapLds cap;
     int seg_beg[LEKF_REG_SEG];
     int seg_pre[LEKF_REG_SEG + 1];
-    int cnt[LEKF_REG_NB];
-    int need[LEKF_REG_NB];
-    int bincnt[LEKF_REG_NB];
-    int mode[LEKF_REG_NB];  // 0 keep all, 1 refine, 2 collect, 3 tie scan
-    int shift[LEKF_REG_NB];
-    int lcnt[LEKF_REG_NB];
-    int thr_pos[LEKF_REG_NB];
-    int run[LEKF_REG_NB];
-    int base[LEKF_REG_NB + 1];
-    unsigned long long pref[LEKF_REG_NB];
-    unsigned long long thr_bits[LEKF_REG_NB];
     int wcnt[kSelW][LEKF_REG_NB];
-    int ecnt[kSelW][kSelU][LEKF_REG_NB];
-    int any;
 };
 
 struct Cand {
@@ -118,276 +145,374 @@ __device__ __forceinline__ int cand_obs(const SelLds& S, int nseg, int q, int& s
     return S.seg_beg[seg] + (q - S.seg_pre[seg]);
 }
 
-// Digits of a positive double's bit pattern (monotone in the value): shifts
-// 53, 43, 33, 23, 13, 3 take 10 bits (bit 63 is the zero sign bit), shift 0
-// the last 3 bits.
-__device__ __forceinline__ int digit_width(int shift) { return shift == 0 ? 3 : 10; }
-
-__device__ __forceinline__ int digit(unsigned long long bits, int shift)
-{
-    return (int)((bits >> shift) & ((1ull << digit_width(shift)) - 1ull));
-}
-
-// the bits at and above shift + width(shift) are already fixed in pref;
-// shift < 0 means every bit is fixed
-__device__ __forceinline__ bool prefix_match(unsigned long long bits, unsigned long long pref,
-                                             int shift)
-{
-    if (shift < 0) return bits == pref;
-    const int fixed_from = shift + digit_width(shift);
-    if (fixed_from >= 63) return true;
-    return (bits >> fixed_from) == (pref >> fixed_from);
-}
-
-__device__ __forceinline__ int next_shift(int shift)
-{
-    return shift == 3 ? 0 : (shift == 0 ? -1 : shift - 10);
-}
-
-// Block-ordered append: every thread with flag set gets a distinct position
-// in block (thread) order after the S.run[slot] entries already placed; the
-// per-slot running count advances by the block's total.  Uniform call.
-__device__ __forceinline__ int block_append(SelLds& S, bool flag, int slot, int tid)
+// Block-wide compaction of U flags per thread in (u, thread) order: pos[u]
+// is the position of entry u among the flagged ones, tot their number.  The
+// counts alternate between two buffers, so one barrier suffices.  Uniform
+// call.
+template <int U>
+__device__ __forceinline__ void block_rank(int (*wc)[U][kSelW], int& par, const bool (&flag)[U],
+                                           int tid, int (&pos)[U], int& tot)
 {
     const int lane = tid & 63, wv = tid >> 6;
-    const unsigned long long m = __ballot(flag);
-    if (lane == 0) S.wcnt[wv][slot] = __popcll(m);
-    __syncthreads();
-    int off = S.run[slot];
-    for (int w = 0; w < wv; ++w) off += S.wcnt[w][slot];
-    const int pos = off + __popcll(m & lane_lt(lane));
-    __syncthreads();
-    if (tid == 0) {
-        int t = 0;
-        for (int w = 0; w < kSelW; ++w) t += S.wcnt[w][slot];
-        S.run[slot] += t;
+    unsigned long long m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        m[u] = __ballot(flag[u]);
+        if (lane == 0) wc[par][u][wv] = __popcll(m[u]);
     }
     __syncthreads();
-    return pos;
+    int base = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        int off = 0, t = 0;
+#pragma unroll
+        for (int w = 0; w < kSelW; ++w) {
+            const int c = wc[par][u][w];
+            off += w < wv ? c : 0;
+            t += c;
+        }
+        pos[u] = base + off + __popcll(m[u] & lane_lt(lane));
+        base += t;
+    }
+    tot = base;
+    par ^= 1;
 }
 
-// Keep, per bucket, the cap entries with the smallest (key, list position) of
-// a list in scan order (key = rdiag bits, ob = obs | bucket << 30, rho) and
-// write them bucket by bucket in list order to out_idx / out_rho; returns the
-// count.  Each over-full bucket's threshold pair is found by digit histograms
-// over the list.  Uniform call by all kSelT threads.
-__device__ int cap_select(SelLds& S, int tid, const unsigned long long* skey, const int* sob,
-                          const double* srho, int nacc, int nb, int cap, int* out_idx,
-                          double* out_rho)
+// V binary searches in lockstep (their loads overlap): out[v] = entries of the
+// ascending (key, candidate) array [0, n) before (x[v], xp[v])
+template <int V>
+__device__ __forceinline__ void count_before(const unsigned long long* key, const int* pos, int n,
+                                             const unsigned long long (&x)[V],
+                                             const int (&xp)[V], int (&out)[V])
 {
-    for (int t = tid; t < LEKF_REG_NB; t += kSelT) {
-        S.cnt[t] = 0;
-        S.lcnt[t] = 0;
-        S.mode[t] = 0;
-        S.run[t] = 0;
+    int lo[V], hi[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        lo[v] = 0;
+        hi[v] = n;
     }
-    for (int t = tid; t < LEKF_REG_NB * 1024; t += kSelT) (&S.hist[0][0])[t] = 0;
-    __syncthreads();
-    for (int i = tid; i < nacc; i += kSelT) {
-        const int t = sob[i] >> 30;
-        atomicAdd(&S.cnt[t], 1);
-        atomicAdd(&S.hist[t][digit(skey[i], 53)], 1);
-    }
-    __syncthreads();
-    // boundary digit of every bucket over its cap
-    if (tid < nb) {
-        const int t = tid;
-        if (S.cnt[t] > cap) {
-            int need = cap, d = 0, acc = 0;
-            for (; d < 1024; ++d) {
-                if (acc + S.hist[t][d] >= need) break;
-                acc += S.hist[t][d];
-            }
-            S.need[t] = need - acc;
-            S.pref[t] = (unsigned long long)d << 53;
-            S.bincnt[t] = S.hist[t][d];
-            S.shift[t] = 43;
-            S.mode[t] = S.bincnt[t] <= LEKF_REG_LIST ? 2 : 1;
+    for (int w = n; w > 0; w >>= 1) {
+        unsigned long long y[V];
+        int yp[V], mid[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            mid[v] = (lo[v] + hi[v]) >> 1;
+            const int m = mid[v] < n ? mid[v] : 0;
+            y[v] = key[m];
+            yp[v] = pos[m];
         }
-    }
-    __syncthreads();
-    // refine while a boundary bin is too large to sort
-    for (int pass = 0; pass < 8; ++pass) {
-        if (tid == 0) {
-            int a = 0;
-            for (int t = 0; t < nb; ++t) a |= (S.mode[t] == 1);
-            S.any = a;
-        }
-        __syncthreads();
-        if (!S.any) break;
-        for (int t = 0; t < nb; ++t)
-            if (S.mode[t] == 1)
-                for (int u = tid; u < 1024; u += kSelT) S.hist[t][u] = 0;
-        __syncthreads();
-        for (int i = tid; i < nacc; i += kSelT) {
-            const unsigned long long bits = skey[i];
-            const int t = sob[i] >> 30;
-            if (S.mode[t] == 1 && prefix_match(bits, S.pref[t], S.shift[t]))
-                atomicAdd(&S.hist[t][digit(bits, S.shift[t])], 1);
-        }
-        __syncthreads();
-        if (tid < nb && S.mode[tid] == 1) {
-            const int t = tid;
-            const int sh = S.shift[t];
-            int need = S.need[t], d = 0, acc = 0;
-            for (; d < 1024; ++d) {
-                if (acc + S.hist[t][d] >= need) break;
-                acc += S.hist[t][d];
-            }
-            S.need[t] = need - acc;
-            S.pref[t] |= (unsigned long long)d << sh;
-            S.bincnt[t] = S.hist[t][d];
-            const int ns = next_shift(sh);
-            S.shift[t] = ns;
-            if (S.bincnt[t] <= LEKF_REG_LIST)
-                S.mode[t] = 2;
-            else if (ns < 0)
-                S.mode[t] = 3;  // every key bit fixed: equal keys, break ties by position
-        }
-        __syncthreads();
-    }
-    // collect the boundary bins; resolve exact ties in list (scan) order
-    for (int t = tid; t < LEKF_REG_NB; t += kSelT) S.run[t] = 0;
-    __syncthreads();
-    for (int i0 = 0; i0 < nacc; i0 += kSelT) {
-        const int i = i0 + tid;
-        bool in = i < nacc;
-        const unsigned long long bits = in ? skey[i] : 0ull;
-        const int t = in ? (sob[i] >> 30) : 0;
-        if (in && S.mode[t] == 2 && prefix_match(bits, S.pref[t], S.shift[t])) {
-            const int pos = atomicAdd(&S.lcnt[t], 1);
-            if (pos < LEKF_REG_LIST) {
-                S.lbits[t][pos] = bits;
-                S.lpos[t][pos] = i;
-            }
-        }
-        for (int u = 0; u < nb; ++u) {
-            if (S.mode[u] != 3) continue;
-            const bool tie = in && t == u && bits == S.pref[u];
-            const int rank = block_append(S, tie, u, tid);
-            if (tie && rank == S.need[u] - 1) {
-                S.thr_bits[u] = bits;
-                S.thr_pos[u] = i;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (lo[v] < hi[v]) {
+                if (y[v] < x[v] || (y[v] == x[v] && yp[v] < xp[v]))
+                    lo[v] = mid[v] + 1;
+                else
+                    hi[v] = mid[v];
             }
         }
     }
-    __syncthreads();
-    // sort each collected bin by (bits, position); the need-th is the threshold
-    for (int t = 0; t < nb; ++t) {
-        if (S.mode[t] != 2) continue;
-        const int n = S.lcnt[t] < LEKF_REG_LIST ? S.lcnt[t] : LEKF_REG_LIST;
-        for (int ph = 0; ph < n; ++ph) {
-            const int i = 2 * tid + (ph & 1);
-            if (i + 1 < n) {
-                const unsigned long long x = S.lbits[t][i], y = S.lbits[t][i + 1];
-                const int px = S.lpos[t][i], py = S.lpos[t][i + 1];
-                if (x > y || (x == y && px > py)) {
-                    S.lbits[t][i] = y;
-                    S.lbits[t][i + 1] = x;
-                    S.lpos[t][i] = py;
-                    S.lpos[t][i + 1] = px;
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = lo[v];
+}
+
+// The reference's cap (scale/letkf/letkf_tools.f90:1041-1156) over one
+// bucket's candidates 0..n-1 in scan order, keyat(i) the priority key of
+// candidate i (smaller first).  Returns the kept count; the kept entries are
+// S[buf][0..kept) with spos = candidate and sslot = storage slot (the output
+// position).  S lives in LDS when kLds (cap <= LEKF_CAP_LDS), else in G.
+// Uniform call by all kSelT threads.
+template <bool kLds, class KeyAt>
+__device__ int cap_slots(CapLds& L, const CapGlobal& G, KeyAt keyat, int n, int cap, int tid,
+                         int& bufo)
+{
+    constexpr int U = kCapU;
+    constexpr int V = LEKF_CAP_LDS / kSelT;  // S entries per thread per round of the merge
+    int s = 0, buf = 0, par = 0;
+    unsigned long long xn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = u * kSelT + tid;
+        xn[u] = i < n ? keyat(i) : 0ull;
+    }
+    for (int b0 = 0; b0 < n; b0 += kCapB) {
+        unsigned long long* Sk = kLds ? L.skey[buf] : G.key + (size_t)buf * cap;
+        int* Sp = kLds ? L.spos[buf] : G.pos + (size_t)buf * cap;
+        int* Ss = kLds ? L.sslot[buf] : G.slot + (size_t)buf * cap;
+        unsigned long long* Tk = kLds ? L.skey[buf ^ 1] : G.key + (size_t)(buf ^ 1) * cap;
+        int* Tp = kLds ? L.spos[buf ^ 1] : G.pos + (size_t)(buf ^ 1) * cap;
+        int* Ts = kLds ? L.sslot[buf ^ 1] : G.slot + (size_t)(buf ^ 1) * cap;
+        unsigned long long x[U];
+        bool pot[U];
+        const unsigned long long tau = s == cap ? Sk[cap - 1] : 0ull;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x[u] = xn[u];
+            const int i = b0 + u * kSelT + tid;
+            // into a full set only below its last entry (a later candidate
+            // with an equal key comes after it)
+            pot[u] = i < n && (s < cap || x[u] < tau);
+            const int inext = i + kCapB;  // the next block's keys, loaded ahead
+            xn[u] = inext < n ? keyat(inext) : 0ull;
+        }
+        int pr[U], np;
+        block_rank<U>(L.wc, par, pot, tid, pr, np);
+        if (np == 0) continue;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (pot[u]) {
+                L.pkey[pr[u]] = x[u];
+                L.ppos[pr[u]] = b0 + u * kSelT + tid;
+            }
+        __syncthreads();
+        // potential q = u * kSelT + tid of this thread: its entries of S
+        // before it (c1) and the earlier potentials not above it (c2)
+        unsigned long long y[U];
+        int yp[U], c1[U], c2[U];
+        int jmax = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = u * kSelT + tid;
+            y[u] = q < np ? L.pkey[q] : ~0ull;
+            yp[u] = q < np ? L.ppos[q] : 0;
+            c2[u] = 0;
+            if (q < np) jmax = q;
+        }
+        count_before<U>(Sk, Sp, s, y, yp, c1);
+        {
+            int j = 0;
+            for (; j + 4 <= jmax; j += 4) {
+                const unsigned long long k0 = L.pkey[j], k1 = L.pkey[j + 1];
+                const unsigned long long k2 = L.pkey[j + 2], k3 = L.pkey[j + 3];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int q = u * kSelT + tid;
+                    c2[u] += (j < q && k0 <= y[u]) + (j + 1 < q && k1 <= y[u]) +
+                             (j + 2 < q && k2 <= y[u]) + (j + 3 < q && k3 <= y[u]);
                 }
             }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            S.thr_bits[t] = S.lbits[t][S.need[t] - 1];
-            S.thr_pos[t] = S.lpos[t][S.need[t] - 1];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        int acc = 0;
-        for (int t = 0; t < nb; ++t) {
-            S.base[t] = acc;
-            acc += S.cnt[t] < cap ? S.cnt[t] : cap;
-            S.run[t] = 0;
-        }
-        S.base[nb] = acc;
-    }
-    __syncthreads();
-    // emit bucket by bucket in scan order (list position i = i0 + u * kSelT + tid)
-    {
-        const int lane = tid & 63, wv = tid >> 6;
-        int runb[LEKF_REG_NB];
+            for (; j < jmax; ++j) {
+                const unsigned long long k0 = L.pkey[j];
 #pragma unroll
-        for (int t = 0; t < LEKF_REG_NB; ++t) runb[t] = 0;
-        for (int i0 = 0; i0 < nacc; i0 += kSelT * kSelU) {
-            unsigned long long bits[kSelU];
-            int ob[kSelU];
-            bool keep[kSelU];
-#pragma unroll
-            for (int u = 0; u < kSelU; ++u) {
-                const int i = i0 + u * kSelT + tid;
-                const bool in = i < nacc;
-                bits[u] = in ? skey[i] : 0ull;
-                ob[u] = in ? sob[i] : 0;
-                const int t = ob[u] >> 30;
-                keep[u] = in && (S.mode[t] == 0 || bits[u] < S.thr_bits[t] ||
-                                 (bits[u] == S.thr_bits[t] && i <= S.thr_pos[t]));
+                for (int u = 0; u < U; ++u) c2[u] += (j < u * kSelT + tid && k0 <= y[u]);
             }
-            unsigned long long m[kSelU][LEKF_REG_NB];
+        }
+        bool ent[U];
 #pragma unroll
-            for (int u = 0; u < kSelU; ++u)
+        for (int u = 0; u < U; ++u) ent[u] = u * kSelT + tid < np && c1[u] + c2[u] < cap;
+        int er[U], e;
+        block_rank<U>(L.wc, par, ent, tid, er, e);
+        if (e == 0) continue;
 #pragma unroll
-                for (int t = 0; t < LEKF_REG_NB; ++t) m[u][t] = __ballot(keep[u] && (ob[u] >> 30) == t);
-            if (lane == 0)
+        for (int u = 0; u < U; ++u)
+            if (ent[u]) {
+                L.nkey[er[u]] = y[u];
+                L.npos[er[u]] = yp[u];
+                L.nc1[er[u]] = c1[u];
+            }
+        __syncthreads();
+        // entrant t = u * kSelT + tid: its rank among the entrants
+        {
+            unsigned long long xk[U];
+            int r[U];
+            int tmax = -1;
 #pragma unroll
-                for (int u = 0; u < kSelU; ++u)
+            for (int u = 0; u < U; ++u) {
+                const int t = u * kSelT + tid;
+                xk[u] = t < e ? L.nkey[t] : 0ull;
+                r[u] = 0;
+                if (t < e) tmax = t;
+            }
+            if (tmax >= 0) {
+                int w = 0;
+                for (; w + 4 <= e; w += 4) {
+                    const unsigned long long z0 = L.nkey[w], z1 = L.nkey[w + 1];
+                    const unsigned long long z2 = L.nkey[w + 2], z3 = L.nkey[w + 3];
 #pragma unroll
-                    for (int t = 0; t < LEKF_REG_NB; ++t) S.ecnt[wv][u][t] = __popcll(m[u][t]);
-            __syncthreads();
-            int pos[kSelU];
-#pragma unroll
-            for (int u = 0; u < kSelU; ++u) pos[u] = -1;
-#pragma unroll
-            for (int u = 0; u < kSelU; ++u)
-#pragma unroll
-                for (int t = 0; t < LEKF_REG_NB; ++t) {
-                    int before = 0, tot = 0;
-                    for (int w = 0; w < kSelW; ++w) {
-                        const int cw = S.ecnt[w][u][t];
-                        before += w < wv ? cw : 0;
-                        tot += cw;
+                    for (int u = 0; u < U; ++u) {
+                        const int t = u * kSelT + tid;
+                        r[u] += (z0 < xk[u] || (z0 == xk[u] && w < t)) +
+                                (z1 < xk[u] || (z1 == xk[u] && w + 1 < t)) +
+                                (z2 < xk[u] || (z2 == xk[u] && w + 2 < t)) +
+                                (z3 < xk[u] || (z3 == xk[u] && w + 3 < t));
                     }
-                    if (keep[u] && (ob[u] >> 30) == t)
-                        pos[u] = runb[t] + before + __popcll(m[u][t] & lane_lt(lane));
-                    runb[t] += tot;
                 }
-            __syncthreads();
+                for (; w < e; ++w) {
+                    const unsigned long long z0 = L.nkey[w];
 #pragma unroll
-            for (int u = 0; u < kSelU; ++u) {
-                const int t = ob[u] >> 30;
-                if (keep[u] && pos[u] < S.base[t + 1] - S.base[t]) {
-                    const long w = S.base[t] + pos[u];
-                    out_idx[w] = ob[u] & 0x3fffffff;
-                    out_rho[w] = srho[i0 + u * kSelT + tid];
+                    for (int u = 0; u < U; ++u)
+                        r[u] += (z0 < xk[u] || (z0 == xk[u] && w < u * kSelT + tid));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = u * kSelT + tid;
+                if (t < e) {
+                    L.ekey[r[u]] = xk[u];
+                    L.epos[r[u]] = L.npos[t];
+                    L.nU[t] = L.nc1[t] + r[u];
                 }
             }
         }
+        __syncthreads();
+        const int snew = s + e < cap ? s + e : cap;
+        const int nev = s + e - snew;
+        // S entries move up by the entrants before them; the top nev leave
+        for (int r0 = 0; r0 < s; r0 += V * kSelT) {
+            unsigned long long xk[V];
+            int xp[V], U2[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int r = r0 + v * kSelT + tid;
+                xk[v] = r < s ? Sk[r] : ~0ull;
+                xp[v] = r < s ? Sp[r] : 0x7fffffff;
+            }
+            count_before<V>(L.ekey, L.epos, e, xk, xp, U2);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int r = r0 + v * kSelT + tid;
+                if (r < s) {
+                    const int Ur = r + U2[v];
+                    if (Ur < cap) {
+                        Tk[Ur] = xk[v];
+                        Tp[Ur] = xp[v];
+                        Ts[Ur] = Ss[r];
+                    } else {
+                        L.ev[Ur - cap] = Ss[r];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = u * kSelT + tid;
+            if (t < e) {
+                const int Ut = L.nU[t];
+                if (Ut < cap) {
+                    Tk[Ut] = L.nkey[t];
+                    Tp[Ut] = L.npos[t];
+                } else {
+                    L.ev[Ut - cap] = -(t + 1);
+                }
+            }
+        }
+        __syncthreads();
+        // slots: the next free one while the set fills, then the slot of the
+        // evicted entry, evictions taken from the largest down
+        const int nfill = cap - s;
+        bool chain = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = u * kSelT + tid;
+            if (t < e) {
+                int sl = -1, ln = -1;
+                if (t < nfill) {
+                    sl = s + t;
+                } else {
+                    const int v = L.ev[nev - 1 - (t - nfill)];
+                    if (v >= 0)
+                        sl = v;
+                    else
+                        ln = -v - 1;
+                }
+                L.slotn[t] = sl;
+                L.link[t] = ln;
+                chain |= sl < 0;
+            }
+        }
+        // an entrant evicted within its block passes its slot on to the
+        // entrant that evicted it: resolve the chains by pointer jumping (a
+        // link points to an earlier entrant, so a chain of at most kCapB
+        // entrants resolves within log2(kCapB) + 1 rounds; the bound only
+        // guards the loop)
+        if (__syncthreads_or(chain)) {
+            for (int round = 0; round < 16; ++round) {
+                bool un = false;
+                int sl[U], ln[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = u * kSelT + tid;
+                    sl[u] = -1;
+                    ln[u] = -1;
+                    if (t < e && L.slotn[t] < 0) {
+                        un = true;
+                        const int l = L.link[t];
+                        sl[u] = L.slotn[l];
+                        ln[u] = L.link[l];
+                    }
+                }
+                if (!__syncthreads_or(un)) break;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = u * kSelT + tid;
+                    if (t < e && L.slotn[t] < 0) {
+                        if (sl[u] >= 0)
+                            L.slotn[t] = sl[u];
+                        else
+                            L.link[t] = ln[u];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = u * kSelT + tid;
+            if (t < e && L.nU[t] < cap) Ts[L.nU[t]] = L.slotn[t];
+        }
+        __syncthreads();
+        s = snew;
+        buf ^= 1;
     }
-    const int kept = S.base[nb];
+    bufo = buf;
+    return s;
+}
+
+// Cap one candidate stream and write the kept entries in slot order:
+// emit(slot, candidate).  Uniform call; ends with a barrier.
+template <class KeyAt, class Emit>
+__device__ int cap_stream(SelLds& S, const CapGlobal& G, KeyAt keyat, Emit emit, int n, int cap,
+                          int tid)
+{
+    int buf = 0, kept;
+    if (cap <= LEKF_CAP_LDS) {
+        kept = cap_slots<true>(S.cap, G, keyat, n, cap, tid, buf);
+        for (int r = tid; r < kept; r += kSelT) {
+            const int sl = S.cap.sslot[buf][r];
+            if ((unsigned)sl < (unsigned)kept) emit(sl, S.cap.spos[buf][r]);
+        }
+    } else {
+        kept = cap_slots<false>(S.cap, G, keyat, n, cap, tid, buf);
+        for (int r = tid; r < kept; r += kSelT) {
+            const int sl = G.slot[(size_t)buf * cap + r];
+            if ((unsigned)sl < (unsigned)kept) emit(sl, G.pos[(size_t)buf * cap + r]);
+        }
+    }
     __syncthreads();
     return kept;
 }
 
 // Persistent selection: workgroup g handles points g, g + gridDim.x, ...
 // Each candidate of a point is evaluated once.  Without a cap the accepted
-// candidates are written in scan order.  With a cap they go to the
-// workgroup's scratch slot in scan order (rdiag bits, obs | bucket << 30,
-// rho); each over-full bucket's threshold pair (key, list position) is
-// found by digit histograms over the list, and the kept entries are written
-// bucket by bucket in scan order.
+// candidates are written in scan order.  With a cap each bucket's accepted
+// candidates go, in scan order, to the bucket's stream in the workgroup's
+// scratch (rdiag bits, obs, rho); a stream within the cap is written as it
+// is, a longer one through cap_stream in slot order, bucket after bucket.
 __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
 {
     __shared__ SelLds S;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int nij = A.nx * A.ny;
     const int nb = A.nbkt;
     const int cap = A.nobs_max;
-    unsigned long long* skey = A.s_key + (size_t)blockIdx.x * A.slot_cap;
-    int* sob = A.s_ob + (size_t)blockIdx.x * A.slot_cap;
-    double* srho = A.s_rho + (size_t)blockIdx.x * A.slot_cap;
+    const size_t sbase = (size_t)blockIdx.x * (size_t)nb * (size_t)A.slot_cap;
+    unsigned long long* skey = A.s_key + sbase;
+    int* sob = A.s_ob + sbase;
+    double* srho = A.s_rho + sbase;
+    CapGlobal G;
+    G.key = A.g_key ? A.g_key + (size_t)blockIdx.x * 2 * cap : nullptr;
+    G.pos = A.g_pos ? A.g_pos + (size_t)blockIdx.x * 2 * cap : nullptr;
+    G.slot = A.g_slot ? A.g_slot + (size_t)blockIdx.x * 2 * cap : nullptr;
 
     for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
         const long b = A.pt_begin + pi;  // global point = lev * nij + ij
@@ -419,88 +544,109 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
         const int ncand = S.seg_pre[nseg];
         const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
 
-        // evaluate every candidate once: kSelU per thread per round, all loads
-        // of a round issued first; scan position q = q0 + u * kSelT + tid
+        // evaluate every candidate once; scan position q = q0 + tid
         int nacc = 0;
-        {
-            const int lane = tid & 63, wv = tid >> 6;
-            int seg = 0;
-            for (int q0 = 0; q0 < ncand; q0 += kSelT * kSelU) {
-                int o[kSelU];
-                ObsRec r[kSelU];
+        int bcnt[LEKF_REG_NB];
 #pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
-                    const int q = q0 + u * kSelT + tid;
-                    o[u] = q < ncand ? cand_obs(S, nseg, q, seg) : -1;
-                    if (o[u] >= 0) r[u] = load_obs(A, o[u]);
+        for (int t = 0; t < LEKF_REG_NB; ++t) bcnt[t] = 0;
+        int seg = 0;
+        for (int q0 = 0; q0 < ncand; q0 += kSelT) {
+            const int q = q0 + tid;
+            const int o = q < ncand ? cand_obs(S, nseg, q, seg) : -1;
+            Cand c;
+            c.ok = false;
+            c.b = 0;
+            if (o >= 0) c = eval_rec(A, load_obs(A, o), o, ri, rj, rz);
+            unsigned long long m[LEKF_REG_NB];
+#pragma unroll
+            for (int t = 0; t < LEKF_REG_NB; ++t)
+                m[t] = __ballot(c.ok && (cap <= 0 || c.b == t));
+            if (lane == 0)
+#pragma unroll
+                for (int t = 0; t < LEKF_REG_NB; ++t) S.wcnt[wv][t] = __popcll(m[t]);
+            __syncthreads();
+            int pos = -1;
+            if (cap <= 0) {
+                int before = 0, tot = 0;
+                for (int w = 0; w < kSelW; ++w) {
+                    const int cw = S.wcnt[w][0];
+                    before += w < wv ? cw : 0;
+                    tot += cw;
                 }
-                Cand c[kSelU];
-                unsigned long long m[kSelU];
+                pos = nacc + before + __popcll(m[0] & lane_lt(lane));
+                nacc += tot;
+            } else {
 #pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
-                    c[u].ok = false;
-                    c[u].b = 0;
-                    if (o[u] >= 0) c[u] = eval_rec(A, r[u], o[u], ri, rj, rz);
-                    m[u] = __ballot(c[u].ok);
-                }
-                if (lane == 0)
-#pragma unroll
-                    for (int u = 0; u < kSelU; ++u) S.wcnt[wv][u] = __popcll(m[u]);
-                __syncthreads();
-                int pos[kSelU];
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
+                for (int t = 0; t < LEKF_REG_NB; ++t) {
                     int before = 0, tot = 0;
                     for (int w = 0; w < kSelW; ++w) {
-                        const int cw = S.wcnt[w][u];
+                        const int cw = S.wcnt[w][t];
                         before += w < wv ? cw : 0;
                         tot += cw;
                     }
-                    pos[u] = nacc + before + __popcll(m[u] & lane_lt(lane));
-                    nacc += tot;
+                    if (c.ok && c.b == t) pos = bcnt[t] + before + __popcll(m[t] & lane_lt(lane));
+                    bcnt[t] += tot;
                 }
-                __syncthreads();
-#pragma unroll
-                for (int u = 0; u < kSelU; ++u) {
-                    if (!c[u].ok) continue;
-                    if (cap <= 0) {
-                        A.loc_idx[obase + pos[u]] = c[u].o;
-                        A.loc_rho[obase + pos[u]] = c[u].rho;
-                    } else {
-                        skey[pos[u]] = c[u].bits;
-                        sob[pos[u]] = c[u].o | (c[u].b << 30);
-                        srho[pos[u]] = c[u].rho;
-                    }
+            }
+            __syncthreads();
+            if (c.ok) {
+                if (cap <= 0) {
+                    A.loc_idx[obase + pos] = c.o;
+                    A.loc_rho[obase + pos] = c.rho;
+                } else {
+                    const size_t w = (size_t)c.b * (size_t)A.slot_cap + (size_t)pos;
+                    skey[w] = c.bits;
+                    sob[w] = c.o;
+                    srho[w] = c.rho;
                 }
             }
         }
         __syncthreads();
-        if (cap <= 0) {
-            if (tid == 0) {
-                A.loc_off[b] = obase;
-                A.loc_cnt[b] = nacc;
+        if (cap > 0) {
+            nacc = 0;
+            for (int t = 0; t < nb; ++t) {
+                const int n = bcnt[t];
+                const size_t so = (size_t)t * (size_t)A.slot_cap;
+                int* oi = A.loc_idx + obase + nacc;
+                double* orh = A.loc_rho + obase + nacc;
+                if (n <= cap) {
+                    for (int i = tid; i < n; i += kSelT) {
+                        oi[i] = sob[so + i];
+                        orh[i] = srho[so + i];
+                    }
+                    nacc += n;
+                } else {
+                    nacc += cap_stream(
+                        S, G, [&](int i) { return skey[so + i]; },
+                        [&](int slot, int i) {
+                            oi[slot] = sob[so + i];
+                            orh[slot] = srho[so + i];
+                        },
+                        n, cap, tid);
+                }
             }
-            __syncthreads();
-            continue;
         }
-        const int kept = cap_select(S, tid, skey, sob, srho, nacc, nb, cap, A.loc_idx + obase,
-                                    A.loc_rho + obase);
         if (tid == 0) {
             A.loc_off[b] = obase;
-            A.loc_cnt[b] = kept;
+            A.loc_cnt[b] = nacc;
         }
         __syncthreads();
     }
 }
 
-// Nearest-first cap of per-point lists (persistent, one point per workgroup).
+// Nearest-first cap of the global grid's per-point lists (persistent, one
+// point per workgroup): a list longer than the cap keeps the cap entries of
+// largest rho (ties: the earlier list entry) in the reference's storage-slot
+// order, by the cap of scale/letkf/letkf_tools.f90:1041-1156 with the key
+// 2^63 - 1 - bits(rho) (rho > 0; a smaller key is a larger rho).
 __global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
 {
     __shared__ SelLds S;
     const int tid = threadIdx.x;
-    unsigned long long* skey = A.s_key + (size_t)blockIdx.x * A.slot_cap;
-    int* sob = A.s_ob + (size_t)blockIdx.x * A.slot_cap;
-    double* srho = A.s_rho + (size_t)blockIdx.x * A.slot_cap;
+    CapGlobal G;
+    G.key = A.g_key ? A.g_key + (size_t)blockIdx.x * 2 * A.cap : nullptr;
+    G.pos = A.g_pos ? A.g_pos + (size_t)blockIdx.x * 2 * A.cap : nullptr;
+    G.slot = A.g_slot ? A.g_slot + (size_t)blockIdx.x * 2 * A.cap : nullptr;
     const long npt = (long)A.ncols * A.nlev;
     for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
         const long b = (pi / A.ncols) * (long)A.nij + A.col_begin + pi % A.ncols;
@@ -514,15 +660,18 @@ __global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
                 A.out_rho[ob + i] = A.in_rho[off + i];
             }
         } else {
-            for (int i = tid; i < n; i += kSelT) {
-                const int o = A.in_idx[off + i];
-                const double rho = A.in_rho[off + i];
-                skey[i] = (unsigned long long)__double_as_longlong(rn_div(A.rdiag[o], rho));
-                sob[i] = o;
-                srho[i] = rho;
-            }
-            __syncthreads();
-            kept = cap_select(S, tid, skey, sob, srho, n, 1, A.cap, A.out_idx + ob, A.out_rho + ob);
+            const int* ii = A.in_idx + off;
+            const double* ir = A.in_rho + off;
+            kept = cap_stream(
+                S, G,
+                [&](int i) {
+                    return 0x7fffffffffffffffull - (unsigned long long)__double_as_longlong(ir[i]);
+                },
+                [&](int slot, int i) {
+                    A.out_idx[ob + slot] = ii[i];
+                    A.out_rho[ob + slot] = ir[i];
+                },
+                n, A.cap, tid);
         }
         if (tid == 0) {
             A.out_off[b] = ob;

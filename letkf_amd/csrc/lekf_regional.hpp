@@ -5,8 +5,8 @@
 #include "lekf_types.h"
 
 #define LEKF_REG_NB 4      // buckets (the reference's (element, obs type) pairs)
-#define LEKF_REG_LIST 256  // boundary-bin candidates sorted exactly per bucket
 #define LEKF_REG_SEG 512   // rectangle rows per point
+#define LEKF_CAP_LDS 1024  // caps up to this size keep their kept set in LDS
 
 struct RegObsArgs {
     int nobs, ne, nx, ny;
@@ -39,17 +39,24 @@ struct RegionalArgs {
     int* loc_cnt;
     int* loc_idx;
     double* loc_rho;
-    // per-workgroup scratch of the capped selection: the point's accepted
-    // candidates in scan order (key = rdiag bits, obs | bucket << 30, rho)
+    // per-workgroup scratch of the capped selection: one stream per bucket
+    // (slot_cap entries each) of the point's accepted candidates in scan
+    // order (key = rdiag bits, obs, rho)
     unsigned long long* s_key;
     int* s_ob;
     double* s_rho;
-    long slot_cap;                          // entries per workgroup slot
+    long slot_cap;                          // entries per bucket stream
+    // caps above LEKF_CAP_LDS: the kept set, 2 * nobs_max entries per
+    // workgroup slot (null otherwise)
+    unsigned long long* g_key;
+    int* g_pos;
+    int* g_slot;
 };
 
 // Nearest-first cap of per-point local lists (the global grid's nobs_max):
-// keeps the cap entries with the smallest (rdiag[o]/rho, list position) of
-// each input list and writes them in list order at out_*[b * cap].
+// keeps the cap entries of largest rho (ties: the earlier list position) of
+// each input list, in the reference cap's storage-slot order, at
+// out_*[b * cap].
 struct ListCapArgs {
     int nij, nlev;                          // points b = lev * nij + ij
     int col_begin, ncols;                   // columns [col_begin, col_begin + ncols), all levels
@@ -58,15 +65,15 @@ struct ListCapArgs {
     const int* in_cnt;
     const int* in_idx;
     const double* in_rho;
-    const double* rdiag;                    // per sorted obs (err^2)
     long* out_off;
     int* out_cnt;
     int* out_idx;
     double* out_rho;
-    unsigned long long* s_key;              // per-workgroup scratch, slot_cap entries each
-    int* s_ob;
-    double* s_rho;
-    long slot_cap;                          // >= the longest input list
+    // caps above LEKF_CAP_LDS: the kept set, 2 * cap entries per workgroup
+    // slot (null otherwise)
+    unsigned long long* g_key;
+    int* g_pos;
+    int* g_slot;
 };
 
 hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st);

@@ -1784,7 +1784,9 @@ __device__ void big_tridiag_sym(double* A, int n, int lda, double* d, double* e,
     __syncthreads();
 }
 
-constexpr int kQlMaxIter = 60;
+// sweeps allowed per eigenvalue before the solve gives up, as tql2's 30
+// (common/netlib.f:718-887; the reference then STOPs, common_mtx.f90:61-64)
+constexpr int kQlMaxIter = 30;
 
 // ql_apply for Z in global memory with Z[r + i * lda] = Z_ri (rows of a
 // column contiguous), NR rows per lane
@@ -1912,114 +1914,23 @@ __device__ int eig_ql(EigLds& L, int n, int lda, int lane)
     return 0;
 }
 
-// The same Householder + implicit-QL eigendecomposition with the matrices in
-// registers, for n <= KR <= 64.  Lane r holds column r of A (a[t] = A[t][r])
-// and row r of Q, later of Z (q[t] = Z[r][t]); d_i and e_i live in lane i
-// and are read by the others with readlane.  Loops over matrix indices run
-// over the static index t with uniform branches, so every register index is
-// a compile-time constant.
-//   Tridiagonalisation, step j: lane j's column gives x = A[j+1.., j] and is
-//   overwritten by v; p = tau A v and y = tau Q v are lane-local dot products
-//   with the broadcast v_t; A -= v w^T + w v^T and Q -= y v^T (Q accumulates
-//   H_0 H_1 ... forward, row by row).
-//   QL: a sweep runs the scalar recurrence in every lane and rotates columns
-//   (i, i+1) of the lane's Z row as it goes.
-// Reads C (+ the diagonal shift) from Z in LDS, writes the eigenvectors back to
-// Z (columns) and the eigenvalues to d.  Returns nonzero if QL did not
+// Householder + implicit-QL eigendecomposition for n <= KR <= 64: the
+// tridiagonalisation in LDS (eig_tridiag), then QL with Q in registers: lane
+// r holds row r of Q, later of Z (q[t] = Z[r][t]); d_i and e_i live in lane
+// i and are read by the others with readlane; a sweep runs the scalar
+// recurrence in every lane and rotates columns (i, i+1) of the lane's row as
+// it goes (loops over matrix indices run over the static index t with
+// uniform branches, so every register index is a compile-time constant).
+// Reads C (+ the diagonal shift) from Z in LDS, writes the eigenvectors back
+// to Z (columns) and the eigenvalues to d.  Returns nonzero if QL did not
 // converge.
+// Implicit QL with shifts on the tridiagonal (d_i, e_i in lane i), rotating
+// the rows q[] of Q as each rotation is formed.  Returns nonzero if an
+// eigenvalue took more than kQlMaxIter sweeps.
 template <int KR>
-__device__ int eig_reg(EigLds& L, int n, int lda, int lane)
+__device__ __forceinline__ int ql_reg(double (&q)[KR], int n, int lane, double& dreg,
+                                      double& ereg)
 {
-    double a[KR], q[KR];
-#pragma unroll
-    for (int t = 0; t < KR; ++t) {
-        a[t] = (t < n && lane < n) ? L.Z[t * lda + lane] : 0.0;
-        q[t] = (t == lane) ? 1.0 : 0.0;
-    }
-    double dreg = 0.0, ereg = 0.0;
-    for (int j = 0; j + 2 < n; ++j) {
-        // alpha = A[j+1][j], sigma = sum_{t > j+1} A[t][j]^2 (from lane j)
-        double al = 0.0, sg = 0.0, dj = 0.0;
-#pragma unroll
-        for (int t = 0; t < KR; ++t) {
-            if (t == j) dj = a[t];
-            if (t == j + 1) al = a[t];
-            if (t > j + 1 && t < n) sg = fma(a[t], a[t], sg);
-        }
-        const double alpha = lane_bcast(al, j), sigma = lane_bcast(sg, j);
-        double tau = 0.0, beta = alpha, scal = 0.0;
-        if (sigma > 0.0) {
-            const double nrm = sqrt(fma(alpha, alpha, sigma));
-            beta = alpha <= 0.0 ? nrm : -nrm;
-            tau = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
-        }
-        if (lane == j) {
-            dreg = dj;
-            ereg = beta;
-        }
-        if (tau == 0.0) continue;
-        // lane j: column j becomes v (v_{j+1} = 1)
-        if (lane == j) {
-#pragma unroll
-            for (int t = 0; t < KR; ++t) {
-                if (t == j + 1) a[t] = 1.0;
-                if (t > j + 1 && t < n) a[t] *= scal;
-            }
-        }
-        // p = tau A v (lane = column), y = tau Q v (lane = row), own v_r
-        double pr = 0.0, yr = 0.0, vr = 0.0;
-#pragma unroll
-        for (int t = 0; t < KR; ++t) {
-            if (t > j && t < n) {
-                const double vt = lane_bcast(a[t], j);
-                pr = fma(a[t], vt, pr);
-                yr = fma(q[t], vt, yr);
-                if (t == lane) vr = vt;
-            }
-        }
-        pr *= tau;
-        yr *= tau;
-        const bool act = lane > j && lane < n;
-        const double K = 0.5 * tau * wave_sum(act ? vr * pr : 0.0);
-        const double wr = fma(-K, vr, pr);
-        // A -= v w^T + w v^T on the trailing block (products rounded, then
-        // added: elements (t, r) and (r, t) stay equal); Q -= y v^T
-#pragma unroll
-        for (int t = 0; t < KR; ++t) {
-            if (t > j && t < n) {
-                const double vt = lane_bcast(a[t], j);
-                const double wt = lane_bcast(wr, t);
-                if (act) a[t] = rn_sub(a[t], rn_add(rn_mul(vt, wr), rn_mul(wt, vr)));
-                q[t] = fma(-yr, vt, q[t]);
-            }
-        }
-    }
-    // last 2 x 2 block
-    {
-        double d0 = 0.0, d1 = 0.0, e0 = 0.0;
-#pragma unroll
-        for (int t = 0; t < KR; ++t) {
-            if (t == n - 2) d0 = a[t];
-            if (t == n - 1) {
-                d1 = a[t];
-                e0 = a[t];
-            }
-        }
-        if (n >= 2) {
-            const double dn2 = lane_bcast(d0, n - 2), en2 = lane_bcast(e0, n - 2);
-            if (lane == n - 2) {
-                dreg = dn2;
-                ereg = en2;
-            }
-        }
-        const double dn1 = lane_bcast(d1, n - 1);
-        if (lane == n - 1) {
-            dreg = dn1;
-            ereg = 0.0;
-        }
-    }
-    // implicit QL
     double tst = 0.0;
     int bad = 0;
     for (int l = 0; l < n && !bad; ++l) {
@@ -2077,6 +1988,21 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
             }
         }
     }
+    return bad;
+}
+
+template <int KR>
+__device__ int eig_reg(EigLds& L, int n, int lda, int lane)
+{
+    // Householder in LDS (each step's work is the active block's), then Q's
+    // row r into lane r's registers for the QL sweeps
+    eig_tridiag(L, n, lda, lane);
+    double q[KR];
+#pragma unroll
+    for (int t = 0; t < KR; ++t) q[t] = (t < n && lane < n) ? L.Z[lane * lda + t] : 0.0;
+    double dreg = lane < n ? L.d[lane] : 0.0, ereg = lane < n ? L.e[lane] : 0.0;
+    __syncthreads();
+    const int bad = ql_reg<KR>(q, n, lane, dreg, ereg);
     // eigenvalues and eigenvectors (columns of Z) to LDS
     if (lane < n) {
         L.d[lane] = dreg;
@@ -2568,9 +2494,84 @@ __device__ void big_dual_point(const LekfJob& J, int b, double* rec, double* rbu
 // the optional trans/transm/pao outputs and the fused apply
 // Xa = x-mean + X' T evaluated as x-mean + (X' V) diag(sqrt((k-1)/L)) V^T
 // + (X' w-bar) 1^T.
-template <int KR, int BIG>
+// The split eigensolver (32 < k <= 64, three kernels per sub-batch): the
+// point's record carries, after the Gram, the Householder Q (transposed: Q_rt
+// at qoff + t k + r), the tridiagonal d and e, then the eigenvalues, the
+// rotation-stream length and the QL status written by the QL kernel.  The
+// rotation stream of a point holds, per QL sweep, a header (m, count) and the
+// sweep's (c, s) pairs for i = m-1 down to m-count.
+struct SplitRec {
+    long qoff, doff, eoff, lamoff, cntoff, flagoff;
+    const double* rot;  // this point's rotation stream
+    long rcap;          // its capacity in (c, s) entries
+};
+constexpr int kSplitOverflow = 8;  // flag: the stream overflowed (the point is solved again in full)
+
+__host__ __device__ inline long split_qoff(int kmax) { return gram_stride(kmax); }
+__host__ __device__ inline long split_stride(int kmax)
+{
+    return (gram_stride(kmax) + (long)kmax * kmax + 3L * kmax + 2 + 1) & ~1L;
+}
+__host__ __device__ inline long split_rcap(int kmax) { return 2L * kmax * kmax + 8L * kmax; }
+
+__device__ inline SplitRec split_rec(int kmax, const double* rot)
+{
+    SplitRec S;
+    S.qoff = split_qoff(kmax);
+    S.doff = S.qoff + (long)kmax * kmax;
+    S.eoff = S.doff + kmax;
+    S.lamoff = S.eoff + kmax;
+    S.cntoff = S.lamoff + kmax;
+    S.flagoff = S.cntoff + 1;
+    S.rot = rot;
+    S.rcap = split_rcap(kmax);
+    return S;
+}
+
+// Eigenvectors from the split solve: Q into Z in LDS (lane r: row r), the
+// recorded QL rotations replayed in order on each lane's row (the same
+// operations ql_reg performs as it goes, so the result is the same bit for
+// bit; a sweep costs its own length), then the eigenvalues to LDS.
+template <int KR>
+__device__ void eig_replay(EigLds& L, int n, int lda, int lane, const double* rec,
+                           const SplitRec& S)
+{
+    for (int t = 0; t < n; ++t)
+        if (lane < n) L.Z[lane * lda + t] = rec[S.qoff + (long)t * n + lane];
+    const long ntot = (long)rec[S.cntoff];
+    const double* rs = S.rot;
+    double* zr = L.Z + lane * lda;
+    long pos = 0;
+    while (pos < ntot) {
+        const int m = (int)rs[2 * pos], cnt = (int)rs[2 * pos + 1];
+        ++pos;
+        for (int j = lane; j < cnt; j += 64) {
+            L.rot[2 * j] = rs[2 * (pos + j)];
+            L.rot[2 * j + 1] = rs[2 * (pos + j) + 1];
+        }
+        __syncthreads();
+        if (lane < n && cnt > 0) {
+            // column i+1 takes its final value at step i; column i carries on
+            double h = zr[m];
+            for (int j = 0; j < cnt; ++j) {
+                const int i = m - 1 - j;
+                const double c = L.rot[2 * j], sn = L.rot[2 * j + 1];
+                const double qi = zr[i];
+                zr[i + 1] = fma(sn, qi, c * h);
+                h = fma(c, qi, -sn * h);
+            }
+            zr[m - cnt] = h;
+        }
+        __syncthreads();
+        pos += cnt;
+    }
+    if (lane < n) L.d[lane] = rec[S.lamoff + lane];
+    __syncthreads();
+}
+
+template <int KR, int BIG, bool REPLAY = false>
 __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstride, int nvmax,
-                          long rot_off, double* smem, int lane)
+                          long rot_off, double* smem, int lane, const SplitRec* SR = nullptr)
 {
     const int k = J.k, lda = BIG ? k : eig_lda(k);
     EigLds L = eig_carve(smem, k, nvmax, BIG ? rec : nullptr);
@@ -2598,7 +2599,9 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
 
     if (p > 0) {
         const double* g = rec;
-        if (!BIG)
+        // the split solve's result: rotations replayed unless the stream overflowed
+        const bool replay = REPLAY && !(((int)rec[SR->flagoff]) & kSplitOverflow);
+        if (!BIG && !replay)
             for (int t = lane; t < k * k; t += 64) {
                 const int i = t / k, j = t - i * k;
                 L.Z[i * lda + j] = g[t];
@@ -2608,14 +2611,21 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
         __syncthreads();
         // P2 = trace(C) / (k-1), summed in index order as the reference does
         double tr = 0.0;
-        for (int i = 0; i < k; ++i) tr += L.Z[i * lda + i];
+        if (replay)
+            for (int i = 0; i < k; ++i) tr += g[(size_t)i * k + i];
+        else
+            for (int i = 0; i < k; ++i) tr += L.Z[i * lda + i];
         const double p2 = tr / (double)(k - 1);
         if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
         const double rho = 1.0 / alpha;
         __syncthreads();
-        for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += (double)(k - 1) * rho;
+        if (!replay)
+            for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += (double)(k - 1) * rho;
         __syncthreads();
-        if constexpr (BIG > 0) {
+        if (replay) {
+            if constexpr (KR > 0) eig_replay<KR>(L, k, lda, lane, rec, *SR);
+            if (((int)rec[SR->flagoff]) & 1) status |= LEKF_PT_NOCONV;
+        } else if constexpr (BIG > 0) {
             big_tridiag<(BIG > 0 ? BIG : 1)>(L.Z, k, lda, L.d, L.e, L.tau, lane);
             if (eig_ql<BIG>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
         } else if constexpr (KR > 0) {
@@ -2770,6 +2780,156 @@ void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
                      0L, smem, threadIdx.x);
 }
 
+// --- split eigensolver, 32 < k <= 64 --------------------------------------
+// Kernel 1 (one wave per point): C + (k-1)/alpha I into LDS and the
+// Householder tridiagonalisation (eig_tridiag, as the fused kernel does);
+// writes Q (transposed), d and e to the point's record.
+__global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                       long gp0, double* __restrict__ ws,
+                                                       long gstride, int kmax)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int k = J.k, lane = threadIdx.x, lda = eig_lda(k);
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
+    if (p <= 0) return;
+    double* rec = ws + (gp - gp0) * gstride;
+    const SplitRec S = split_rec(kmax, nullptr);
+    EigLds L = eig_carve(smem, k, 0);
+    double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
+    if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
+    const double shift = (double)(k - 1) * (1.0 / alpha);
+    for (int t = lane; t < k * k; t += 64) {
+        const int i = t / k, j = t - i * k;
+        L.Z[i * lda + j] = rec[t];
+    }
+    __syncthreads();
+    for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += shift;
+    __syncthreads();
+    eig_tridiag(L, k, lda, lane);
+    if (lane < k) {
+        for (int t = 0; t < k; ++t) rec[S.qoff + (long)t * k + lane] = L.Z[lane * lda + t];
+        rec[S.doff + lane] = L.d[lane];
+        rec[S.eoff + lane] = L.e[lane];
+    }
+}
+
+// Kernel 2 (one lane per point, 64 points per wave): implicit QL on the
+// point's tridiagonal held in LDS (d_i at i*64 + lane, e_i after the KR rows
+// of d), the same recurrence as ql_reg; each rotation (c, s) goes to the
+// point's stream instead of rotating Q.  Writes the eigenvalues, the stream
+// length and the flags (1: an eigenvalue did not converge, kSplitOverflow:
+// the stream was too short) to the record.
+template <int KR>
+__global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                      long gp0, long nb, double* __restrict__ ws,
+                                                      long gstride, double* __restrict__ rot,
+                                                      int kmax)
+{
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x;
+    const long pt = (long)blockIdx.x * 64 + lane;
+    if (pt >= nb) return;  // no barrier below
+    const long gp = gp0 + pt;
+    const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int n = J.k;
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
+    if (p <= 0) return;
+    double* rec = ws + pt * gstride;
+    const long rcap = split_rcap(kmax);
+    double* rs = rot + pt * 2 * rcap;
+    const SplitRec S = split_rec(kmax, nullptr);
+    double* d = sm + lane;
+    double* e = sm + (long)KR * 64 + lane;
+    for (int i = 0; i < n; ++i) {
+        d[i * 64] = rec[S.doff + i];
+        e[i * 64] = rec[S.eoff + i];
+    }
+    long nr = 0;
+    int bad = 0;
+    double tst = 0.0;
+    for (int l = 0; l < n && !bad; ++l) {
+        tst = fmax(tst, fabs(d[l * 64]) + fabs(e[l * 64]));
+        for (int iter = 0;; ++iter) {
+            int m = l;
+            while (m < n - 1 && !(tst + fabs(e[m * 64]) == tst)) ++m;
+            if (m == l) break;
+            if (iter == kQlMaxIter) {
+                bad = 1;
+                break;
+            }
+            const double dl = d[l * 64], el = e[l * 64];
+            double g = (d[(l + 1) * 64] - dl) / (2.0 * el);
+            double r = fabs(g) > 1.0e100 ? fabs(g) : sqrt(fma(g, g, 1.0));
+            g = d[m * 64] - dl + el / (g + (g >= 0.0 ? r : -r));
+            double sn = 1.0, c = 1.0, pp = 0.0, di1 = d[m * 64];
+            const long hpos = nr++;
+            int cnt = 0;
+            bool split = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double ei = e[i * 64], di = d[i * 64];
+                const double f = sn * ei, bb = c * ei;
+                const double x = fma(f, f, g * g);
+                if (x == 0.0) {
+                    d[(i + 1) * 64] = di1 - pp;
+                    e[m * 64] = 0.0;
+                    split = true;
+                    break;
+                }
+                const double y = rsq_nr(x);
+                r = x * y;
+                sn = f * y;
+                c = g * y;
+                e[(i + 1) * 64] = r;
+                g = di1 - pp;
+                r = fma(di - g, sn, 2.0 * c * bb);
+                pp = sn * r;
+                d[(i + 1) * 64] = g + pp;
+                g = fma(c, r, -bb);
+                di1 = di;
+                if (nr < rcap) {
+                    rs[2 * nr] = c;
+                    rs[2 * nr + 1] = sn;
+                }
+                ++nr;
+                ++cnt;
+            }
+            if (!split) {
+                d[l * 64] -= pp;
+                e[l * 64] = g;
+                e[m * 64] = 0.0;
+            }
+            if (hpos < rcap) {
+                rs[2 * hpos] = (double)m;
+                rs[2 * hpos + 1] = (double)cnt;
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) rec[S.lamoff + i] = d[i * 64];
+    rec[S.cntoff] = (double)(nr < rcap ? nr : rcap);
+    rec[S.flagoff] = (double)(bad | (nr > rcap ? kSplitOverflow : 0));
+}
+
+// Kernel 3 (one wave per point): the eigenvectors by replaying the rotations
+// on Q, then the W assembly, inflation, outputs and the fused apply of
+// eig_point.  A point whose stream overflowed is solved again in full.
+template <int KR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(eig_waves_per_simd(KR))))
+void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
+                      double* __restrict__ ws, long gstride, int nvmax, int kmax,
+                      const double* __restrict__ rot)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    const SplitRec S = split_rec(kmax, rot + (gp - gp0) * 2 * split_rcap(kmax));
+    eig_point<KR, 0, true>(J, (int)(gp - J.pt_begin + J.pt0), ws + (gp - gp0) * gstride, gstride,
+                           nvmax, 0L, smem, threadIdx.x, &S);
+}
+
 // LEKF_KSPLIT < k: one persistent wave per workgroup slot strides over the
 // points; for each it forms the Gram blocks on MFMA into the slot's
 // workspace record, then runs the eigen/assembly/apply steps in place.  The
@@ -2829,15 +2989,35 @@ size_t lekf_solve_lds_bytes(int k, int nvmax)
     return eig_lds_doubles(k, nvmax) * sizeof(double);
 }
 
+// register eigensolver variant of an ensemble size: k rounded up to a multiple
+// of 4 for k <= 64, 0 (LDS eigensolver) above
+static int eig_kr(int kmax) { return kmax <= 64 ? (kmax + 3) & ~3 : 0; }
+
+static bool split_path(int kmax)
+{
+    return kmax > 32 && kmax <= LEKF_KSPLIT && g_lekf_tune.eig_split && eig_kr(kmax) > 0;
+}
+
+// bytes per point of a sub-batch: the record, plus the rotation stream on the
+// split path
+static size_t ws_rec_bytes(int kmax)
+{
+    if (kmax > LEKF_KSPLIT) return (size_t)big_stride(kmax) * sizeof(double);
+    if (split_path(kmax))
+        return (size_t)(split_stride(kmax) + 2 * split_rcap(kmax)) * sizeof(double);
+    return (size_t)gram_stride(kmax) * sizeof(double);
+}
+
 size_t lekf_solve_ws_bytes(long npt, int kmax)
 {
     if (kmax <= 32 || npt <= 0) return 0;
     const bool big = kmax > LEKF_KSPLIT;
-    const size_t rec = (size_t)(big ? big_stride(kmax) : gram_stride(kmax)) * sizeof(double);
+    const size_t rec = ws_rec_bytes(kmax);
     size_t per = g_lekf_tune.gram_ws_bytes / rec;
     if (per < 1) per = 1;
-    // large-k records are MBs each: keep at least 1024 points per launch
-    if (big && per < 1024) per = 1024;
+    // large-k records are MBs each, and the split path's QL kernel takes 64
+    // points per wave: keep at least 1024 points per launch
+    if ((big || split_path(kmax)) && per < 1024) per = 1024;
     if ((size_t)npt < per) per = (size_t)npt;
     return per * rec;
 }
@@ -2854,10 +3034,6 @@ static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int km
                        lds, st, djobs, njobs, npt, kmax, nvmax);
     return hipGetLastError();
 }
-
-// register eigensolver variant of an ensemble size: k rounded up to a multiple
-// of 4 for k <= 64, 0 (LDS eigensolver) above
-static int eig_kr(int kmax) { return kmax <= 64 ? (kmax + 3) & ~3 : 0; }
 
 static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const LekfJob* djobs,
                              int njobs, long g0, double* ws, long gs, int nvmax)
@@ -2883,13 +3059,46 @@ static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const
     return hipGetLastError();
 }
 
+static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
+                                   const LekfJob* djobs, int njobs, long g0, double* ws, long gs,
+                                   int nvmax, int kmax, double* rot)
+{
+    const unsigned nwq = (unsigned)((nb + 63) / 64);
+    const size_t qlds = 2 * (size_t)kr * 64 * sizeof(double);
+    const size_t tlds = eig_lds_doubles(kmax, 0) * sizeof(double);
+    switch (kr) {
+#define LEKF_SPLIT_CASE(K)                                                                       \
+    case K:                                                                                      \
+        hipLaunchKernelGGL(lekf_tri_kernel, dim3((unsigned)nb), dim3(LEKF_WAVE), tlds, st, djobs, \
+                           njobs, g0, ws, gs, kmax);                                            \
+        hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs, njobs, \
+                           g0, nb, ws, gs, rot, kmax);                                          \
+        hipLaunchKernelGGL(lekf_eig3_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,    \
+                           djobs, njobs, g0, ws, gs, nvmax, kmax, rot);                         \
+        break;
+        LEKF_SPLIT_CASE(36)
+        LEKF_SPLIT_CASE(40)
+        LEKF_SPLIT_CASE(44)
+        LEKF_SPLIT_CASE(48)
+        LEKF_SPLIT_CASE(52)
+        LEKF_SPLIT_CASE(56)
+        LEKF_SPLIT_CASE(60)
+        LEKF_SPLIT_CASE(64)
+#undef LEKF_SPLIT_CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int NT, int MODE>
 static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
                                void* ws, size_t ws_bytes, hipStream_t st)
 {
-    const long gs = gram_stride(kmax);
-    const long per = (long)(ws_bytes / ((size_t)gs * sizeof(double)));
+    const bool s3 = split_path(kmax);
+    const long gs = s3 ? split_stride(kmax) : gram_stride(kmax);
+    const long per = (long)(ws_bytes / ws_rec_bytes(kmax));
     if (per < 1 || !ws) return hipErrorInvalidValue;
+    double* rot = (double*)ws + (size_t)per * gs;  // the rotation streams after the records
     const int W = g_lekf_tune.gram_waves;
     const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 4 + W) * sizeof(double);
     const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
@@ -2906,7 +3115,11 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
                                st, djobs, njobs, g0, (double*)ws, gs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs, nvmax);
+        if (s3)
+            e = launch_eig_split(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs,
+                                 nvmax, kmax, rot);
+        else
+            e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs, nvmax);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -5,8 +5,9 @@
 
 struct LekfTune {
     int gram_waves = 4;                 // waves per point of the k > 32 Gram kernel: 2, 4, 8
-    size_t gram_ws_bytes = 1ull << 30;  // Gram workspace per launch (points per sub-batch)
-    int sel_per_cu = 4;                 // selection workgroups (scratch slots) per CU: 1..4
+    size_t gram_ws_bytes = 4ull << 30;  // Gram workspace per launch (points per sub-batch)
+    int sel_per_cu = 3;                 // selection workgroups (scratch slots) per CU: 1..4
+    int eig_split = 1;                  // 32 < k <= 64: three-kernel eigensolver (1) or fused (0)
 };
 
 extern LekfTune g_lekf_tune;

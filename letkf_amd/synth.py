@@ -104,12 +104,13 @@ def _bilinear(field, lon, lat, olon, olat):
 
 def global_problem(*, nlon, lat, sig, ne, nstation, nmode=64, seed, cluster=None,
                    amp=(10.0, 10.0, 5.0, 2.0e-3, 1000.0), errs=(1.0, 1.0, 1.0, 1.0e-4, 100.0),
-                   pert=0.3):
+                   pert=0.3, obs_levels=None):
     """Global sigma-level problem (SPEEDY-like layout: u,v,T,q 3-D + ps 2-D).
 
-    Returns dict with grid (lon, lat, sig), gues3d (nij, nlev, ne, 4) and
-    gues2d (nij, ne, 1) Fortran-ordered, and raw obs records (elm, lon, lat,
-    lev, dat, err) with hxf (nobs, ne) = H(x) of each member.
+    Every station reports u, v, T, q at the model levels obs_levels (indices;
+    default all) and ps.  Returns dict with grid (lon, lat, sig), gues3d (nij,
+    nlev, ne, 4) and gues2d (nij, ne, 1) Fortran-ordered, and raw obs records
+    (elm, lon, lat, lev, dat, err) with hxf (nobs, ne) = H(x) of each member.
     """
     rng = np.random.default_rng(seed)
     lon = 360.0 / nlon * np.arange(nlon)
@@ -148,7 +149,7 @@ def global_problem(*, nlon, lat, sig, ne, nstation, nmode=64, seed, cluster=None
     ids = (ID_U, ID_V, ID_T, ID_Q)
     ps_t = _bilinear(truth_ps[None], lon, lat, slon, slat)[0]
     ps_m = np.stack([_bilinear(p[None], lon, lat, slon, slat)[0] for _, p in mem], axis=1)
-    for lv in range(nlev):
+    for lv in (range(nlev) if obs_levels is None else obs_levels):
         for v in range(4):
             yt = _bilinear(truth3[v][lv][None], lon, lat, slon, slat)[0]
             hx = np.stack([_bilinear(v3[v][lv][None], lon, lat, slon, slat)[0]
@@ -194,30 +195,38 @@ def gaussian_lats(nlat):
     return np.rad2deg(np.arcsin(x))
 
 
-def gfs(ne=40, seed=SEED_BASE + 3, nobs_target=600_000, nlev=64):
+def gfs(ne=40, seed=SEED_BASE + 3, nlev=64, n_land=480, n_ocean=120, nobs_lev=16):
     """Config 3: GFS-sized 192x94x64 with obs clustered 100x denser on land-like patches.
 
+    Stations report u, v, T, q at nobs_lev of the 64 levels (evenly spaced in
+    ln sigma, like mandatory levels) and ps: 480 stations in 8 land-like
+    patches (sd 4 deg) and 120 uniform on the sphere, so the patches are 100x
+    denser.  That gives the configuration's raw local counts of 0 to about
+    3000 (99th percentile ~3000) under sigma_h = 700 km and GFS's vertical
+    scale 0.4 ln p (gfs/letkf/letkf_obs.f90:33), with 39,000 obs.  The stated
+    ~600k obs and counts of at most ~3000 cannot hold together at these
+    scales: the mean local count is nobs x the cutoff disk's share of the
+    sphere (0.040) x the vertical window's share of the levels (~0.63), about
+    15,000 for 600k obs.  The counts (the per-point workload) are kept.
     Gaussian latitudes are computed here (the reference takes them from the
     external NCEP SPLAT routine, gfs/common/common_gfs.f90:114).
     """
     lat = gaussian_lats(94)
     sig = np.exp(np.linspace(np.log(0.995), np.log(0.01), nlev))
-    nst = max(1, nobs_target // (nlev * 4 + 1))
+    obs_levels = np.unique(np.round(np.linspace(0, nlev - 1, nobs_lev)).astype(int))
 
     def cluster(rng, n):
-        # 8 patches holding half the stations at ~100x the background density
         clon = rng.uniform(0, 360, 8)
         clat = np.rad2deg(np.arcsin(rng.uniform(-0.8, 0.8, 8)))
-        nd = n // 2
-        k = rng.integers(0, 8, nd)
-        plon = (clon[k] + rng.normal(0, 4.0, nd)) % 360.0
-        plat = np.clip(clat[k] + rng.normal(0, 4.0, nd), -85, 85)
-        ulat = np.rad2deg(np.arcsin(rng.uniform(-1.0, 1.0, n - nd)))
-        ulon = rng.uniform(0, 360, n - nd)
+        k = rng.integers(0, 8, n_land)
+        plon = (clon[k] + rng.normal(0, 4.0, n_land)) % 360.0
+        plat = np.clip(clat[k] + rng.normal(0, 4.0, n_land), -85, 85)
+        ulat = np.rad2deg(np.arcsin(rng.uniform(-1.0, 1.0, n - n_land)))
+        ulon = rng.uniform(0, 360, n - n_land)
         return np.concatenate([plon, ulon]), np.concatenate([plat, ulat])
 
-    p = global_problem(nlon=192, lat=lat, sig=sig, ne=ne, nstation=nst, seed=seed, cluster=cluster,
-                       nmode=32)
+    p = global_problem(nlon=192, lat=lat, sig=sig, ne=ne, nstation=n_land + n_ocean, seed=seed,
+                       cluster=cluster, nmode=32, obs_levels=obs_levels)
     p.update(sigma_obs=700.0e3, sigma_obsv=0.4, gross_error=5.0, infl=1.01, ips=0, iq=-1,
              q_top_lev=0)
     return p

@@ -74,6 +74,8 @@ def _lib_c():
         _c.orc_obs_local_regional.argtypes = (
             [c_double] * 3 + [c_int, c_int, _ip] + [_dp] * 4 + [_ip, c_int] + [c_double] * 6 +
             [c_int, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _dp])
+        _c.orc_cap_stream.restype = c_int
+        _c.orc_cap_stream.argtypes = [c_int, _dp, c_int, _ip, _ip]
         _c.orc_das_global.restype = c_int
         _c.orc_das_global.argtypes = (
             [c_int] * 5 + [_dp] * 4 + [_dp, _dp, _dp, c_int, _dp, _dp, c_int, c_int, _dp, _dp,
@@ -367,6 +369,31 @@ def obs_local_regional(ri, rj, rz, nx, ny, nobsgrd, obs, dx, dy, sigma_h, sigma_
                                         dlat_zero, nobs_max, _i(isort), _i(sobs), _d(srd), _d(srl),
                                         _i(cnt), _i(idx), _d(rl), _d(rd))
     return idx[:m].copy(), rl[:m].copy(), rd[:m].copy()
+
+
+def cap_stream(key, cap):
+    """The reference's per-bucket cap over one candidate stream in scan order
+    (scale/letkf/letkf_tools.f90:1041-1156): key = priority (smaller first).
+    Returns the candidate index of each storage slot, in slot order (the
+    reference's output order)."""
+    key = _f64(key)
+    cap = int(cap)
+    isort = np.zeros(max(cap, 1), dtype=np.int32)
+    src = np.zeros(max(cap, 1), dtype=np.int32)
+    m = _lib_c().orc_cap_stream(key.size, _d(key), cap, _i(isort), _i(src))
+    return src[:m].copy()
+
+
+def cap_nearest_first(idx, rho, cap):
+    """The global grid's nobs_max cap: the reference cap with key = -rho, so the
+    cap entries of largest rho (ties: the earlier list entry) are kept, in
+    storage-slot order.  Returns (idx, rho) capped."""
+    idx = np.asarray(idx)
+    rho = _f64(rho)
+    if cap <= 0 or idx.size <= cap:
+        return idx.copy(), rho.copy()
+    keep = cap_stream(-rho, cap)
+    return idx[keep].copy(), rho[keep].copy()
 
 
 def das_global(prob, *, q_top_lev=5, iq=3, var_local_row=None, return_nobsl=False):

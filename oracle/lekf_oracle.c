@@ -835,6 +835,49 @@ int orc_obs_local_regional(double ri, double rj, double rz, int nx, int ny, cons
     return nsel;
 }
 
+/*
+ * The reference's cap over one candidate stream (scale/letkf/letkf_tools.f90:
+ * 1041-1156, one (element, obs type) bucket): candidates 0..n-1 in scan order
+ * with priority key[i] (smaller first).  Case 0 skips a candidate when the set
+ * is full and key >= the last-priority key; otherwise the first sorted
+ * position s with key < key(s) (strict, so an earlier candidate wins a tie),
+ * or the first empty one, takes it: into the next storage slot while the set
+ * is not full (cases 1, 2), else into the storage slot of the last-priority
+ * entry, which it evicts (case 3).  slot_src[slot] = candidate in that slot,
+ * the reference's output order (:1139-1150).  The global grid's nearest-first
+ * cap uses key = -rho.  Scratch isort: cap ints.  Returns the kept count.
+ */
+int orc_cap_stream(int n, const double *key, int cap, int *isort, int *slot_src)
+{
+    int cnt = 0;
+    for (int s2 = 0; s2 < cap; ++s2) isort[s2] = -1;
+    for (int i = 0; i < n; ++i) {
+        const double k = key[i];
+        if (cnt >= cap && k >= key[slot_src[isort[cap - 1]]]) continue; /* case 0 */
+        for (int s2 = 0; s2 < cap; ++s2) {
+            if (isort[s2] < 0) { /* case 1 */
+                const int slot = cnt++;
+                isort[s2] = slot;
+                slot_src[slot] = i;
+                break;
+            } else if (k < key[slot_src[isort[s2]]]) {
+                int slot;
+                if (cnt < cap) { /* case 2 */
+                    slot = cnt++;
+                    for (int ss = cnt - 1; ss > s2; --ss) isort[ss] = isort[ss - 1];
+                } else { /* case 3 */
+                    slot = isort[cap - 1];
+                    for (int ss = cap - 1; ss > s2; --ss) isort[ss] = isort[ss - 1];
+                }
+                isort[s2] = slot;
+                slot_src[slot] = i;
+                break;
+            }
+        }
+    }
+    return cnt;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Driver: das_letkf for a global sigma-level grid                            */
 /* speedy/letkf/letkf_tools.f90:45-267                                        */

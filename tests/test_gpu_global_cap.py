@@ -1,11 +1,19 @@
 """Global grid with the nearest-first cap (nobs_max), as the GFS-sized config uses.
 
 Lists: the oracle's uncapped obs_local (speedy/letkf/letkf_tools.f90:272-573;
-bit-exact to the GPU's uncapped lists, test_gpu_das.py) capped here by an
-independent full sort: keep the nobs_max smallest (err^2/rho, scan position),
-in scan order.  Analysis: the reference letkf_core + apply (oracle/_ref) on the
-capped lists, for the 3-D u, v, T rows, within 1e-10 of the analysis
-perturbation scale.  The column-chunked raw lists must give the same lists.
+bit-exact to the GPU's uncapped lists, test_gpu_das.py) capped nearest first:
+the reference's cap (scale/letkf/letkf_tools.f90:1041-1156) with the key -rho,
+so a list keeps its nobs_max entries of largest rho (ties: the earlier entry)
+in the reference's storage-slot order (oracle.cap_nearest_first, pinned in
+tests/test_oracle_regional.py).  The kept set is also checked against an
+independent sort by (-rho, list position).  The GPU lists must equal the
+oracle's in content and order, and the capped lists must hold u, v, T and q
+obs (a key by err^2/rho would keep only the q obs, whose error is 1e-4).
+Analysis: the reference letkf_core + apply (oracle/_ref) on the capped lists,
+for the 3-D u, v, T and q rows and, with the level-1 transform, ps, within
+1e-10 of the analysis perturbation scale; q above level 4 is the background
+rebuilt from its mean and perturbations, bit for bit (the SPEEDY rules,
+speedy/letkf/letkf_tools.f90:172-197).
 """
 import numpy as np
 import pytest
@@ -52,7 +60,6 @@ def _oracle_lists(p):
         psm = psm + g2[:, m]
     psm = psm / p["ne"]
     lists = []
-    err2 = sobs["err"] ** 2
     for b in range(nij * nlev):
         lev, ij = divmod(b, nij)
         dlon = dlat / np.cos(oracle.PI_REF * lat1[ij] / 180.0)
@@ -61,10 +68,10 @@ def _oracle_lists(p):
                                               p["lat"], grd, sobs, dz, dzv, p["sigma_obs"],
                                               p["sigma_obsv"])
         if idx.size > CAP:
-            key = err2[idx] / rho
-            pos = np.arange(idx.size)
-            keep = np.sort(np.lexsort((pos, key))[:CAP])
-            idx, rho = idx[keep], rho[keep]
+            keep = np.lexsort((np.arange(idx.size), -rho))[:CAP]  # independent: a full sort
+            cidx, crho = oracle.cap_nearest_first(idx, rho, CAP)
+            np.testing.assert_array_equal(np.sort(cidx), np.sort(idx[keep]))
+            idx, rho = cidx, crho
         lists.append((idx, rho))
     return sobs, lists
 
@@ -79,25 +86,53 @@ def test_global_cap_lists_and_analysis(gpu, prob):
     nij, nlev = h.nij, h.nlev
     assert max(len(l[0]) for l in lists) == CAP        # the cap binds somewhere
     rng = np.random.default_rng(2)
+    ncapped = 0
+    nall4 = 0
     for b in rng.choice(nij * nlev, 200, replace=False):
         idx, rho = h.local(b)
         ridx, rrho = lists[b]
-        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_array_equal(idx, ridx)  # content and order
         np.testing.assert_allclose(rho, rrho, rtol=1e-13)
         lev, ij = divmod(int(b), nij)
         assert nl[ij, lev] == ridx.size
-    a3, _ = h.analysis()
+        if ridx.size == CAP:
+            ncapped += 1
+            kinds = set(np.round(sobs["elm"][idx]).astype(int).tolist())
+            assert len(kinds) >= 3, kinds  # never a one-element list
+            nall4 += {oracle.ID_U, oracle.ID_V, oracle.ID_T, oracle.ID_Q} <= kinds
+    # u, v, T and q at one station and level share rho; a list misses one
+    # only where the gross-error check rejected it at the nearest stations
+    assert ncapped > 20 and nall4 >= 0.8 * ncapped, (nall4, ncapped)
+    a3, a2 = h.analysis()
     off = np.cumsum([0] + [l[0].size for l in lists])
-    xb = np.ascontiguousarray(p["gues3d"].transpose(1, 0, 3, 2)).reshape(nlev * nij, 4, p["ne"])
+    ne = p["ne"]
+    # the point's rows: u, v, T, q, and at level 1 also ps (speedy/letkf/
+    # letkf_tools.f90:180-197: the 2-D variable takes the level-1 transform)
+    xb = np.zeros((nlev * nij, 5, ne))
+    xb[:, :4] = np.ascontiguousarray(p["gues3d"].transpose(1, 0, 3, 2)).reshape(nlev * nij, 4, ne)
+    xb[:nij, 4] = p["gues2d"][:, :, 0]
     xa, _ = oracle.ref_solve_apply(off, np.concatenate([l[0] for l in lists]),
                                    np.concatenate([l[1] for l in lists]),
                                    np.ascontiguousarray(sobs["hdxf"].T), sobs["err"] ** 2,
-                                   sobs["dep"], xb[:, :3], np.ones(nlev * nij), fixed_infl=1.0)
-    ref = xa.reshape(nlev, nij, 3, p["ne"]).transpose(1, 0, 3, 2)
+                                   sobs["dep"], xb, np.ones(nlev * nij), fixed_infl=1.0)
+    ref = xa[:, :4].reshape(nlev, nij, 4, ne).transpose(1, 0, 3, 2)
     dev = ref - ref.mean(axis=2, keepdims=True)
-    for v in range(3):
-        err = np.abs(a3[..., v] - ref[..., v]).max() / np.abs(dev[..., v]).max()
+    upd = np.ones((nij, nlev, ne, 4), dtype=bool)
+    upd[:, 4:, :, 3] = False  # q at levels 5.. is not analysed (letkf_tools.f90:172-177)
+    for v in range(4):
+        err = np.abs(a3[..., v] - ref[..., v])[upd[..., v]].max() / np.abs(dev[..., v]).max()
         assert err < 1e-10, (v, err)
+    # the frozen q rows are the background rebuilt from its mean and
+    # perturbations, mean summed in member order (common_speedy.f90:335-370)
+    xq = p["gues3d"][:, 4:, :, 3]
+    mq = xq[:, :, 0].copy()
+    for m in range(1, ne):
+        mq = mq + xq[:, :, m]
+    mq = mq / ne
+    np.testing.assert_array_equal(a3[:, 4:, :, 3], mq[:, :, None] + (xq - mq[:, :, None]))
+    ps_ref = xa[:nij, 4]
+    ps_dev = ps_ref - ps_ref.mean(axis=1, keepdims=True)
+    assert np.abs(a2[:, :, 0] - ps_ref).max() / np.abs(ps_dev).max() < 1e-10
 
 
 def test_global_cap_large_cap_equals_uncapped(gpu, prob):

@@ -3,8 +3,20 @@
 Reference: the oracle's restatement of the SCALE obs index and obs_local
 (pinned against brute force in tests/test_oracle_regional.py) for the local
 lists, and the reference Fortran letkf_core + apply (oracle/_ref) for the
-analysis.  Selection sets must be identical (bit-exact cutoff arithmetic);
-the analysis within 1e-10 of each variable's analysis perturbation scale.
+analysis.  Local lists must be identical in content and order: scan order
+uncapped, bucket by bucket in the reference's storage-slot order capped
+(scale/letkf/letkf_tools.f90:1139-1150; the oracle's order is pinned in
+tests/test_oracle_regional.py).  The analysis within 1e-10 of each
+variable's analysis perturbation scale.
+
+rho = exp(-ndist/2) comes from the device exp on the GPU and from the C
+library's exp in the oracle; the two differ in the last bit for a small share
+of arguments.  The cutoff decisions do not depend on exp, but the cap's keys
+rdiag = err^2/rho do: two keys one unit in the last place apart can swap.
+Gridded obs at rounded positions produce such near-ties between different
+geometries (equal ndist in exact arithmetic, one ulp apart after rounding),
+so there the cap itself is checked bit for bit on the GPU's own keys: the
+uncapped GPU lists (same scan order, same rho) through the oracle's cap.
 """
 import numpy as np
 import pytest
@@ -24,6 +36,7 @@ def _problem(ne, seed, nobs_max):
 
 def _oracle(p):
     nx, ny, nlev, nij = p["nx"], p["ny"], p["nlev"], p["nij"]
+    nbkt = int(np.max(p["bucket"])) + 1
     perm, grd = oracle.obs_bins_regional(p["ri"], p["rj"], nx, ny)
     so = {k: np.asarray(p[k])[perm] for k in ("ri", "rj", "z", "err", "bucket")}
     idx_all, rho_all, off, lists = [], [], [0], []
@@ -31,7 +44,7 @@ def _oracle(p):
         lev, ij = divmod(b, nij)
         idx, rl, _ = oracle.obs_local_regional(float(ij % nx + 1), float(ij // nx + 1),
                                                p["zlev"][lev], nx, ny, grd, so, p["dx"], p["dy"],
-                                               p["sigma_h"], p["sigma_v"], p["nobs_max"], 2)
+                                               p["sigma_h"], p["sigma_v"], p["nobs_max"], nbkt)
         lists.append((idx, rl))
         idx_all.append(idx)
         rho_all.append(rl)
@@ -55,6 +68,19 @@ def _handle(p, budget=0, prec="f64"):
     return h
 
 
+def _capped_from_gpu_keys(h0, b, err, bucket, cap):
+    """The reference cap (oracle.cap_stream) applied to the GPU's uncapped
+    list of point b, keys err^2/rho from the GPU's rho, bucket by bucket."""
+    idx, rho = h0.local(b)
+    out = []
+    for t in range(int(np.max(bucket)) + 1 if bucket.size else 0):
+        sel = np.flatnonzero(bucket[idx] == t)
+        if sel.size > cap:
+            sel = sel[oracle.cap_stream(err[idx[sel]] ** 2 / rho[sel], cap)]
+        out.append(idx[sel])
+    return np.concatenate(out) if out else idx[:0]
+
+
 def _scaled(a, ref):
     dev = ref - ref.mean(axis=2, keepdims=True)
     out = 0.0
@@ -75,17 +101,10 @@ def test_regional_matches_reference(gpu, ne, cap):
     for b in rng.choice(p["nij"] * p["nlev"], 60, replace=False):
         idx, rho = h.local(b)
         ridx, rrho = ref["lists"][b]
-        assert idx.size == ridx.size
-        bk = ref["so"]["bucket"]
-        for t in range(2):
-            np.testing.assert_array_equal(np.sort(idx[bk[idx] == t]), np.sort(ridx[bk[ridx] == t]))
+        np.testing.assert_array_equal(idx, ridx)  # content and order
+        np.testing.assert_allclose(rho, rrho, rtol=1e-13)
         if cap > 0:
-            assert np.all(np.diff(bk[idx]) >= 0)
-            for t in range(2):
-                assert np.all(np.diff(idx[bk[idx] == t]) > 0)  # scan order in a bucket
-        else:
-            np.testing.assert_array_equal(idx, ridx)
-            np.testing.assert_allclose(rho, rrho, rtol=1e-13)
+            assert np.all(np.diff(ref["so"]["bucket"][idx]) >= 0)  # bucket by bucket
         lev, ij = divmod(int(b), p["nij"])
         assert nl[ij, lev] == ridx.size
     a3, _ = h.analysis()
@@ -134,20 +153,26 @@ def test_regional_exact_ties(gpu):
     h = _handle(p)
     h.run()
     h.sync()
+    q = dict(p, nobs_max=0)
+    h0 = _handle(q)
+    h0.run()
+    h0.sync()
+    so = ref["so"]
     for b in range(0, p["nij"] * p["nlev"], 97):
         idx, _ = h.local(b)
         ridx, _ = ref["lists"][b]
-        np.testing.assert_array_equal(np.sort(idx), np.sort(ridx))
+        np.testing.assert_array_equal(np.sort(idx), np.sort(ridx))  # the kept set
+        np.testing.assert_array_equal(idx, _capped_from_gpu_keys(h0, b, so["err"],
+                                                                 so["bucket"], 15))
     a3, _ = h.analysis()
     assert _scaled(a3, ref["anal3d"]) < RTOL
 
 
 @pytest.mark.parametrize("nsame,split", [(700, False), (500, True)])
 def test_regional_many_equal_keys(gpu, nsame, split):
-    """More than 256 co-located obs in one bucket.  700 bit-identical rdiag keys:
-    the threshold is found by the tie scan in scan order.  500 keys in two
-    groups of 250 that differ only in the lowest 3 bits: the last digit pass
-    leaves a sortable bin whose every key bit is fixed."""
+    """More than one block of co-located obs in one bucket: 700 bit-identical
+    rdiag keys (earlier obs win every tie), or 500 keys in two groups of 250
+    that differ only in the lowest bit."""
     p = _problem(12, seed=9, nobs_max=30)
     n = p["ri"].size
     sel = np.arange(0, n, 3)[:nsame]
@@ -162,9 +187,62 @@ def test_regional_many_equal_keys(gpu, nsame, split):
     h = _handle(p)
     h.run()
     h.sync()
+    h0 = _handle(dict(p, nobs_max=0))
+    h0.run()
+    h0.sync()
+    so = ref["so"]
     for b in [3 * p["nij"] + 9 * p["nx"] + 11] + list(range(3 * p["nij"], 4 * p["nij"], 13)):
         idx, _ = h.local(b)
         ridx, _ = ref["lists"][b]
         np.testing.assert_array_equal(np.sort(idx), np.sort(ridx))
+        np.testing.assert_array_equal(idx, _capped_from_gpu_keys(h0, b, so["err"],
+                                                                 so["bucket"], 30))
+    a3, _ = h.analysis()
+    assert _scaled(a3, ref["anal3d"]) < RTOL
+
+
+def test_regional_four_buckets_capped(gpu):
+    """Buckets 0..3 (the largest bucket index the C ABI takes) with the cap:
+    every bucket's obs reach the lists, in the reference's order."""
+    p = _problem(12, seed=13, nobs_max=20)
+    rng = np.random.default_rng(4)
+    p["bucket"] = rng.integers(0, 4, p["ri"].size).astype(np.int32)
+    ref = _oracle(p)
+    h = _handle(p)
+    h.run()
+    h.sync()
+    seen = set()
+    for b in range(0, p["nij"] * p["nlev"], 29):
+        idx, rho = h.local(b)
+        ridx, rrho = ref["lists"][b]
+        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_allclose(rho, rrho, rtol=1e-13)
+        seen.update(ref["so"]["bucket"][idx].tolist())
+    assert seen == {0, 1, 2, 3}
+    a3, _ = h.analysis()
+    assert _scaled(a3, ref["anal3d"]) < RTOL
+
+
+def test_regional_cap_above_lds(gpu):
+    """A cap above 1024 per bucket (the kept set held in global memory) that
+    binds: three obs per grid cell and level."""
+    p = _problem(12, seed=17, nobs_max=1100)
+    rng = np.random.default_rng(6)
+    n = p["ri"].size
+    rep = lambda a: np.concatenate([a, a, a])  # noqa: E731
+    for k in ("ri", "rj", "z", "err", "dep", "bucket"):
+        p[k] = rep(np.asarray(p[k]))
+    p["hdxf"] = np.asfortranarray(np.concatenate([p["hdxf"]] * 3, axis=0))
+    p["ri"] = np.clip(p["ri"] + rng.uniform(-0.3, 0.3, 3 * n), 1.0, p["nx"])
+    p["rj"] = np.clip(p["rj"] + rng.uniform(-0.3, 0.3, 3 * n), 1.0, p["ny"])
+    ref = _oracle(p)
+    assert max(np.bincount(ref["so"]["bucket"][l[0]], minlength=2).max()
+               for l in ref["lists"]) == 1100  # the cap binds
+    h = _handle(p)
+    h.run()
+    h.sync()
+    for b in range(0, p["nij"] * p["nlev"], 53):
+        idx, _ = h.local(b)
+        np.testing.assert_array_equal(idx, ref["lists"][b][0])
     a3, _ = h.analysis()
     assert _scaled(a3, ref["anal3d"]) < RTOL

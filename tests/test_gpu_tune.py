@@ -1,7 +1,8 @@
 """Launch tuning leaves the analysis unchanged.
 
-The Gram workspace size and the selection slot count only re-partition the
-same per-point work, so the analysis must be bit-identical for every value;
+The Gram workspace size, the selection slot count and the choice of the
+fused or three-kernel eigensolver only re-partition the same per-point work,
+so the analysis must be bit-identical for every value;
 the Gram wave count re-orders the waves' partial-sum reduction, so the
 analysis agrees to rounding (1e-12 of the analysis perturbation scale).
 A graph captured before a change is retaken after it, and the autotuner's
@@ -80,6 +81,21 @@ def test_slots_and_workspace_bit_identical(gpu, mk):
     _eq(_analysis(job), a0)
 
 
+@pytest.mark.parametrize("mk", [_radar, _global_capped])
+def test_eig_split_bit_identical(gpu, mk):
+    """k = 50 and k = 40: the three-kernel eigensolver (Householder, QL with one
+    lane per point recording its rotations, replay on Q) performs the fused
+    kernel's operations in the same order, so the analysis is bit-identical."""
+    job = mk()
+    tune.set("eig_split", 1)
+    a1 = _analysis(job)
+    tune.set("eig_split", 0)
+    _eq(_analysis(job), a1)
+    tune.set("eig_split", 1)
+    tune.set("gram_ws_mb", 16)  # many sub-batches
+    _eq(_analysis(job), a1)
+
+
 @pytest.mark.parametrize("waves", [2, 8])
 def test_gram_waves_agree(gpu, waves):
     job = _radar()
@@ -109,7 +125,8 @@ def test_autotune_picks_candidate_and_caches(gpu, tmp_path):
     h = _make(job)
     path = str(tmp_path / "tune.json")
     tune.apply({"gram_waves": 4, "gram_ws_mb": 1024, "sel_per_cu": 4})
-    cands = {"gram_waves": (2, 4), "gram_ws_mb": (64, 1024), "sel_per_cu": (2, 4)}
+    cands = {"gram_waves": (2, 4), "gram_ws_mb": (64, 1024), "sel_per_cu": (2, 4),
+             "eig_split": (1, 0)}
     out = tune.autotune(h, candidates=cands, runs=2, key="test|radar", path=path)
     for n, v in out["params"].items():
         assert v in cands[n]

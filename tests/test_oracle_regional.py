@@ -7,8 +7,14 @@
 * capped selection keeps, per bucket, exactly the nobs_max smallest
   (rdiag, scan position) pairs of the uncapped list (a full sort, not the
   reference's insertion sort), so the insertion-sort restatement is pinned
-  by an independent algorithm; exact ties go to the earlier obs.
+  by an independent algorithm; exact ties go to the earlier obs;
+* the order of a capped bucket is the reference's storage-slot order
+  (scale/letkf/letkf_tools.f90:1063-1150): checked against hand-worked
+  cases and against an independent max-heap simulation of the slot process
+  (a newcomer below the set's largest entry evicts it and takes its slot).
 """
+import heapq
+
 import numpy as np
 import pytest
 
@@ -97,3 +103,67 @@ def test_selection_uncapped_and_capped(pt, dup):
         for b in range(2):
             np.testing.assert_array_equal(np.sort(cidx[got_b == b]), want[b])
         np.testing.assert_allclose(crd, so["err"][cidx] ** 2 / crl, rtol=1e-15)
+
+
+def _heap_slots(key, cap):
+    """Storage slots of the reference cap by a max-heap over (key, position):
+    an independent algorithm (no sorted index, no shifting)."""
+    heap = []  # (-key, -pos, slot): the top is the largest (key, pos)
+    slot_src = []
+    for i, k in enumerate(key):
+        if len(slot_src) < cap:
+            heapq.heappush(heap, (-k, -i, len(slot_src)))
+            slot_src.append(i)
+        elif (k, i) < (-heap[0][0], -heap[0][1]):
+            _, _, sl = heapq.heapreplace(heap, (-k, -i, heap[0][2]))
+            slot_src[sl] = i
+    return np.array(slot_src, dtype=np.int64)
+
+
+def test_cap_stream_known_answers():
+    # a falling key sequence: 3 evicts 5 (slot 0), 2 evicts 4 (slot 1), 1 evicts 3 (slot 0)
+    np.testing.assert_array_equal(oracle.cap_stream([5.0, 4.0, 3.0, 2.0, 1.0], 2), [4, 3])
+    # ties: an equal key never evicts, so the earlier obs stay
+    np.testing.assert_array_equal(oracle.cap_stream([1.0, 1.0, 1.0], 2), [0, 1])
+    np.testing.assert_array_equal(oracle.cap_stream([2.0, 1.0, 1.0], 1), [1])
+    # within the cap nothing moves: scan order
+    np.testing.assert_array_equal(oracle.cap_stream([3.0, 1.0, 2.0], 5), [0, 1, 2])
+    # a rising sequence keeps the first cap in scan order
+    np.testing.assert_array_equal(oracle.cap_stream(np.arange(10.0), 3), [0, 1, 2])
+    assert oracle.cap_stream([], 3).size == 0
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_cap_stream_equals_heap_simulation_and_sort(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(0, 3000))
+    cap = int(rng.integers(1, 400))
+    kind = seed % 3
+    if kind == 0:
+        key = rng.integers(0, 6, n).astype(float)  # many exact ties
+    elif kind == 1:
+        key = rng.standard_normal(n)
+    else:  # a radar-like sweep: far, near, far again
+        key = np.abs(np.arange(n) - n / 2.0) + rng.uniform(0.0, 40.0, n)
+    got = oracle.cap_stream(key, cap)
+    np.testing.assert_array_equal(got, _heap_slots(key, cap))
+    keep = np.sort(np.lexsort((np.arange(n), key))[:cap])  # the set: a full sort
+    np.testing.assert_array_equal(np.sort(got), keep)
+
+
+def test_capped_buckets_in_slot_order():
+    o = _obs(6000, 7, dup=True)
+    perm, grd = oracle.obs_bins_regional(o["ri"], o["rj"], NX, NY)
+    so = _sorted(o, perm)
+    ri, rj, rz = 14.0, 12.0, 3000.0
+    sh = sv = 2000.0
+    idx, rl, rd = oracle.obs_local_regional(ri, rj, rz, NX, NY, grd, so, DX, DY, sh, sv, 0, 2)
+    for cap in (1, 13, 150):
+        cidx, _, _ = oracle.obs_local_regional(ri, rj, rz, NX, NY, grd, so, DX, DY, sh, sv, cap,
+                                               2)
+        want = []
+        for b in range(2):
+            sel = np.flatnonzero(so["bucket"][idx] == b)
+            assert sel.size > cap  # the cap binds
+            want.append(idx[sel][_heap_slots(rd[sel], cap)])
+        np.testing.assert_array_equal(cidx, np.concatenate(want))

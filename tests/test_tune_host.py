@@ -18,14 +18,18 @@ def test_defaults_and_roundtrip():
     cur = tune.current()
     assert cur["gram_waves"] in (2, 4, 8)
     assert cur["sel_per_cu"] in (1, 2, 3, 4)
+    assert cur["eig_split"] in (0, 1)
     tune.set("gram_waves", 8)
     tune.set("gram_ws_mb", 64)
     tune.set("sel_per_cu", 3)
-    assert tune.current() == {"gram_waves": 8, "gram_ws_mb": 64, "sel_per_cu": 3}
+    tune.set("eig_split", 0)
+    assert tune.current() == {"gram_waves": 8, "gram_ws_mb": 64, "sel_per_cu": 3,
+                              "eig_split": 0}
 
 
 @pytest.mark.parametrize("name,bad", [("gram_waves", 3), ("gram_waves", 16), ("gram_ws_mb", 8),
-                                      ("sel_per_cu", 0), ("sel_per_cu", 5)])
+                                      ("sel_per_cu", 0), ("sel_per_cu", 5), ("eig_split", 2),
+                                      ("eig_split", -1)])
 def test_out_of_range_rejected(name, bad):
     before = tune.get(name)
     with pytest.raises(_lib.LetkfError):
@@ -45,4 +49,6 @@ def test_cache_load(tmp_path):
                                                  "sel_per_cu": 1}, "ms": 1.0}}))
     assert tune.load("missing", str(path)) is None
     assert tune.load("k", str(path)) == {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
-    assert tune.current() == {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
+    cur = tune.current()
+    assert {n: cur[n] for n in ("gram_waves", "gram_ws_mb", "sel_per_cu")} == \
+        {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
