@@ -125,6 +125,20 @@ int lekf_core_f64(int ne, int nobs, int nobsl, const double *hdxb, const double 
  * NULL = default stream); prec selects the Gram arithmetic.  Asynchronous. */
 int lekf_core_batch(const lekf_job *jobs, int njobs, int prec, void *stream);
 
+/* --- 2b. batched local solve without allocation -----------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same solve as lekf_core_batch, bit for bit, with
+ * the job descriptors and the Gram workspace in a fixed device array of
+ * LEKF_NOALLOC_BYTES per GPU (held until the process ends) instead of
+ * library allocations; it calls no hipMalloc/hipFree.  At most
+ * LEKF_NOALLOC_MAX_JOBS jobs; a point record (k > 32) larger than the
+ * workspace part returns LEKF_ERR_SIZE (lekf_last_error says which).  Calls
+ * that overlap on one GPU share the array and race: serialise them. */
+#define LEKF_NOALLOC_BYTES (256L << 20)
+#define LEKF_NOALLOC_MAX_JOBS 1024
+int lekf_core_batch_noalloc(const lekf_job *jobs, int njobs, int prec, void *stream);
+
 /* --- 3. selection on device memory ---------------------------------------
  * Lorenz-96 rule (lorenz96/letkf/letkf.f90:216-254): obs i of grid index
  * gidx[i]; point ix keeps |periodic distance| < 2*sqrt(10/3)*xlocal and

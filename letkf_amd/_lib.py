@@ -48,6 +48,8 @@ LEKF_TUNE_GRAM_WAVES = 0
 LEKF_TUNE_GRAM_WS_MB = 1
 LEKF_TUNE_SEL_PER_CU = 2
 LEKF_TUNE_EIG_SPLIT = 3
+LEKF_NOALLOC_BYTES = 256 << 20
+LEKF_NOALLOC_MAX_JOBS = 1024
 LEKF_PT_NOCONV = 2
 LEKF_NVMAX = 16
 LEKF_KMAX = 256
@@ -148,6 +150,8 @@ def lib():
                                 c_int, _dp]
     L.lekf_core_batch.restype = c_int
     L.lekf_core_batch.argtypes = [POINTER(LekfJob), c_int, c_int, c_void_p]
+    L.lekf_core_batch_noalloc.restype = c_int
+    L.lekf_core_batch_noalloc.argtypes = [POINTER(LekfJob), c_int, c_int, c_void_p]
     L.lekf_obs_local_l96.restype = c_int
     L.lekf_obs_local_l96.argtypes = [c_int, c_int, c_void_p, c_double, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p]
@@ -262,7 +266,7 @@ def lib():
 
 # every symbol include/lekf.h declares (checked by the CPU test suite)
 EXPORTED = [
-    "lekf_core_f64", "lekf_core_batch", "lekf_obs_local_l96", "lekf_global_create",
+    "lekf_core_f64", "lekf_core_batch", "lekf_core_batch_noalloc", "lekf_obs_local_l96", "lekf_global_create",
     "lekf_l96_create", "lekf_regional_create", "lekf_destroy", "lekf_set_ensemble",
     "lekf_set_infl", "lekf_set_obs_global", "lekf_set_obs_l96", "lekf_set_obs_regional",
     "lekf_das_run", "lekf_das_select", "lekf_das_solve", "lekf_set_stream", "lekf_das_run_many", "lekf_sync", "lekf_get_analysis", "lekf_get_infl",

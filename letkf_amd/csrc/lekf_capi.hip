@@ -153,6 +153,46 @@ int core_init()
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Device scratch of lekf_core_batch_noalloc: the jobs' descriptors, then the
+// Gram workspace, in a file-scope array of fixed size (each GPU holds its own
+// copy for the life of the process).  Host code never uses the array's name
+// as an address: the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.  No solve kernel reads workspace it has
+// not written in the same call, so nothing is cleared between calls; calls
+// that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[LEKF_NOALLOC_BYTES];
+
+// descriptors first, the workspace from a 4 KiB boundary after them
+constexpr size_t kNoallocJobBytes =
+    ((size_t)LEKF_NOALLOC_MAX_JOBS * sizeof(lekf_job) + 4095) & ~(size_t)4095;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// device address of this device's scratch (nullptr and an error message on failure)
+void* scratch_ptr()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        snprintf(g_errmsg, sizeof(g_errmsg), "lekf_core_batch_noalloc: no usable device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            snprintf(g_errmsg, sizeof(g_errmsg), "hipGetSymbolAddress(scratch): %s",
+                     hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // One solve launch; ws is grown to the Gram workspace the class needs.
 int run_solve(const lekf_job* djobs, int njobs, long npt, int kmax, int nvmax, int mode,
               DevBuf& ws, hipStream_t st)
@@ -166,6 +206,43 @@ int run_solve(const lekf_job* djobs, int njobs, long npt, int kmax, int nvmax, i
 
 // state rows a job's points apply their transform to (0 without the apply)
 int job_rows(const lekf_job& j) { return (j.flags & LEKF_F_APPLY) ? j.nv3d + j.nv2d : 0; }
+
+// The grouped launches of launch_grouped over caller-provided device memory:
+// descriptors at dj (room for J.size()), workspace ws of ws_bytes.  Nothing is
+// allocated.  Returns LEKF_ERR_SIZE when a class's point record does not fit.
+int launch_grouped_fixed(std::vector<lekf_job> J, int mode, lekf_job* dj, void* ws,
+                         size_t ws_bytes, hipStream_t st)
+{
+    std::stable_sort(J.begin(), J.end(), [](const lekf_job& a, const lekf_job& b) {
+        return lekf_solve_class(a.k) < lekf_solve_class(b.k);
+    });
+    std::vector<size_t> starts;
+    for (size_t i = 0; i < J.size(); ++i)
+        if (i == 0 || lekf_solve_class(J[i].k) != lekf_solve_class(J[i - 1].k)) starts.push_back(i);
+    starts.push_back(J.size());
+    std::vector<long> tot(starts.size(), 0);
+    std::vector<int> kmax(starts.size(), 0), nvmax(starts.size(), 0);
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        for (size_t i = starts[g]; i < starts[g + 1]; ++i) {
+            J[i].pt_begin = tot[g];
+            tot[g] += J[i].npt;
+            kmax[g] = std::max(kmax[g], J[i].k);
+            nvmax[g] = std::max(nvmax[g], job_rows(J[i]));
+        }
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        if (tot[g] > 0 && lekf_solve_ws_bytes(1, kmax[g]) > ws_bytes) {
+            snprintf(g_errmsg, sizeof(g_errmsg),
+                     "lekf_core_batch_noalloc: a point record of k = %d needs %zu bytes, the "
+                     "workspace holds %zu", kmax[g], lekf_solve_ws_bytes(1, kmax[g]), ws_bytes);
+            return LEKF_ERR_SIZE;
+        }
+    HIPCHK(hipMemcpyAsync(dj, J.data(), sizeof(lekf_job) * J.size(), hipMemcpyHostToDevice, st));
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        HIPCHK(lekf_solve_launch(dj + starts[g], (int)(starts[g + 1] - starts[g]), tot[g], kmax[g],
+                                 nvmax[g], mode, kmax[g] > 32 ? ws : nullptr,
+                                 kmax[g] > 32 ? ws_bytes : 0, st));
+    return 0;
+}
 
 // Launch the solves of jobs (all of one arithmetic mode) grouped by kernel
 // class: descriptors are uploaded once, sorted by class, and each class is
@@ -475,6 +552,34 @@ int lekf_core_batch(const lekf_job* jobs, int njobs, int prec, void* stream)
     DevBuf*& wb = t_core.ws_by_stream[st];
     if (!wb) wb = new DevBuf();
     return launch_grouped(J, mode, *db, *wb, st);
+}
+
+int lekf_core_batch_noalloc(const lekf_job* jobs, int njobs, int prec, void* stream)
+{
+    if (njobs <= 0 || !jobs) return LEKF_ERR_ARG;
+    if (njobs > LEKF_NOALLOC_MAX_JOBS) {
+        snprintf(g_errmsg, sizeof(g_errmsg), "lekf_core_batch_noalloc: %d jobs, at most %d", njobs,
+                 LEKF_NOALLOC_MAX_JOBS);
+        return LEKF_ERR_SIZE;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return LEKF_ERR_NODEV;
+    std::vector<lekf_job> J(jobs, jobs + njobs);
+    long tot = 0;
+    const int mode = mode_of_prec(prec);
+    for (auto& j : J) {
+        if (j.k < 2 || j.k > LEKF_KMAX || j.npt < 0) return LEKF_ERR_SIZE;
+        if ((j.flags & LEKF_F_APPLY) && (j.nv3d + j.nv2d > LEKF_NVMAX || j.nij <= 0))
+            return LEKF_ERR_SIZE;
+        if (mode != 0 && !j.hdxf32) return LEKF_ERR_ARG;
+        if (!j.loc_off || (j.npt > 0 && (!j.loc_idx || !j.loc_rho))) return LEKF_ERR_ARG;
+        tot += j.npt;
+    }
+    if (tot == 0) return LEKF_OK;
+    unsigned char* base = (unsigned char*)scratch_ptr();
+    if (!base) return LEKF_ERR_HIP;
+    return launch_grouped_fixed(J, mode, (lekf_job*)base, base + kNoallocJobBytes,
+                                (size_t)LEKF_NOALLOC_BYTES - kNoallocJobBytes, (hipStream_t)stream);
 }
 
 int lekf_obs_local_l96(int nx, int ny, const int* gidx, double xlocal, long* loc_off, int* loc_cnt,
