@@ -417,6 +417,18 @@ int lekf_event_sync(void *ev);
 int lekf_tune_set(int param, long value);
 long lekf_tune_get(int param);
 
+/* per-kernel timing, for measurement (no counterpart in the reference, which
+ * times whole phases with its TIMER prints, speedy/letkf/letkf.f90:31,70-72).
+ * lekf_ktime_enable(1) clears the record and starts recording a HIP event
+ * pair on the launch's stream around each launch of the named kernels
+ * ("select" the regional selection, "list_cap" the global cap, "cull" the
+ * global candidate scan, "solve" the fused k <= 32 kernel, "gram", "eig",
+ * "eig_tri", "eig_ql", "eig_apply", "big"); launches captured into a graph
+ * are not recorded.  lekf_ktime_read waits for the recorded launches of one
+ * name and returns their summed time and count.  Process-wide; 0 stops. */
+int lekf_ktime_enable(int on);
+int lekf_ktime_read(const char *name, double *total_ms, long *launches);
+
 /* library info */
 int lekf_device_count(void);
 /* free and total device memory of the current GPU, bytes */

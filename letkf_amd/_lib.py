@@ -260,6 +260,8 @@ def lib():
     L.lekf_tune_set.argtypes = [c_int, c_long]
     L.lekf_tune_get.restype = c_long
     L.lekf_tune_get.argtypes = [c_int]
+    L.lekf_ktime_enable.argtypes = [c_int]
+    L.lekf_ktime_read.argtypes = [c_char_p, POINTER(c_double), POINTER(c_long)]
     _lib = L
     return L
 
@@ -277,7 +279,7 @@ EXPORTED = [
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
-    "lekf_tune_get", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
+    "lekf_tune_get", "lekf_ktime_enable", "lekf_ktime_read", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
     "lekf_obsope_radar", "lekf_obs_departure_scale", "lekf_set_obs_radar",
 ]
 

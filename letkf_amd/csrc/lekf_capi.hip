@@ -21,6 +21,7 @@
 #include "lekf_relax.hpp"
 #include "lekf_l96.hpp"
 #include "lekf_tune.hpp"
+#include "lekf_ktime.hpp"
 
 #define LEKF_VERSION "lekf-mi355x 0.1 (gfx950)"
 
@@ -434,6 +435,63 @@ int lekf_tune_set(int param, long value)
     return LEKF_OK;
 }
 
+}  // extern "C"
+
+// --- per-kernel timing (measurement) ---------------------------------------
+namespace {
+std::atomic<int> g_ktime_on{0};
+std::mutex g_ktime_mu;
+struct KRec {
+    const char* name;
+    hipStream_t st;
+    hipEvent_t a, b;
+};
+std::vector<KRec> g_ktime_recs;       // event pairs in launch order
+std::vector<hipEvent_t> g_ktime_pool; // events of cleared pairs, for reuse
+
+void ktime_clear_locked()
+{
+    for (KRec& r : g_ktime_recs) {
+        if (r.a) g_ktime_pool.push_back(r.a);
+        if (r.b) g_ktime_pool.push_back(r.b);
+    }
+    g_ktime_recs.clear();
+}
+}  // namespace
+
+void lekf_ktime_mark(const char* name, hipStream_t st, bool begin)
+{
+    if (!g_ktime_on.load(std::memory_order_relaxed)) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    std::lock_guard<std::mutex> lk(g_ktime_mu);
+    hipEvent_t e = nullptr;
+    if (!g_ktime_pool.empty()) {
+        e = g_ktime_pool.back();
+        g_ktime_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return;
+    }
+    if (hipEventRecord(e, st) != hipSuccess) {
+        g_ktime_pool.push_back(e);
+        return;
+    }
+    if (begin) {
+        g_ktime_recs.push_back({name, st, e, nullptr});
+        return;
+    }
+    for (size_t i = g_ktime_recs.size(); i-- > 0;) {
+        KRec& r = g_ktime_recs[i];
+        if (!r.b && r.st == st && std::strcmp(r.name, name) == 0) {
+            r.b = e;
+            return;
+        }
+    }
+    g_ktime_pool.push_back(e);
+}
+
+extern "C" {
+
 long lekf_tune_get(int param)
 {
     switch (param) {
@@ -443,6 +501,33 @@ long lekf_tune_get(int param)
     case LEKF_TUNE_EIG_SPLIT: return g_lekf_tune.eig_split;
     default: return -LEKF_ERR_ARG;
     }
+}
+
+int lekf_ktime_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_ktime_mu);
+    if (on) ktime_clear_locked();
+    g_ktime_on.store(on ? 1 : 0);
+    return LEKF_OK;
+}
+
+int lekf_ktime_read(const char* name, double* total_ms, long* launches)
+{
+    if (!name || !total_ms || !launches) return LEKF_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_ktime_mu);
+    double t = 0.0;
+    long n = 0;
+    for (const KRec& r : g_ktime_recs) {
+        if (!r.b || std::strcmp(r.name, name) != 0) continue;
+        HIPCHK(hipEventSynchronize(r.b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        ++n;
+    }
+    *total_ms = t;
+    *launches = n;
+    return LEKF_OK;
 }
 
 int lekf_mem_info(size_t* free_bytes, size_t* total_bytes)

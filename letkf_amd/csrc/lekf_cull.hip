@@ -15,6 +15,7 @@
 // Lorenz-96 (lorenz96/letkf/letkf.f90:216-254): periodic grid distance.
 #include "lekf_dev.hpp"
 #include "lekf_cull.hpp"
+#include "lekf_ktime.hpp"
 
 // Every FP64 operation in this file is rounded as written: no FMA contraction
 // (the reference's results are reproduced operation by operation).
@@ -536,6 +537,7 @@ static unsigned fill_level_groups(int ncols, int nlev)
 hipError_t lekf_cull_global_launch(const GlobalCullArgs& A, hipStream_t st)
 {
     const int nb = (A.nij + 255) / 256;
+    KTime kt("cull", st);
     hipLaunchKernelGGL(cull_global_count, dim3(nb), dim3(256), 0, st, A);
     hipLaunchKernelGGL(scan_ex, dim3(1), dim3(1024), 0, st, A.cand_cnt, A.cand_off, A.nij);
     hipLaunchKernelGGL(cull_global_fill, dim3(A.nij, fill_level_groups(A.nij, A.nlev)), dim3(64),
@@ -554,6 +556,7 @@ hipError_t lekf_cull_global_count_launch(const GlobalCullArgs& A, hipStream_t st
 hipError_t lekf_cull_global_fill_launch(const GlobalCullArgs& A, int ncols, hipStream_t st)
 {
     if (ncols <= 0) return hipSuccess;
+    KTime kt("cull", st);
     hipLaunchKernelGGL(cull_global_fill, dim3(ncols, fill_level_groups(ncols, A.nlev)), dim3(64), 0,
                        st, A);
     return hipGetLastError();

@@ -34,6 +34,7 @@
 #include "lekf_dev.hpp"
 #include "lekf_regional.hpp"
 #include "lekf_tune.hpp"
+#include "lekf_ktime.hpp"
 
 // Every FP64 operation in this file is rounded as written: no FMA contraction
 // (the reference's results are reproduced operation by operation).
@@ -54,12 +55,9 @@ constexpr int kSelW = kSelT / 64;
 constexpr int kCapU = 1;               // candidates per thread in a block of the cap
 constexpr int kCapB = kSelT * kCapU;   // candidates per block
 
-// The kept set S of the cap (ascending (key, candidate), double-buffered) and
-// one block's potential entrants, entrants and evictions.
+// One block's potential entrants, entrants and evictions of the workgroup
+// form of the cap (caps above LEKF_CAP_LDS, the kept set S in global memory).
 struct CapLds {
-    unsigned long long skey[2][LEKF_CAP_LDS];
-    int spos[2][LEKF_CAP_LDS];
-    int sslot[2][LEKF_CAP_LDS];
     unsigned long long pkey[kCapB];  // potential entrants, scan order
     int ppos[kCapB];
     unsigned long long nkey[kCapB];  // entrants, scan order
@@ -81,8 +79,33 @@ struct CapGlobal {
     int* slot;
 };
 
+// One wave's cap (caps up to LEKF_CAP_LDS): the kept set S ascending by
+// (key, candidate) with each entry's storage slot, a queue of potential
+// entrants in scan order, and one block's entrant c1 values, evictions and
+// slot links.
+struct CapEnt {
+    unsigned long long key;
+    int pos;
+    int slot;
+};
+
+struct WaveCapLds {
+    CapEnt s[LEKF_CAP_LDS];
+    unsigned long long qkey[128];
+    int qpos[128];
+    int c1l[64];
+    int ev[64];
+    int slotn[64];
+    int link[64];
+};
+
+constexpr int kWaveCaps = 2;  // waves of a selection workgroup that cap bucket streams
+
 struct SelLds {
-    CapLds cap;
+    union {
+        CapLds cap;
+        WaveCapLds wcap[kWaveCaps];
+    };
     int seg_beg[LEKF_REG_SEG];
     int seg_pre[LEKF_REG_SEG + 1];
     int wcnt[kSelW][LEKF_REG_NB];
@@ -217,14 +240,15 @@ __device__ __forceinline__ void count_before(const unsigned long long* key, cons
 
 // The reference's cap (scale/letkf/letkf_tools.f90:1041-1156) over one
 // bucket's candidates 0..n-1 in scan order, keyat(i) the priority key of
-// candidate i (smaller first).  Returns the kept count; the kept entries are
-// S[buf][0..kept) with spos = candidate and sslot = storage slot (the output
-// position).  S lives in LDS when kLds (cap <= LEKF_CAP_LDS), else in G.
-// Uniform call by all kSelT threads.
-template <bool kLds, class KeyAt>
+// candidate i (smaller first), by all kSelT threads of a workgroup with the
+// kept set in global memory (caps above LEKF_CAP_LDS).  Returns the kept
+// count; the kept entries are G[buf][0..kept) with pos = candidate and
+// slot = storage slot (the output position).  Uniform call.
+template <class KeyAt>
 __device__ int cap_slots(CapLds& L, const CapGlobal& G, KeyAt keyat, int n, int cap, int tid,
                          int& bufo)
 {
+    constexpr bool kLds = false;
     constexpr int U = kCapU;
     constexpr int V = LEKF_CAP_LDS / kSelT;  // S entries per thread per round of the merge
     int s = 0, buf = 0, par = 0;
@@ -235,12 +259,13 @@ __device__ int cap_slots(CapLds& L, const CapGlobal& G, KeyAt keyat, int n, int 
         xn[u] = i < n ? keyat(i) : 0ull;
     }
     for (int b0 = 0; b0 < n; b0 += kCapB) {
-        unsigned long long* Sk = kLds ? L.skey[buf] : G.key + (size_t)buf * cap;
-        int* Sp = kLds ? L.spos[buf] : G.pos + (size_t)buf * cap;
-        int* Ss = kLds ? L.sslot[buf] : G.slot + (size_t)buf * cap;
-        unsigned long long* Tk = kLds ? L.skey[buf ^ 1] : G.key + (size_t)(buf ^ 1) * cap;
-        int* Tp = kLds ? L.spos[buf ^ 1] : G.pos + (size_t)(buf ^ 1) * cap;
-        int* Ts = kLds ? L.sslot[buf ^ 1] : G.slot + (size_t)(buf ^ 1) * cap;
+        (void)kLds;
+        unsigned long long* Sk = G.key + (size_t)buf * cap;
+        int* Sp = G.pos + (size_t)buf * cap;
+        int* Ss = G.slot + (size_t)buf * cap;
+        unsigned long long* Tk = G.key + (size_t)(buf ^ 1) * cap;
+        int* Tp = G.pos + (size_t)(buf ^ 1) * cap;
+        int* Ts = G.slot + (size_t)(buf ^ 1) * cap;
         unsigned long long x[U];
         bool pot[U];
         const unsigned long long tau = s == cap ? Sk[cap - 1] : 0ull;
@@ -467,28 +492,283 @@ __device__ int cap_slots(CapLds& L, const CapGlobal& G, KeyAt keyat, int n, int 
     return s;
 }
 
-// Cap one candidate stream and write the kept entries in slot order:
-// emit(slot, candidate).  Uniform call; ends with a barrier.
+// Cap one candidate stream (cap above LEKF_CAP_LDS) with the whole workgroup
+// and write the kept entries in slot order: emit(slot, candidate).  Uniform
+// call; ends with a barrier.
 template <class KeyAt, class Emit>
 __device__ int cap_stream(SelLds& S, const CapGlobal& G, KeyAt keyat, Emit emit, int n, int cap,
                           int tid)
 {
-    int buf = 0, kept;
-    if (cap <= LEKF_CAP_LDS) {
-        kept = cap_slots<true>(S.cap, G, keyat, n, cap, tid, buf);
-        for (int r = tid; r < kept; r += kSelT) {
-            const int sl = S.cap.sslot[buf][r];
-            if ((unsigned)sl < (unsigned)kept) emit(sl, S.cap.spos[buf][r]);
-        }
-    } else {
-        kept = cap_slots<false>(S.cap, G, keyat, n, cap, tid, buf);
-        for (int r = tid; r < kept; r += kSelT) {
-            const int sl = G.slot[(size_t)buf * cap + r];
-            if ((unsigned)sl < (unsigned)kept) emit(sl, G.pos[(size_t)buf * cap + r]);
-        }
+    int buf = 0;
+    const int kept = cap_slots(S.cap, G, keyat, n, cap, tid, buf);
+    for (int r = tid; r < kept; r += kSelT) {
+        const int sl = G.slot[(size_t)buf * cap + r];
+        if ((unsigned)sl < (unsigned)kept) emit(sl, G.pos[(size_t)buf * cap + r]);
     }
     __syncthreads();
     return kept;
+}
+
+// ---------------------------------------------------------------------------
+// The cap by one wave (caps up to LEKF_CAP_LDS).  The same merge as
+// cap_slots, over blocks of 64 potential entrants: candidates after the
+// first cap are filtered against the current last entry of S into a queue
+// (the last entry only moves down, so a candidate filtered out can never
+// enter, and a queued one is tested exactly later); each block of 64 queued
+// candidates is sorted across the lanes by (key, candidate), which gives at
+// once each one's count of earlier queued candidates not above it and the
+// entrants' ranks; S is merged in place (every lane reads its 16 entries
+// before any lane writes).  The first cap candidates fill S in scan order and
+// are sorted by a bitonic network in LDS.
+// ---------------------------------------------------------------------------
+
+// order of the lanes' LDS traffic within the wave (no cross-wave sharing)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ bool ent_less(unsigned long long ak, int ap, unsigned long long bk, int bp)
+{
+    return ak < bk || (ak == bk && ap < bp);
+}
+
+// entries of S[0, n) (ascending) before (x, xp)
+__device__ __forceinline__ int wcount_before(const CapEnt* S, int n, unsigned long long x, int xp)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ent_less(S[mid].key, S[mid].pos, x, xp))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ascending bitonic sort of the 64 lanes' (key, id) pairs (ids distinct)
+__device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int lane)
+{
+#pragma unroll
+    for (int sz = 2; sz <= 64; sz <<= 1) {
+#pragma unroll
+        for (int j = sz >> 1; j > 0; j >>= 1) {
+            const unsigned long long ok = __shfl_xor(k, j, 64);
+            const int oid = __shfl_xor(id, j, 64);
+            const bool up = (lane & sz) == 0;
+            const bool lower = (lane & j) == 0;
+            const bool oless = ent_less(ok, oid, k, id);
+            if (lower == up ? oless : !oless) {
+                k = ok;
+                id = oid;
+            }
+        }
+    }
+}
+
+// One block of np (<= 64) queued candidates against the full set S (cap
+// entries): lane l takes queue entry l.  Returns nothing; S, its slots and
+// its last key tau are updated.
+__device__ void wave_cap_block(WaveCapLds& L, int np, int cap, int lane, unsigned long long& tau)
+{
+    const bool valid = lane < np;
+    const unsigned long long y = valid ? L.qkey[lane] : ~0ull;
+    const int yp = valid ? L.qpos[lane] : 0x7fffffff;
+    const bool live = valid && y < tau;  // a queued candidate the threshold has passed stays out
+    const int c1 = live ? wcount_before(L.s, cap, y, yp) : cap;
+    // sort the live candidates by (key, lane): lane L holds the L-th
+    unsigned long long sk = live ? y : ~0ull;
+    int sid = lane;
+    wave_sort64(sk, sid, lane);
+    const bool slive = sk != ~0ull;
+    // earlier (in scan order) live candidates sorted before this one
+    unsigned long long pm = slive ? (1ull << sid) : 0ull;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long t = __shfl_up(pm, d, 64);
+        if (lane >= d) pm |= t;
+    }
+    const unsigned long long before = pm & ~(slive ? (1ull << sid) : 0ull);
+    const int c2 = __popcll(before & (sid == 0 ? 0ull : (~0ull >> (64 - sid))));
+    const int c1s = __shfl(c1, sid, 64);
+    const bool ents = slive && c1s + c2 < cap;
+    const unsigned long long emS = __ballot(ents);
+    const int e = __popcll(emS);
+    if (e == 0) return;
+    const int r = __popcll(emS & lane_lt(lane));  // rank among the entrants
+    const int U = c1s + r;                        // position in S + entrants
+    // scan-order index of each entrant: its flag back to the candidate's lane
+    const int ento = __builtin_amdgcn_ds_permute(sid << 2, ents ? 1 : 0);
+    const unsigned long long emO = __ballot(ento != 0);
+    const int t = __shfl((int)__popcll(emO & lane_lt(lane)), sid, 64);
+    if (ents) L.c1l[r] = c1s;
+    wave_sync();
+    // S entries move up by the entrants before them (entrant q precedes S[i]
+    // when c1l[q] <= i); the top e leave S
+    {
+        constexpr int kPer = LEKF_CAP_LDS / 64;
+        const int i0 = lane * kPer;
+        unsigned long long vk[kPer];
+        int vp[kPer], vs[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int i = i0 + u < cap ? i0 + u : 0;
+            vk[u] = L.s[i].key;
+            vp[u] = L.s[i].pos;
+            vs[u] = L.s[i].slot;
+        }
+        int lo = 0, hi = e;  // entrants with c1 < i0
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L.c1l[mid] < i0)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        int ptr = lo;
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int i = i0 + u;
+            if (i < cap) {
+                while (ptr < e && L.c1l[ptr] <= i) ++ptr;
+                const int ni = i + ptr;
+                if (ni < cap) {
+                    L.s[ni].key = vk[u];
+                    L.s[ni].pos = vp[u];
+                    L.s[ni].slot = vs[u];
+                } else {
+                    L.ev[ni - cap] = vs[u];
+                }
+            }
+        }
+    }
+    if (ents) {
+        if (U < cap) {
+            CapEnt c;
+            c.key = sk;
+            c.pos = L.qpos[sid];
+            c.slot = -1;
+            L.s[U] = c;
+        } else {
+            L.ev[U - cap] = -(t + 1);
+        }
+    }
+    wave_sync();
+    // the i-th entrant in scan order evicts the i-th largest entry evicted
+    // and takes its slot; an entrant evicted in its own block passes its slot
+    // on to the entrant that evicted it (pointer jumping)
+    int sl = -1, ln = -1;
+    if (ents) {
+        const int v = L.ev[e - 1 - t];
+        if (v >= 0)
+            sl = v;
+        else
+            ln = -v - 1;
+        L.slotn[t] = sl;
+        L.link[t] = ln;
+    }
+    wave_sync();
+    for (int round = 0; round < 8 && __ballot(ents && sl < 0); ++round) {
+        int s2 = -1, l2 = -1;
+        if (ents && sl < 0) {
+            s2 = L.slotn[ln];
+            l2 = L.link[ln];
+        }
+        wave_sync();
+        if (ents && sl < 0) {
+            if (s2 >= 0)
+                sl = s2;
+            else
+                ln = l2;
+            L.slotn[t] = sl;
+            L.link[t] = ln;
+        }
+        wave_sync();
+    }
+    if (ents && U < cap) L.s[U].slot = sl;
+    wave_sync();
+    tau = L.s[cap - 1].key;
+}
+
+// The reference's cap over one bucket's candidates 0..n-1 (n > cap) by one
+// wave; then emit(slot, candidate) for the kept entries.  Returns cap.
+template <class KeyAt, class Emit>
+__device__ int wave_cap(WaveCapLds& L, KeyAt keyat, Emit emit, int n, int cap, int lane)
+{
+    // fill: the first cap candidates take slots 0..cap-1 in scan order
+    int F = 64;
+    while (F < cap) F <<= 1;
+    for (int i = lane; i < F; i += 64) {
+        CapEnt c;
+        c.key = i < cap ? keyat(i) : ~0ull;
+        c.pos = i < cap ? i : 0x7fffffff;
+        c.slot = i;
+        L.s[i] = c;
+    }
+    wave_sync();
+    for (int sz = 2; sz <= F; sz <<= 1) {
+        for (int j = sz >> 1; j > 0; j >>= 1) {
+            for (int q = lane; q < F / 2; q += 64) {
+                const int i = (q / j) * 2 * j + (q % j);
+                const CapEnt a = L.s[i], bb = L.s[i + j];
+                const bool up = (i & sz) == 0;
+                if (ent_less(bb.key, bb.pos, a.key, a.pos) == up) {
+                    L.s[i] = bb;
+                    L.s[i + j] = a;
+                }
+            }
+            wave_sync();
+        }
+    }
+    unsigned long long tau = L.s[cap - 1].key;
+    int q = 0;
+    unsigned long long xn = cap + lane < n ? keyat(cap + lane) : 0ull;
+    for (int b0 = cap; b0 < n; b0 += 64) {
+        const unsigned long long x = xn;
+        const int i = b0 + lane;
+        xn = i + 64 < n ? keyat(i + 64) : 0ull;
+        const bool pot = i < n && x < tau;
+        const unsigned long long pm = __ballot(pot);
+        if (pot) {
+            const int o = q + __popcll(pm & lane_lt(lane));
+            L.qkey[o] = x;
+            L.qpos[o] = i;
+        }
+        q += __popcll(pm);
+        wave_sync();
+        if (q >= 64 || (b0 + 64 >= n && q > 0)) {
+            const int nb = q < 64 ? q : 64;
+            wave_cap_block(L, nb, cap, lane, tau);
+            // the rest of the queue moves to its front
+            unsigned long long k2 = 0;
+            int p2 = 0;
+            const bool mv = nb + lane < q;
+            if (mv) {
+                k2 = L.qkey[nb + lane];
+                p2 = L.qpos[nb + lane];
+            }
+            wave_sync();
+            if (mv) {
+                L.qkey[lane] = k2;
+                L.qpos[lane] = p2;
+            }
+            q -= nb;
+            wave_sync();
+            if (b0 + 64 >= n && q > 0) {
+                wave_cap_block(L, q, cap, lane, tau);
+                q = 0;
+            }
+        }
+    }
+    for (int r = lane; r < cap; r += 64) {
+        const int sl = L.s[r].slot;
+        if ((unsigned)sl < (unsigned)cap) emit(sl, L.s[r].pos);
+    }
+    wave_sync();
+    return cap;
 }
 
 // Persistent selection: workgroup g handles points g, g + gridDim.x, ...
@@ -602,7 +882,35 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
             }
         }
         __syncthreads();
-        if (cap > 0) {
+        if (cap > 0 && cap <= LEKF_CAP_LDS) {
+            // wave w caps buckets w, w + kWaveCaps, ... (output offsets from the
+            // counts, which every thread holds)
+            nacc = 0;
+            for (int t = 0; t < nb; ++t) {
+                const int n = bcnt[t];
+                if (wv < kWaveCaps && t % kWaveCaps == wv) {
+                    const size_t so = (size_t)t * (size_t)A.slot_cap;
+                    int* oi = A.loc_idx + obase + nacc;
+                    double* orh = A.loc_rho + obase + nacc;
+                    if (n <= cap) {
+                        for (int i = lane; i < n; i += 64) {
+                            oi[i] = sob[so + i];
+                            orh[i] = srho[so + i];
+                        }
+                    } else {
+                        wave_cap(
+                            S.wcap[wv], [&](int i) { return skey[so + i]; },
+                            [&](int slot, int i) {
+                                oi[slot] = sob[so + i];
+                                orh[slot] = srho[so + i];
+                            },
+                            n, cap, lane);
+                    }
+                }
+                nacc += n < cap ? n : cap;
+            }
+            __syncthreads();
+        } else if (cap > 0) {
             nacc = 0;
             for (int t = 0; t < nb; ++t) {
                 const int n = bcnt[t];
@@ -634,11 +942,51 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
     }
 }
 
-// Nearest-first cap of the global grid's per-point lists (persistent, one
-// point per workgroup): a list longer than the cap keeps the cap entries of
-// largest rho (ties: the earlier list entry) in the reference's storage-slot
-// order, by the cap of scale/letkf/letkf_tools.f90:1041-1156 with the key
-// 2^63 - 1 - bits(rho) (rho > 0; a smaller key is a larger rho).
+// Nearest-first cap of the global grid's per-point lists: a list longer than
+// the cap keeps the cap entries of largest rho (ties: the earlier list entry)
+// in the reference's storage-slot order, by the cap of
+// scale/letkf/letkf_tools.f90:1041-1156 with the key 2^63 - 1 - bits(rho)
+// (rho > 0; a smaller key is a larger rho).  list_cap_wave: one wave per point
+// (persistent), caps up to LEKF_CAP_LDS; list_cap: one workgroup per point.
+__device__ __forceinline__ unsigned long long rho_key(double rho)
+{
+    return 0x7fffffffffffffffull - (unsigned long long)__double_as_longlong(rho);
+}
+
+__global__ __launch_bounds__(64) void list_cap_wave(ListCapArgs A)
+{
+    __shared__ WaveCapLds L;
+    const int lane = threadIdx.x;
+    const long npt = (long)A.ncols * A.nlev;
+    for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
+        const long b = (pi / A.ncols) * (long)A.nij + A.col_begin + pi % A.ncols;
+        const long off = A.in_off[b];
+        const int n = A.in_cnt[b];
+        const long ob = b * (long)A.cap;
+        const int* ii = A.in_idx + off;
+        const double* ir = A.in_rho + off;
+        int kept = n;
+        if (n <= A.cap) {
+            for (int i = lane; i < n; i += 64) {
+                A.out_idx[ob + i] = ii[i];
+                A.out_rho[ob + i] = ir[i];
+            }
+        } else {
+            kept = wave_cap(
+                L, [&](int i) { return rho_key(ir[i]); },
+                [&](int slot, int i) {
+                    A.out_idx[ob + slot] = ii[i];
+                    A.out_rho[ob + slot] = ir[i];
+                },
+                n, A.cap, lane);
+        }
+        if (lane == 0) {
+            A.out_off[b] = ob;
+            A.out_cnt[b] = kept;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
 {
     __shared__ SelLds S;
@@ -663,10 +1011,7 @@ __global__ __launch_bounds__(kSelT) void list_cap(ListCapArgs A)
             const int* ii = A.in_idx + off;
             const double* ir = A.in_rho + off;
             kept = cap_stream(
-                S, G,
-                [&](int i) {
-                    return 0x7fffffffffffffffull - (unsigned long long)__double_as_longlong(ir[i]);
-                },
+                S, G, [&](int i) { return rho_key(ir[i]); },
                 [&](int slot, int i) {
                     A.out_idx[ob + slot] = ii[i];
                     A.out_rho[ob + slot] = ir[i];
@@ -842,6 +1187,18 @@ hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st
 {
     const long npt = (long)A.ncols * A.nlev;
     if (npt <= 0) return hipSuccess;
+    KTime kt("list_cap", st);
+    if (A.cap <= LEKF_CAP_LDS) {
+        // one wave per point: as many as the LDS lets a CU hold
+        int dev = 0, ncu = 256, per_cu = 1;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, list_cap_wave, 64, 0);
+        long g = (long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 256);
+        if (g > npt) g = npt;
+        hipLaunchKernelGGL(list_cap_wave, dim3((unsigned)g), dim3(64), 0, st, A);
+        return hipGetLastError();
+    }
     const long g = npt < nslots ? npt : nslots;
     hipLaunchKernelGGL(list_cap, dim3((unsigned)g), dim3(kSelT), 0, st, A);
     return hipGetLastError();
@@ -851,6 +1208,7 @@ hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, h
 {
     if (npt <= 0) return hipSuccess;
     const long g = npt < nslots ? npt : nslots;
+    KTime kt("select", st);
     hipLaunchKernelGGL(reg_select, dim3((unsigned)g), dim3(kSelT), 0, st, A, npt);
     return hipGetLastError();
 }

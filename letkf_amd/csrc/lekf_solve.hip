@@ -32,6 +32,7 @@
 // same whatever else shares the launch.
 #include "lekf_dev.hpp"
 #include "lekf_tune.hpp"
+#include "lekf_ktime.hpp"
 
 namespace {
 
@@ -1276,20 +1277,23 @@ struct EigLds {
     double* M;    // nvmax  row means
     double* Cw;   // nvmax  X' w-bar
     double* sw;   // k  sqrt of the obs weights (large-k dual form)
+    double* vb;   // 64 Householder vector of a step (n <= 64 reduction)
+    double* wb;   // 64 its w = p - K v
 };
 
 __host__ __device__ inline int eig_lda(int k) { return k | 1; }
 
 __host__ __device__ inline size_t eig_lds_doubles(int k, int nvmax)
 {
-    return (size_t)k * eig_lda(k) + 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
+    return (size_t)k * eig_lda(k) + 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax +
+           128;
 }
 
 // Z in LDS (zg == nullptr) or, for the large-k kernel, the point's workspace
 // record in global memory (zg)
 __host__ __device__ inline size_t eig_lds_doubles_big(int k, int nvmax)
 {
-    return 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax;
+    return 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax + 128;
 }
 
 __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nullptr)
@@ -1326,6 +1330,10 @@ __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nul
     L.Cw = s;
     s += nvmax;
     L.sw = s;
+    s += k;
+    L.vb = s;
+    s += 64;
+    L.wb = s;
     return L;
 }
 
@@ -1447,6 +1455,131 @@ __device__ void eig_tridiag(EigLds& L, int n, int lda, int lane)
     for (int r = lane; r < n; r += 64) {
         A[r] = r == 0 ? 1.0 : 0.0;
         A[r * lda] = r == 0 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+}
+
+// The reduction of eig_tridiag for n <= 64, lane r owning row/column r of the
+// trailing block, with the step's v and w in LDS buffers: every inner loop is
+// a column read plus broadcast reads (no cross-lane shuffles), unrolled with
+// independent loads, and a step's work is its trailing block's.  The matrix-
+// vector product sums in four interleaved partial sums.
+__device__ void eig_tridiag64(EigLds& L, int n, int lda, int lane)
+{
+    double* A = L.Z;
+    double* vb = L.vb;
+    double* wb = L.wb;
+    for (int j = 0; j + 2 < n; ++j) {
+        const int o = j + 1, m = n - o;
+        const double x0 = lane < m ? A[(o + lane) * lda + j] : 0.0;
+        const double sigma = wave_sum(lane > 0 ? x0 * x0 : 0.0);
+        const double alpha = lane_bcast(x0, 0);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (sigma > 0.0) {
+            const double nrm = sqrt(fma(alpha, alpha, sigma));
+            beta = alpha <= 0.0 ? nrm : -nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        double v0 = lane == 0 ? 1.0 : x0 * scal;
+        if (tau == 0.0) v0 = lane == 0 ? 1.0 : 0.0;
+        if (lane == 0) {
+            L.d[j] = A[j * lda + j];
+            L.e[j] = beta;
+            L.tau[j] = tau;
+        }
+        if (lane < m) {
+            A[(o + lane) * lda + j] = v0;
+            vb[lane] = v0;
+        }
+        __syncthreads();
+        if (tau == 0.0) continue;
+        // p = tau A22 v: lane r reads column r (= row r) of the trailing block
+        double p = 0.0;
+        if (lane < m) {
+            const double* col = A + o * lda + o + lane;
+            double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+            int c = 0;
+            for (; c + 4 <= m; c += 4) {
+                p0 = fma(col[c * lda], vb[c], p0);
+                p1 = fma(col[(c + 1) * lda], vb[c + 1], p1);
+                p2 = fma(col[(c + 2) * lda], vb[c + 2], p2);
+                p3 = fma(col[(c + 3) * lda], vb[c + 3], p3);
+            }
+            for (; c < m; ++c) p0 = fma(col[c * lda], vb[c], p0);
+            p = tau * ((p0 + p1) + (p2 + p3));
+        }
+        const double K = 0.5 * tau * wave_sum(lane < m ? v0 * p : 0.0);
+        const double w0 = fma(-K, v0, p);
+        if (lane < m) wb[lane] = w0;
+        __syncthreads();
+        // A22 -= v w^T + w v^T (lane r updates column r; both products are
+        // rounded, then added, so (c, r) and (r, c) stay equal)
+        if (lane < m) {
+            double* col = A + o * lda + o + lane;
+            int c = 0;
+            for (; c + 4 <= m; c += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    col[(c + u) * lda] = rn_sub(col[(c + u) * lda],
+                                                rn_add(rn_mul(vb[c + u], w0), rn_mul(wb[c + u], v0)));
+            }
+            for (; c < m; ++c)
+                col[c * lda] = rn_sub(col[c * lda], rn_add(rn_mul(vb[c], w0), rn_mul(wb[c], v0)));
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        if (n >= 2) {
+            L.d[n - 2] = A[(n - 2) * lda + n - 2];
+            L.e[n - 2] = A[(n - 1) * lda + n - 2];
+        }
+        L.d[n - 1] = A[(n - 1) * lda + n - 1];
+        L.e[n - 1] = 0.0;
+    }
+    __syncthreads();
+    // Q = H_0 H_1 ... H_{n-3}, accumulated from the right end in place (as
+    // eig_tridiag): before step j, row and column j+1 become unit vectors, then
+    // Q[j+1.., j+1..] -= tau v (v^T Q[j+1.., j+1..]).
+    const int t0 = n >= 2 ? n - 2 : 0;
+    if (lane < n - t0)
+        for (int c = 0; c < n - t0; ++c) A[(t0 + c) * lda + t0 + lane] = (lane == c) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int j = n - 3; j >= 0; --j) {
+        const int o = j + 1, m = n - o;
+        if (lane < m) {
+            A[o * lda + o + lane] = lane == 0 ? 1.0 : 0.0;
+            A[(o + lane) * lda + o] = lane == 0 ? 1.0 : 0.0;
+        }
+        const double tau = L.tau[j];
+        if (tau != 0.0 && lane < m) vb[lane] = A[(o + lane) * lda + j];
+        __syncthreads();
+        if (tau == 0.0) continue;
+        if (lane < m) {
+            double* col = A + o * lda + o + lane;
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+            int c = 0;
+            for (; c + 4 <= m; c += 4) {
+                y0 = fma(vb[c], col[c * lda], y0);
+                y1 = fma(vb[c + 1], col[(c + 1) * lda], y1);
+                y2 = fma(vb[c + 2], col[(c + 2) * lda], y2);
+                y3 = fma(vb[c + 3], col[(c + 3) * lda], y3);
+            }
+            for (; c < m; ++c) y0 = fma(vb[c], col[c * lda], y0);
+            const double y = tau * ((y0 + y1) + (y2 + y3));
+            c = 0;
+            for (; c + 4 <= m; c += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    col[(c + u) * lda] = fma(-vb[c + u], y, col[(c + u) * lda]);
+            }
+            for (; c < m; ++c) col[c * lda] = fma(-vb[c], y, col[c * lda]);
+        }
+        __syncthreads();
+    }
+    if (lane < n) {
+        A[lane] = lane == 0 ? 1.0 : 0.0;
+        A[lane * lda] = lane == 0 ? 1.0 : 0.0;
     }
     __syncthreads();
 }
@@ -1915,7 +2048,7 @@ __device__ int eig_ql(EigLds& L, int n, int lda, int lane)
 }
 
 // Householder + implicit-QL eigendecomposition for n <= KR <= 64: the
-// tridiagonalisation in LDS (eig_tridiag), then QL with Q in registers: lane
+// tridiagonalisation in LDS (eig_tridiag64), then QL with Q in registers: lane
 // r holds row r of Q, later of Z (q[t] = Z[r][t]); d_i and e_i live in lane
 // i and are read by the others with readlane; a sweep runs the scalar
 // recurrence in every lane and rotates columns (i, i+1) of the lane's row as
@@ -1996,7 +2129,7 @@ __device__ int eig_reg(EigLds& L, int n, int lda, int lane)
 {
     // Householder in LDS (each step's work is the active block's), then Q's
     // row r into lane r's registers for the QL sweeps
-    eig_tridiag(L, n, lda, lane);
+    eig_tridiag64(L, n, lda, lane);
     double q[KR];
 #pragma unroll
     for (int t = 0; t < KR; ++t) q[t] = (t < n && lane < n) ? L.Z[lane * lda + t] : 0.0;
@@ -2552,8 +2685,24 @@ __device__ void eig_replay(EigLds& L, int n, int lda, int lane, const double* re
         __syncthreads();
         if (lane < n && cnt > 0) {
             // column i+1 takes its final value at step i; column i carries on
+            // (four steps at a time: their loads first, then the chain)
             double h = zr[m];
-            for (int j = 0; j < cnt; ++j) {
+            int j = 0;
+            for (; j + 4 <= cnt; j += 4) {
+                double qi[4], c[4], sn[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    qi[u] = zr[m - 1 - (j + u)];
+                    c[u] = L.rot[2 * (j + u)];
+                    sn[u] = L.rot[2 * (j + u) + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    zr[m - (j + u)] = fma(sn[u], qi[u], c[u] * h);
+                    h = fma(c[u], qi[u], -sn[u] * h);
+                }
+            }
+            for (; j < cnt; ++j) {
                 const int i = m - 1 - j;
                 const double c = L.rot[2 * j], sn = L.rot[2 * j + 1];
                 const double qi = zr[i];
@@ -2782,7 +2931,7 @@ void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
 
 // --- split eigensolver, 32 < k <= 64 --------------------------------------
 // Kernel 1 (one wave per point): C + (k-1)/alpha I into LDS and the
-// Householder tridiagonalisation (eig_tridiag, as the fused kernel does);
+// Householder tridiagonalisation (eig_tridiag64, as the fused kernel does);
 // writes Q (transposed), d and e to the point's record.
 __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict__ jobs, int njobs,
                                                        long gp0, double* __restrict__ ws,
@@ -2808,7 +2957,7 @@ __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict_
     __syncthreads();
     for (int i = lane; i < k; i += 64) L.Z[i * lda + i] += shift;
     __syncthreads();
-    eig_tridiag(L, k, lda, lane);
+    eig_tridiag64(L, k, lda, lane);
     if (lane < k) {
         for (int t = 0; t < k; ++t) rec[S.qoff + (long)t * k + lane] = L.Z[lane * lda + t];
         rec[S.doff + lane] = L.d[lane];
@@ -3030,6 +3179,7 @@ static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int km
     if (kmax > KR) return hipErrorInvalidValue;
     const size_t lds = solve_lds_doubles(kmax, nvmax, PPW) * sizeof(double);
     const long nwg = (npt + PPW - 1) / PPW;
+    KTime kt("solve", st);
     hipLaunchKernelGGL((lekf_solve_kernel<NT, KR, MODE, PPW>), dim3((unsigned)nwg), dim3(LEKF_WAVE),
                        lds, st, djobs, njobs, npt, kmax, nvmax);
     return hipGetLastError();
@@ -3038,6 +3188,7 @@ static hipError_t launch_fused(const LekfJob* djobs, int njobs, long npt, int km
 static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const LekfJob* djobs,
                              int njobs, long g0, double* ws, long gs, int nvmax)
 {
+    KTime kt("eig", st);
     switch (kr) {
 #define LEKF_EIG_CASE(K)                                                                       \
     case K:                                                                                    \
@@ -3068,14 +3219,22 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
     const size_t tlds = eig_lds_doubles(kmax, 0) * sizeof(double);
     switch (kr) {
 #define LEKF_SPLIT_CASE(K)                                                                       \
-    case K:                                                                                      \
-        hipLaunchKernelGGL(lekf_tri_kernel, dim3((unsigned)nb), dim3(LEKF_WAVE), tlds, st, djobs, \
-                           njobs, g0, ws, gs, kmax);                                            \
-        hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs, njobs, \
-                           g0, nb, ws, gs, rot, kmax);                                          \
+    case K: {                                                                                    \
+        {                                                                                        \
+            KTime kt("eig_tri", st);                                                             \
+            hipLaunchKernelGGL(lekf_tri_kernel, dim3((unsigned)nb), dim3(LEKF_WAVE), tlds, st,    \
+                               djobs, njobs, g0, ws, gs, kmax);                                 \
+        }                                                                                        \
+        {                                                                                        \
+            KTime kt("eig_ql", st);                                                              \
+            hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs,    \
+                               njobs, g0, nb, ws, gs, rot, kmax);                               \
+        }                                                                                        \
+        KTime kt("eig_apply", st);                                                               \
         hipLaunchKernelGGL(lekf_eig3_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,    \
                            djobs, njobs, g0, ws, gs, nvmax, kmax, rot);                         \
-        break;
+        break;                                                                                   \
+    }
         LEKF_SPLIT_CASE(36)
         LEKF_SPLIT_CASE(40)
         LEKF_SPLIT_CASE(44)
@@ -3104,15 +3263,18 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
     const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
     for (long g0 = 0; g0 < npt; g0 += per) {
         const long nb = per < npt - g0 ? per : npt - g0;
-        if (W == 2)
-            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 2>), dim3((unsigned)nb), dim3(128), glds,
-                               st, djobs, njobs, g0, (double*)ws, gs);
-        else if (W == 8)
-            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 8>), dim3((unsigned)nb), dim3(512), glds,
-                               st, djobs, njobs, g0, (double*)ws, gs);
-        else
-            hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 4>), dim3((unsigned)nb), dim3(256), glds,
-                               st, djobs, njobs, g0, (double*)ws, gs);
+        {
+            KTime kt("gram", st);
+            if (W == 2)
+                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 2>), dim3((unsigned)nb), dim3(128),
+                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
+            else if (W == 8)
+                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 8>), dim3((unsigned)nb), dim3(512),
+                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
+            else
+                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 4>), dim3((unsigned)nb), dim3(256),
+                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (s3)
@@ -3144,6 +3306,7 @@ static hipError_t launch_big(const LekfJob* djobs, int njobs, long npt, int kmax
     long nwg = (long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 256);
     if (nwg > slots) nwg = slots;
     if (nwg > npt) nwg = npt;
+    KTime kt("big", st);
     hipLaunchKernelGGL((lekf_big_kernel<MODE>), dim3((unsigned)nwg), dim3(LEKF_WAVE), elds, st, djobs,
                        njobs, npt, (double*)ws, gs, nvmax, (long)gram_stride(kmax), gram_nblk(kmax));
     return hipGetLastError();

@@ -134,3 +134,32 @@ def mem_info():
     t = ctypes.c_size_t(0)
     _lib.check(_lib.lib().lekf_mem_info(ctypes.byref(f), ctypes.byref(t)), "mem_info")
     return int(f.value), int(t.value)
+
+
+KTIME_NAMES = ("select", "list_cap", "cull", "solve", "gram", "eig", "eig_tri", "eig_ql",
+               "eig_apply", "big")
+
+
+def ktime_enable(on=True):
+    """Start (clearing the record) or stop recording a HIP event pair around each
+    launch of the library's hot kernels, on the launch's stream."""
+    _lib.check(_lib.lib().lekf_ktime_enable(int(bool(on))), "ktime_enable")
+
+
+def ktime_read(name):
+    """(total ms, launches) of the recorded launches of one kernel name."""
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_long(0)
+    _lib.check(_lib.lib().lekf_ktime_read(name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+               "ktime_read")
+    return float(ms.value), int(n.value)
+
+
+def ktime_all():
+    """{name: (total ms, launches)} of every name recorded."""
+    out = {}
+    for n in KTIME_NAMES:
+        t, c = ktime_read(n)
+        if c:
+            out[n] = (t, c)
+    return out
