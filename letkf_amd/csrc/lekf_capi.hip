@@ -737,6 +737,8 @@ struct lekf_handle {
     DevBuf gws;                      // Gram workspace of the k > 32 solve
     DevBuf sel_key, sel_ob, sel_rho;  // per-slot bucket streams of the capped regional selection
     DevBuf sel_gkey, sel_gpos, sel_gslot;  // per-slot kept sets of caps above LEKF_CAP_LDS
+    // the sorted obs bucket by bucket, for the LDS-resident capped selection
+    DevBuf b_cnt, b_cc, b_ri, b_rj, b_z, b_err, b_o;
     DevBuf maxcand;
     long slot_cap = 0;
     int sel_slots = 0;
@@ -769,7 +771,8 @@ long handle_bytes(const lekf_handle* h)
                           &h->r_dep, &h->r_bkt, &h->r_z, &h->gws, &h->sel_key,
                           &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos, &h->sel_gslot,
                           &h->maxcand, &h->capl_off, &h->capl_cnt,
-                          &h->capl_idx, &h->capl_rho, &h->d_hgt};
+                          &h->capl_idx, &h->capl_rho, &h->d_hgt, &h->b_cnt, &h->b_cc,
+                          &h->b_ri, &h->b_rj, &h->b_z, &h->b_err, &h->b_o};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -789,7 +792,8 @@ void free_all(lekf_handle* h)
                     &h->o_ri, &h->o_rj, &h->o_z, &h->o_bkt, &h->r_dep, &h->r_bkt, &h->r_z,
                     &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos,
                     &h->sel_gslot, &h->maxcand, &h->capl_off,
-                    &h->capl_cnt, &h->capl_idx, &h->capl_rho, &h->d_hgt};
+                    &h->capl_cnt, &h->capl_idx, &h->capl_rho, &h->d_hgt, &h->b_cnt, &h->b_cc,
+                    &h->b_ri, &h->b_rj, &h->b_z, &h->b_err, &h->b_o};
     for (auto* b : bs) b->release();
 }
 
@@ -926,6 +930,12 @@ RegionalArgs make_reg_args(const lekf_handle* h)
     R.oz = h->o_z.as<double>();
     R.oerr = h->o_err.as<double>();
     R.obkt = h->o_bkt.as<int>();
+    R.bcc = h->b_cc.as<int>();
+    R.bri = h->b_ri.as<double>();
+    R.brj = h->b_rj.as<double>();
+    R.bz = h->b_z.as<double>();
+    R.berr = h->b_err.as<double>();
+    R.bo = h->b_o.as<int>();
     R.list_off = h->cap_off.as<long>();
     R.loc_off = h->loc_off.as<long>();
     R.loc_cnt = h->loc_cnt.as<int>();
@@ -1936,6 +1946,20 @@ static int regional_obs_index(lekf_handle* h, int nobs, bool has_qc, int qc_keep
     h->nbinned = (int)nbin;
     h->nobs = nobs;
     HIPCHK(lekf_reg_scatter_launch(A, h->nbinned, ncell, h->cellcum.as<int>(), st));
+    if (lekf_reg_select_fused(h->r.nobs_max)) {
+        const size_t nb1 = (size_t)(h->nbinned > 0 ? h->nbinned : 1);
+        HIPCHK(h->b_cnt.ensure(sizeof(int) * (size_t)h->nbkt * ncell));
+        HIPCHK(h->b_cc.ensure(sizeof(int) * (size_t)h->nbkt * ncell));
+        HIPCHK(h->b_ri.ensure(sizeof(double) * nb1));
+        HIPCHK(h->b_rj.ensure(sizeof(double) * nb1));
+        HIPCHK(h->b_z.ensure(sizeof(double) * nb1));
+        HIPCHK(h->b_err.ensure(sizeof(double) * nb1));
+        HIPCHK(h->b_o.ensure(sizeof(int) * nb1));
+        HIPCHK(lekf_reg_buckets_launch(A, h->nbinned, h->cellcum.as<int>(), h->nbkt,
+                                       h->b_cnt.as<int>(), h->b_cc.as<int>(), h->b_ri.as<double>(),
+                                       h->b_rj.as<double>(), h->b_z.as<double>(),
+                                       h->b_err.as<double>(), h->b_o.as<int>(), st));
+    }
     // list capacity per point -> chunks of points whose lists fit the budget
     const long npt = (long)h->nij * h->nlev;
     HIPCHK(h->cap_n.ensure(sizeof(int) * npt));
@@ -1969,7 +1993,8 @@ static int regional_obs_index(lekf_handle* h, int nobs, bool has_qc, int qc_keep
     // capped selection: per workgroup slot one stream per bucket, each as
     // long as the largest rectangle's candidate count
     h->sel_slots = lekf_reg_select_slots_max();
-    h->slot_cap = h->r.nobs_max > 0 ? std::max(maxcand, 1) : 0;
+    h->slot_cap = h->r.nobs_max > 0 && !lekf_reg_select_fused(h->r.nobs_max)
+                      ? std::max(maxcand, 1) : 0;
     if (h->slot_cap > 0) {
         const size_t ne_ = (size_t)h->sel_slots * (size_t)h->nbkt * (size_t)h->slot_cap;
         HIPCHK(h->sel_key.ensure(sizeof(unsigned long long) * ne_));

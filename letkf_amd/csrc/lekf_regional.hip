@@ -79,33 +79,8 @@ struct CapGlobal {
     int* slot;
 };
 
-// One wave's cap (caps up to LEKF_CAP_LDS): the kept set S ascending by
-// (key, candidate) with each entry's storage slot, a queue of potential
-// entrants in scan order, and one block's entrant c1 values, evictions and
-// slot links.
-struct CapEnt {
-    unsigned long long key;
-    int pos;
-    int slot;
-};
-
-struct WaveCapLds {
-    CapEnt s[LEKF_CAP_LDS];
-    unsigned long long qkey[128];
-    int qpos[128];
-    int c1l[64];
-    int ev[64];
-    int slotn[64];
-    int link[64];
-};
-
-constexpr int kWaveCaps = 2;  // waves of a selection workgroup that cap bucket streams
-
 struct SelLds {
-    union {
-        CapLds cap;
-        WaveCapLds wcap[kWaveCaps];
-    };
+    CapLds cap;
     int seg_beg[LEKF_REG_SEG];
     int seg_pre[LEKF_REG_SEG + 1];
     int wcnt[kSelW][LEKF_REG_NB];
@@ -510,16 +485,27 @@ __device__ int cap_stream(SelLds& S, const CapGlobal& G, KeyAt keyat, Emit emit,
 }
 
 // ---------------------------------------------------------------------------
-// The cap by one wave (caps up to LEKF_CAP_LDS).  The same merge as
-// cap_slots, over blocks of 64 potential entrants: candidates after the
-// first cap are filtered against the current last entry of S into a queue
-// (the last entry only moves down, so a candidate filtered out can never
-// enter, and a queued one is tested exactly later); each block of 64 queued
-// candidates is sorted across the lanes by (key, candidate), which gives at
-// once each one's count of earlier queued candidates not above it and the
-// entrants' ranks; S is merged in place (every lane reads its 16 entries
-// before any lane writes).  The first cap candidates fill S in scan order and
-// are sorted by a bitonic network in LDS.
+// The cap by one wave (caps up to LEKF_CAP_LDS), fed chunk by chunk with the
+// candidates of one bucket stream in scan order.
+//
+// The kept set S (the cap smallest (key, scan position) of the stream so
+// far) is held ascending in key[], with each entry's storage slot in slot[];
+// so[s] is the candidate in storage slot s (an obs or list position, which
+// increases in scan order).  The first cap candidates fill slots 0..cap-1
+// and S is then sorted once.  A later candidate can enter only if its key is
+// below the key tau of S's last entry (an earlier candidate with an equal key
+// precedes it), and tau only falls, so candidates are filtered against tau
+// into a queue and each block of 64 queued candidates is merged at once:
+// * a candidate enters iff fewer than cap of S and of the earlier queued
+//   candidates precede it; all of S precedes it in scan order, so its count
+//   in S is the number of keys <= its key, and the earlier queued candidates
+//   before it come from a 64-lane sort by (key, lane);
+// * the entries of S move up by the entrants before them, and the top e of
+//   S and the entrants leave;
+// * the i-th entrant in scan order takes the slot of the i-th largest entry
+//   that left: the reference evicts the current last entry each time, and
+//   the last key only falls; an entrant that leaves in its own block hands
+//   its slot on (pointer jumping).
 // ---------------------------------------------------------------------------
 
 // order of the lanes' LDS traffic within the wave (no cross-wave sharing)
@@ -529,24 +515,24 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ bool ent_less(unsigned long long ak, int ap, unsigned long long bk, int bp)
-{
-    return ak < bk || (ak == bk && ap < bp);
-}
+constexpr int kWcPer = LEKF_CAP_LDS / 64;  // entries of S per lane in the merge
 
-// entries of S[0, n) (ascending) before (x, xp)
-__device__ __forceinline__ int wcount_before(const CapEnt* S, int n, unsigned long long x, int xp)
-{
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ent_less(S[mid].key, S[mid].pos, x, xp))
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
+struct WaveCap {
+    unsigned long long key[LEKF_CAP_LDS];
+    unsigned short slot[LEKF_CAP_LDS];
+    int so[LEKF_CAP_LDS];
+    unsigned long long qkey[128];  // queued candidates, scan order
+    int qo[128];
+    int ev[64];                    // entries that left a block, ascending: slot, or -(entrant + 1)
+    int slotn[64];
+    int link[64];
+};
+
+struct WcState {
+    int n;                   // stream candidates so far
+    int q;                   // queued candidates
+    unsigned long long tau;  // key of S's last entry (once n >= cap)
+};
 
 // ascending bitonic sort of the 64 lanes' (key, id) pairs (ids distinct)
 __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int lane)
@@ -559,7 +545,7 @@ __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int 
             const int oid = __shfl_xor(id, j, 64);
             const bool up = (lane & sz) == 0;
             const bool lower = (lane & j) == 0;
-            const bool oless = ent_less(ok, oid, k, id);
+            const bool oless = ok < k || (ok == k && oid < id);
             if (lower == up ? oless : !oless) {
                 k = ok;
                 id = oid;
@@ -568,28 +554,71 @@ __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int 
     }
 }
 
-// One block of np (<= 64) queued candidates against the full set S (cap
-// entries): lane l takes queue entry l.  Returns nothing; S, its slots and
-// its last key tau are updated.
-__device__ void wave_cap_block(WaveCapLds& L, int np, int cap, int lane, unsigned long long& tau)
+// the first cap stream candidates (slots 0..cap-1 in scan order) sorted by
+// (key, slot); entries up to the next power of two are padding
+__device__ void wc_sort_fill(WaveCap& L, int cap, int lane)
+{
+    int F = 64, lf = 6;
+    while (F < cap) {
+        F <<= 1;
+        ++lf;
+    }
+    for (int i = cap + lane; i < F; i += 64) {
+        L.key[i] = ~0ull;
+        L.slot[i] = 0xffff;
+    }
+    wave_sync();
+    for (int ls = 1; ls <= lf; ++ls) {
+        for (int lj = ls - 1; lj >= 0; --lj) {
+            const int j = 1 << lj;
+            for (int q = lane; q < (F >> 1); q += 64) {
+                const int i = ((q >> lj) << (lj + 1)) | (q & (j - 1));
+                const unsigned long long ka = L.key[i], kb = L.key[i + j];
+                const int sa = L.slot[i], sb = L.slot[i + j];
+                const bool up = ((i >> ls) & 1) == 0;
+                const bool bless = kb < ka || (kb == ka && sb < sa);
+                if (bless == up) {
+                    L.key[i] = kb;
+                    L.key[i + j] = ka;
+                    L.slot[i] = (unsigned short)sb;
+                    L.slot[i + j] = (unsigned short)sa;
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+// One block: queued candidates 0..np-1 (np <= 64) against the full set S.
+__device__ void wc_block(WaveCap& L, int np, int cap, int lane, unsigned long long& tau)
 {
     const bool valid = lane < np;
     const unsigned long long y = valid ? L.qkey[lane] : ~0ull;
-    const int yp = valid ? L.qpos[lane] : 0x7fffffff;
-    const bool live = valid && y < tau;  // a queued candidate the threshold has passed stays out
-    const int c1 = live ? wcount_before(L.s, cap, y, yp) : cap;
+    const bool live = valid && y < tau;  // tau has fallen below it since it was queued: out
+    int c1 = cap;
+    if (live) {
+        int lo = 0, hi = cap;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L.key[mid] <= y)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        c1 = lo;
+    }
     // sort the live candidates by (key, lane): lane L holds the L-th
     unsigned long long sk = live ? y : ~0ull;
     int sid = lane;
     wave_sort64(sk, sid, lane);
     const bool slive = sk != ~0ull;
-    // earlier (in scan order) live candidates sorted before this one
     unsigned long long pm = slive ? (1ull << sid) : 0ull;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const unsigned long long t = __shfl_up(pm, d, 64);
         if (lane >= d) pm |= t;
     }
+    // earlier (in scan order) live candidates sorted before this one
     const unsigned long long before = pm & ~(slive ? (1ull << sid) : 0ull);
     const int c2 = __popcll(before & (sid == 0 ? 0ull : (~0ull >> (64 - sid))));
     const int c1s = __shfl(c1, sid, 64);
@@ -603,63 +632,49 @@ __device__ void wave_cap_block(WaveCapLds& L, int np, int cap, int lane, unsigne
     const int ento = __builtin_amdgcn_ds_permute(sid << 2, ents ? 1 : 0);
     const unsigned long long emO = __ballot(ento != 0);
     const int t = __shfl((int)__popcll(emO & lane_lt(lane)), sid, 64);
-    if (ents) L.c1l[r] = c1s;
+    // merge: lane takes the entries u * 64 + lane; every lane reads its
+    // entries before any lane writes
+    unsigned long long vk[kWcPer];
+    int vs[kWcPer];
+#pragma unroll
+    for (int u = 0; u < kWcPer; ++u) {
+        const int i = u * 64 + lane;
+        const int ii = i < cap ? i : 0;
+        vk[u] = L.key[ii];
+        vs[u] = L.slot[ii];
+    }
     wave_sync();
-    // S entries move up by the entrants before them (entrant q precedes S[i]
-    // when c1l[q] <= i); the top e leave S
-    {
-        constexpr int kPer = LEKF_CAP_LDS / 64;
-        const int i0 = lane * kPer;
-        unsigned long long vk[kPer];
-        int vp[kPer], vs[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int i = i0 + u < cap ? i0 + u : 0;
-            vk[u] = L.s[i].key;
-            vp[u] = L.s[i].pos;
-            vs[u] = L.s[i].slot;
+    for (int u = 0; u < kWcPer; ++u) {
+        const int lo = u * 64;
+        if (lo >= cap) break;
+        // entrants before entry i = lo + lane: those with c1 <= i
+        int ptr = __popcll(__ballot(ents && c1s < lo));
+        unsigned long long wm = __ballot(ents && c1s >= lo && c1s < lo + 64);
+        while (wm) {
+            const int bl = __builtin_ctzll(wm);
+            wm &= wm - 1;
+            const int d = __builtin_amdgcn_readlane(c1s, bl) - lo;
+            ptr += lane >= d ? 1 : 0;
         }
-        int lo = 0, hi = e;  // entrants with c1 < i0
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (L.c1l[mid] < i0)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        int ptr = lo;
-        wave_sync();
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int i = i0 + u;
-            if (i < cap) {
-                while (ptr < e && L.c1l[ptr] <= i) ++ptr;
-                const int ni = i + ptr;
-                if (ni < cap) {
-                    L.s[ni].key = vk[u];
-                    L.s[ni].pos = vp[u];
-                    L.s[ni].slot = vs[u];
-                } else {
-                    L.ev[ni - cap] = vs[u];
-                }
+        const int i = lo + lane;
+        if (i < cap) {
+            const int ni = i + ptr;
+            if (ni < cap) {
+                L.key[ni] = vk[u];
+                L.slot[ni] = (unsigned short)vs[u];
+            } else {
+                L.ev[ni - cap] = vs[u];
             }
         }
     }
     if (ents) {
-        if (U < cap) {
-            CapEnt c;
-            c.key = sk;
-            c.pos = L.qpos[sid];
-            c.slot = -1;
-            L.s[U] = c;
-        } else {
+        if (U < cap)
+            L.key[U] = sk;
+        else
             L.ev[U - cap] = -(t + 1);
-        }
     }
     wave_sync();
-    // the i-th entrant in scan order evicts the i-th largest entry evicted
-    // and takes its slot; an entrant evicted in its own block passes its slot
-    // on to the entrant that evicted it (pointer jumping)
     int sl = -1, ln = -1;
     if (ents) {
         const int v = L.ev[e - 1 - t];
@@ -688,87 +703,77 @@ __device__ void wave_cap_block(WaveCapLds& L, int np, int cap, int lane, unsigne
         }
         wave_sync();
     }
-    if (ents && U < cap) L.s[U].slot = sl;
+    if (ents && U < cap && sl >= 0) {
+        L.slot[U] = (unsigned short)sl;
+        L.so[sl] = L.qo[sid];
+    }
     wave_sync();
-    tau = L.s[cap - 1].key;
+    tau = L.key[cap - 1];
 }
 
-// The reference's cap over one bucket's candidates 0..n-1 (n > cap) by one
-// wave; then emit(slot, candidate) for the kept entries.  Returns cap.
-template <class KeyAt, class Emit>
-__device__ int wave_cap(WaveCapLds& L, KeyAt keyat, Emit emit, int n, int cap, int lane)
+// One chunk of the stream: lane l's position is a stream candidate iff ok
+// (key k, candidate o); positions are in scan order across the lanes.
+__device__ void wc_push(WaveCap& L, WcState& S, bool ok, unsigned long long k, int o, int cap,
+                        int lane)
 {
-    // fill: the first cap candidates take slots 0..cap-1 in scan order
-    int F = 64;
-    while (F < cap) F <<= 1;
-    for (int i = lane; i < F; i += 64) {
-        CapEnt c;
-        c.key = i < cap ? keyat(i) : ~0ull;
-        c.pos = i < cap ? i : 0x7fffffff;
-        c.slot = i;
-        L.s[i] = c;
-    }
-    wave_sync();
-    for (int sz = 2; sz <= F; sz <<= 1) {
-        for (int j = sz >> 1; j > 0; j >>= 1) {
-            for (int q = lane; q < F / 2; q += 64) {
-                const int i = (q / j) * 2 * j + (q % j);
-                const CapEnt a = L.s[i], bb = L.s[i + j];
-                const bool up = (i & sz) == 0;
-                if (ent_less(bb.key, bb.pos, a.key, a.pos) == up) {
-                    L.s[i] = bb;
-                    L.s[i + j] = a;
-                }
-            }
-            wave_sync();
+    const unsigned long long am = __ballot(ok);
+    if (!am) return;
+    const int rank = __popcll(am & lane_lt(lane));
+    const int n0 = S.n;
+    S.n += __popcll(am);
+    bool pend = ok;
+    if (n0 < cap) {
+        if (ok && n0 + rank < cap) {
+            const int f = n0 + rank;
+            L.key[f] = k;
+            L.slot[f] = (unsigned short)f;
+            L.so[f] = o;
+            pend = false;
         }
-    }
-    unsigned long long tau = L.s[cap - 1].key;
-    int q = 0;
-    unsigned long long xn = cap + lane < n ? keyat(cap + lane) : 0ull;
-    for (int b0 = cap; b0 < n; b0 += 64) {
-        const unsigned long long x = xn;
-        const int i = b0 + lane;
-        xn = i + 64 < n ? keyat(i + 64) : 0ull;
-        const bool pot = i < n && x < tau;
-        const unsigned long long pm = __ballot(pot);
-        if (pot) {
-            const int o = q + __popcll(pm & lane_lt(lane));
-            L.qkey[o] = x;
-            L.qpos[o] = i;
-        }
-        q += __popcll(pm);
+        if (S.n < cap) return;
         wave_sync();
-        if (q >= 64 || (b0 + 64 >= n && q > 0)) {
-            const int nb = q < 64 ? q : 64;
-            wave_cap_block(L, nb, cap, lane, tau);
-            // the rest of the queue moves to its front
-            unsigned long long k2 = 0;
-            int p2 = 0;
-            const bool mv = nb + lane < q;
-            if (mv) {
-                k2 = L.qkey[nb + lane];
-                p2 = L.qpos[nb + lane];
-            }
-            wave_sync();
-            if (mv) {
-                L.qkey[lane] = k2;
-                L.qpos[lane] = p2;
-            }
-            q -= nb;
-            wave_sync();
-            if (b0 + 64 >= n && q > 0) {
-                wave_cap_block(L, q, cap, lane, tau);
-                q = 0;
-            }
-        }
+        wc_sort_fill(L, cap, lane);
+        S.tau = L.key[cap - 1];
+        if (S.n == cap) return;
     }
-    for (int r = lane; r < cap; r += 64) {
-        const int sl = L.s[r].slot;
-        if ((unsigned)sl < (unsigned)cap) emit(sl, L.s[r].pos);
+    const bool pot = pend && k < S.tau;
+    const unsigned long long pm = __ballot(pot);
+    if (!pm) return;
+    if (pot) {
+        const int qp = S.q + __popcll(pm & lane_lt(lane));
+        L.qkey[qp] = k;
+        L.qo[qp] = o;
     }
+    S.q += __popcll(pm);
     wave_sync();
-    return cap;
+    if (S.q >= 64) {
+        wc_block(L, 64, cap, lane, S.tau);
+        // the rest of the queue moves to its front
+        const bool mv = 64 + lane < S.q;
+        unsigned long long k2 = 0;
+        int o2 = 0;
+        if (mv) {
+            k2 = L.qkey[64 + lane];
+            o2 = L.qo[64 + lane];
+        }
+        wave_sync();
+        if (mv) {
+            L.qkey[lane] = k2;
+            L.qo[lane] = o2;
+        }
+        S.q -= 64;
+        wave_sync();
+    }
+}
+
+// End of the stream: the last queued block.  Returns the kept count; the
+// kept candidates are so[0 .. kept) in slot order.
+__device__ int wc_finish(WaveCap& L, WcState& S, int cap, int lane)
+{
+    if (S.q > 0) wc_block(L, S.q, cap, lane, S.tau);
+    S.q = 0;
+    wave_sync();
+    return S.n < cap ? S.n : cap;
 }
 
 // Persistent selection: workgroup g handles points g, g + gridDim.x, ...
@@ -882,35 +887,7 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
             }
         }
         __syncthreads();
-        if (cap > 0 && cap <= LEKF_CAP_LDS) {
-            // wave w caps buckets w, w + kWaveCaps, ... (output offsets from the
-            // counts, which every thread holds)
-            nacc = 0;
-            for (int t = 0; t < nb; ++t) {
-                const int n = bcnt[t];
-                if (wv < kWaveCaps && t % kWaveCaps == wv) {
-                    const size_t so = (size_t)t * (size_t)A.slot_cap;
-                    int* oi = A.loc_idx + obase + nacc;
-                    double* orh = A.loc_rho + obase + nacc;
-                    if (n <= cap) {
-                        for (int i = lane; i < n; i += 64) {
-                            oi[i] = sob[so + i];
-                            orh[i] = srho[so + i];
-                        }
-                    } else {
-                        wave_cap(
-                            S.wcap[wv], [&](int i) { return skey[so + i]; },
-                            [&](int slot, int i) {
-                                oi[slot] = sob[so + i];
-                                orh[slot] = srho[so + i];
-                            },
-                            n, cap, lane);
-                    }
-                }
-                nacc += n < cap ? n : cap;
-            }
-            __syncthreads();
-        } else if (cap > 0) {
+        if (cap > 0) {
             nacc = 0;
             for (int t = 0; t < nb; ++t) {
                 const int n = bcnt[t];
@@ -942,6 +919,172 @@ __global__ __launch_bounds__(kSelT) void reg_select(RegionalArgs A, long npt)
     }
 }
 
+// Capped selection with the kept sets in LDS (0 < nobs_max <= LEKF_CAP_LDS):
+// persistent, one point per workgroup, wave t caps bucket t's stream (NW =
+// nbkt waves).  A bucket's candidates in the point's rectangle are, row by
+// row, one contiguous range of the bucket-by-bucket obs copy, in the
+// reference's scan order restricted to the bucket; each is evaluated once
+// (eval_rec) and fed to the wave's cap.  The lists are written bucket by
+// bucket in slot order, rho recomputed from the kept obs.
+struct RowTab {
+    int beg[64];
+    int pre[65];
+};
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long npt)
+{
+    __shared__ WaveCap L[NW];
+    __shared__ RowTab R[NW];
+    __shared__ int kept_s[NW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nij = A.nx * A.ny;
+    const int cap = A.nobs_max;
+    const int* cc = A.bcc + (size_t)wv * nij;
+    int bs = 0;  // bucket wv's first position in the bucket-by-bucket copy
+    for (int u = 0; u < wv; ++u) bs += A.bcc[(size_t)u * nij + nij - 1];
+    WaveCap& W = L[wv];
+    RowTab& T = R[wv];
+
+    for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
+        const long b = A.pt_begin + pi;  // global point = lev * nij + ij
+        const int lev = (int)(b / nij), ij = (int)(b % nij);
+        const double ri = (double)(ij % A.nx + 1), rj = (double)(ij / A.nx + 1), rz = A.zlev[lev];
+        int imin = (int)floor(ri - A.dlon_zero), imax = (int)ceil(ri + A.dlon_zero);
+        int jmin = (int)floor(rj - A.dlat_zero), jmax = (int)ceil(rj + A.dlat_zero);
+        if (imin < 1) imin = 1;
+        if (imax > A.nx) imax = A.nx;
+        if (jmin < 1) jmin = 1;
+        if (jmax > A.ny) jmax = A.ny;
+        WcState S{0, 0, ~0ull};
+        if (imin <= imax) {
+            for (int j0 = jmin; j0 <= jmax; j0 += 64) {
+                const int nr = jmax - j0 + 1 < 64 ? jmax - j0 + 1 : 64;
+                int beg = 0, len = 0;
+                if (lane < nr) {
+                    const int rowbase = (j0 + lane - 1) * A.nx;
+                    const int lo = rowbase + imin - 1, hi = rowbase + imax - 1;
+                    const int c0 = lo > 0 ? cc[lo - 1] : 0;
+                    beg = bs + c0;
+                    len = cc[hi] - c0;
+                }
+                int pre = len;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int v = __shfl_up(pre, d, 64);
+                    if (lane >= d) pre += v;
+                }
+                T.beg[lane] = beg;
+                T.pre[lane + 1] = pre;
+                if (lane == 0) T.pre[0] = 0;
+                const int tot = __shfl(pre, 63, 64);
+                wave_sync();
+                // candidate q of the row group -> its position; loads of the next
+                // chunk are issued before the current one is evaluated
+                int r0 = 0;
+                ObsRec nrec;
+                int no = -1;
+                auto fetch = [&](int qb) {
+                    while (r0 < nr && T.pre[r0 + 1] <= qb) ++r0;
+                    const int q = qb + lane;
+                    no = -1;
+                    if (q < tot) {
+                        int r = r0;
+                        while (T.pre[r + 1] <= q) ++r;
+                        const int p = T.beg[r] + (q - T.pre[r]);
+                        nrec.ri = A.bri[p];
+                        nrec.rj = A.brj[p];
+                        nrec.z = A.bz[p];
+                        nrec.err = A.berr[p];
+                        nrec.b = wv;
+                        no = A.bo[p];
+                    }
+                };
+                fetch(0);
+                for (int q0 = 0; q0 < tot; q0 += 64) {
+                    const ObsRec cur = nrec;
+                    const int co = no;
+                    if (q0 + 64 < tot)
+                        fetch(q0 + 64);
+                    else
+                        no = -1;
+                    Cand c;
+                    c.ok = false;
+                    c.bits = 0;
+                    if (co >= 0) c = eval_rec(A, cur, co, ri, rj, rz);
+                    wc_push(W, S, c.ok, c.bits, co, cap, lane);
+                }
+                wave_sync();
+            }
+        }
+        const int kept = wc_finish(W, S, cap, lane);
+        if (lane == 0) kept_s[wv] = kept;
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int u = 0; u < NW; ++u) {
+            const int k = kept_s[u];
+            off += u < wv ? k : 0;
+            tot += k;
+        }
+        const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
+        for (int sl = lane; sl < kept; sl += 64) {
+            const int o = W.so[sl];
+            const Cand c = eval_rec(A, load_obs(A, o), o, ri, rj, rz);
+            A.loc_idx[obase + off + sl] = o;
+            A.loc_rho[obase + off + sl] = c.rho;
+        }
+        if (threadIdx.x == 0) {
+            A.loc_off[b] = obase;
+            A.loc_cnt[b] = tot;
+        }
+        __syncthreads();
+    }
+}
+
+// The bucket-by-bucket copy of the sorted obs: per cell, its obs of each
+// bucket (count), then each bucket's inclusive scan over cells, then the copy
+// (every cell's obs in order, each to its bucket's next position).
+__global__ void reg_bucket_count(const int* cellcum, const int* obkt, int ncell, int nbkt,
+                                 int* bcnt)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int n[LEKF_REG_NB] = {0, 0, 0, 0};
+    const int beg = c > 0 ? cellcum[c - 1] : 0, end = cellcum[c];
+    for (int i = beg; i < end; ++i) {
+        const int t = obkt[i];
+        if (t >= 0 && t < LEKF_REG_NB) ++n[t];
+    }
+    for (int t = 0; t < nbkt; ++t) bcnt[(size_t)t * ncell + c] = n[t];
+}
+
+__global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const double* orj,
+                                   const double* oz, const double* oerr, const int* obkt,
+                                   int ncell, int nbkt, const int* bcc, double* bri, double* brj,
+                                   double* bz, double* berr, int* bo)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int pos[LEKF_REG_NB] = {0, 0, 0, 0};
+    int bs = 0;
+    for (int t = 0; t < nbkt; ++t) {
+        const int* cc = bcc + (size_t)t * ncell;
+        pos[t] = bs + (c > 0 ? cc[c - 1] : 0);
+        bs += cc[ncell - 1];
+    }
+    const int beg = c > 0 ? cellcum[c - 1] : 0, end = cellcum[c];
+    for (int i = beg; i < end; ++i) {
+        const int t = obkt[i];
+        if (t < 0 || t >= nbkt) continue;
+        const int p = pos[t]++;
+        bri[p] = ori[i];
+        brj[p] = orj[i];
+        bz[p] = oz[i];
+        berr[p] = oerr[i];
+        bo[p] = i;
+    }
+}
+
 // Nearest-first cap of the global grid's per-point lists: a list longer than
 // the cap keeps the cap entries of largest rho (ties: the earlier list entry)
 // in the reference's storage-slot order, by the cap of
@@ -955,7 +1098,7 @@ __device__ __forceinline__ unsigned long long rho_key(double rho)
 
 __global__ __launch_bounds__(64) void list_cap_wave(ListCapArgs A)
 {
-    __shared__ WaveCapLds L;
+    __shared__ WaveCap L;
     const int lane = threadIdx.x;
     const long npt = (long)A.ncols * A.nlev;
     for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
@@ -972,13 +1115,21 @@ __global__ __launch_bounds__(64) void list_cap_wave(ListCapArgs A)
                 A.out_rho[ob + i] = ir[i];
             }
         } else {
-            kept = wave_cap(
-                L, [&](int i) { return rho_key(ir[i]); },
-                [&](int slot, int i) {
-                    A.out_idx[ob + slot] = ii[i];
-                    A.out_rho[ob + slot] = ir[i];
-                },
-                n, A.cap, lane);
+            WcState S{0, 0, ~0ull};
+            double rn = lane < n ? ir[lane] : 0.0;
+            for (int i0 = 0; i0 < n; i0 += 64) {
+                const double rc = rn;
+                const int i = i0 + lane;
+                rn = i + 64 < n ? ir[i + 64] : 0.0;
+                wc_push(L, S, i < n, rho_key(rc), i, A.cap, lane);
+            }
+            kept = wc_finish(L, S, A.cap, lane);
+            for (int sl = lane; sl < kept; sl += 64) {
+                const int i = L.so[sl];
+                A.out_idx[ob + sl] = ii[i];
+                A.out_rho[ob + sl] = ir[i];
+            }
+            wave_sync();
         }
         if (lane == 0) {
             A.out_off[b] = ob;
@@ -1204,11 +1355,52 @@ hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st
     return hipGetLastError();
 }
 
+template <int NW>
+static hipError_t launch_select_wave(const RegionalArgs& A, long npt, hipStream_t st)
+{
+    // as many workgroups as the LDS lets the CUs hold
+    int dev = 0, ncu = 256, per_cu = 1;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reg_select_wave<NW>, 64 * NW, 0);
+    long g = (long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 256);
+    if (g > npt) g = npt;
+    hipLaunchKernelGGL(reg_select_wave<NW>, dim3((unsigned)g), dim3(64 * NW), 0, st, A, npt);
+    return hipGetLastError();
+}
+
 hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st)
 {
     if (npt <= 0) return hipSuccess;
-    const long g = npt < nslots ? npt : nslots;
     KTime kt("select", st);
+    if (lekf_reg_select_fused(A.nobs_max) && A.bcc) {
+        switch (A.nbkt) {
+        case 1: return launch_select_wave<1>(A, npt, st);
+        case 2: return launch_select_wave<2>(A, npt, st);
+        case 3: return launch_select_wave<3>(A, npt, st);
+        case 4: return launch_select_wave<4>(A, npt, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    const long g = npt < nslots ? npt : nslots;
     hipLaunchKernelGGL(reg_select, dim3((unsigned)g), dim3(kSelT), 0, st, A, npt);
+    return hipGetLastError();
+}
+
+hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
+                                   int* bcnt, int* bcc, double* bri, double* brj, double* bz,
+                                   double* berr, int* bo, hipStream_t st)
+{
+    (void)nbinned;
+    const int ncell = A.nx * A.ny;
+    if (ncell <= 0 || nbkt < 1 || nbkt > LEKF_REG_NB) return hipErrorInvalidValue;
+    const int nb = (ncell + 255) / 256;
+    hipLaunchKernelGGL(reg_bucket_count, dim3(nb), dim3(256), 0, st, cellcum, A.o_bkt, ncell, nbkt,
+                       bcnt);
+    for (int t = 0; t < nbkt; ++t)
+        hipLaunchKernelGGL(scan_inclusive_int, dim3(1), dim3(1024), 0, st, bcnt + (size_t)t * ncell,
+                           bcc + (size_t)t * ncell, ncell);
+    hipLaunchKernelGGL(reg_bucket_scatter, dim3(nb), dim3(256), 0, st, cellcum, A.o_ri, A.o_rj,
+                       A.o_z, A.o_err, A.o_bkt, ncell, nbkt, bcc, bri, brj, bz, berr, bo);
     return hipGetLastError();
 }

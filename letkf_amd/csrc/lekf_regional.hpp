@@ -32,6 +32,13 @@ struct RegionalArgs {
     const int* cellcum;                     // nx * ny inclusive cumulative obs counts, (j, i) order
     const double *ori, *orj, *oz, *oerr;    // sorted obs
     const int* obkt;
+    // the sorted obs again, bucket by bucket (each bucket's obs in the sorted
+    // order): bcc[t * nx * ny + c] = inclusive count of bucket t's obs through
+    // cell c; bucket t starts at the sum of the earlier buckets' totals; bo =
+    // the obs's position in the sorted order
+    const int* bcc;
+    const double *bri, *brj, *bz, *berr;
+    const int* bo;
     long pt_begin;                          // first global point of this launch
     const long* list_off;                   // per point list offsets (global numbering)
     long list_base;                         // list_off of the launch's first point
@@ -79,6 +86,13 @@ struct ListCapArgs {
 hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st);
 
 hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
+// the bucket-by-bucket copy of the sorted obs (RegionalArgs bcc, bri .. bo):
+// bcnt (nbkt * ncell ints) is work space
+hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
+                                   int* bcnt, int* bcc, double* bri, double* brj, double* bz,
+                                   double* berr, int* bo, hipStream_t st);
+// the capped selection keeps its kept sets in LDS, one wave per bucket
+inline bool lekf_reg_select_fused(int nobs_max) { return nobs_max > 0 && nobs_max <= LEKF_CAP_LDS; }
 hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
                                    hipStream_t st);
 // per point list capacity (rectangle count, capped) and its exclusive scan;
