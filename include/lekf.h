@@ -414,6 +414,10 @@ int lekf_event_sync(void *ev);
 #define LEKF_TUNE_EIG_SPLIT 3  /* 32 < k <= 64: 1 = tridiagonalisation, QL (one lane per point)
                                   and eigenvector/apply kernels, 0 = one fused eigen kernel;
                                   both give bit-identical results */
+#define LEKF_TUNE_SPLIT_RCAP 4 /* rotations a point's QL stream may hold in the three-kernel
+                                  eigensolver; a point whose stream is longer is solved by
+                                  the fused kernel's steps instead (bit-identical); 0: the
+                                  default 2k^2 + 8k (smaller values test that fallback) */
 int lekf_tune_set(int param, long value);
 long lekf_tune_get(int param);
 

@@ -48,6 +48,7 @@ LEKF_TUNE_GRAM_WAVES = 0
 LEKF_TUNE_GRAM_WS_MB = 1
 LEKF_TUNE_SEL_PER_CU = 2
 LEKF_TUNE_EIG_SPLIT = 3
+LEKF_TUNE_SPLIT_RCAP = 4
 LEKF_NOALLOC_BYTES = 256 << 20
 LEKF_NOALLOC_MAX_JOBS = 1024
 LEKF_PT_NOCONV = 2

@@ -429,6 +429,10 @@ int lekf_tune_set(int param, long value)
         if (value != 0 && value != 1) return LEKF_ERR_ARG;
         g_lekf_tune.eig_split = (int)value;
         break;
+    case LEKF_TUNE_SPLIT_RCAP:
+        if (value < 0) return LEKF_ERR_ARG;
+        g_lekf_tune.split_rcap = value;
+        break;
     default: return LEKF_ERR_ARG;
     }
     g_tune_epoch.fetch_add(1);
@@ -499,6 +503,7 @@ long lekf_tune_get(int param)
     case LEKF_TUNE_GRAM_WS_MB: return (long)(g_lekf_tune.gram_ws_bytes >> 20);
     case LEKF_TUNE_SEL_PER_CU: return g_lekf_tune.sel_per_cu;
     case LEKF_TUNE_EIG_SPLIT: return g_lekf_tune.eig_split;
+    case LEKF_TUNE_SPLIT_RCAP: return g_lekf_tune.split_rcap;
     default: return -LEKF_ERR_ARG;
     }
 }

@@ -945,6 +945,7 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
     for (int u = 0; u < wv; ++u) bs += A.bcc[(size_t)u * nij + nij - 1];
     WaveCap& W = L[wv];
     RowTab& T = R[wv];
+    const double r2cut = kNdMax * A.sigma_h * A.sigma_h * (1.0 + 1e-9);
 
     for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
         const long b = A.pt_begin + pi;  // global point = lev * nij + ij
@@ -962,11 +963,29 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
                 const int nr = jmax - j0 + 1 < 64 ? jmax - j0 + 1 : 64;
                 int beg = 0, len = 0;
                 if (lane < nr) {
-                    const int rowbase = (j0 + lane - 1) * A.nx;
-                    const int lo = rowbase + imin - 1, hi = rowbase + imax - 1;
-                    const int c0 = lo > 0 ? cc[lo - 1] : 0;
-                    beg = bs + c0;
-                    len = cc[hi] - c0;
+                    // the row's cells that can hold an obs within the cutoff: a
+                    // cell's obs lie on its side of the faces at i +- 0.5, so one
+                    // whose nearest face is beyond the cutoff radius holds none
+                    // (nd >= nd_h^2 > kNdMax); a 1e-9 margin covers rounding
+                    const int j = j0 + lane;
+                    const double dyg = fmax(0.0, fabs((double)j - rj) - 0.5) * A.dy;
+                    const double rem = r2cut - dyg * dyg;
+                    int ilo = imin, ihi = imax;
+                    if (rem < 0.0) {
+                        ihi = ilo - 1;
+                    } else {
+                        const double rx = sqrt(rem) / A.dx + 0.5;
+                        const int a = (int)ceil(ri - rx), z = (int)floor(ri + rx);
+                        ilo = a > imin ? a : imin;
+                        ihi = z < imax ? z : imax;
+                    }
+                    if (ilo <= ihi) {
+                        const int rowbase = (j - 1) * A.nx;
+                        const int lo = rowbase + ilo - 1, hi = rowbase + ihi - 1;
+                        const int c0 = lo > 0 ? cc[lo - 1] : 0;
+                        beg = bs + c0;
+                        len = cc[hi] - c0;
+                    }
                 }
                 int pre = len;
 #pragma unroll

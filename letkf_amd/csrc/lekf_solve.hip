@@ -2748,8 +2748,10 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
 
     if (p > 0) {
         const double* g = rec;
-        // the split solve's result: rotations replayed unless the stream overflowed
+        // the split solve's result: rotations replayed; a point whose stream
+        // overflowed is left to lekf_eig_fix_kernel
         const bool replay = REPLAY && !(((int)rec[SR->flagoff]) & kSplitOverflow);
+        if (REPLAY && !replay) return;
         if (!BIG && !replay)
             for (int t = lane; t < k * k; t += 64) {
                 const int i = t / k, j = t - i * k;
@@ -2777,7 +2779,7 @@ __device__ void eig_point(const LekfJob& J, int b, double* const rec, long gstri
         } else if constexpr (BIG > 0) {
             big_tridiag<(BIG > 0 ? BIG : 1)>(L.Z, k, lda, L.d, L.e, L.tau, lane);
             if (eig_ql<BIG>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
-        } else if constexpr (KR > 0) {
+        } else if constexpr (KR > 0 && !REPLAY) {
             if (eig_reg<(KR > 0 ? KR : 4)>(L, k, lda, lane)) status |= LEKF_PT_NOCONV;
         } else {
             eig_tridiag(L, k, lda, lane);
@@ -2975,7 +2977,7 @@ template <int KR>
 __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__ jobs, int njobs,
                                                       long gp0, long nb, double* __restrict__ ws,
                                                       long gstride, double* __restrict__ rot,
-                                                      int kmax)
+                                                      int kmax, long rlim)
 {
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
@@ -2988,8 +2990,10 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
     if (p <= 0) return;
     double* rec = ws + pt * gstride;
-    const long rcap = split_rcap(kmax);
-    double* rs = rot + pt * 2 * rcap;
+    // a stream holds up to rlim rotations (at most its stride split_rcap)
+    const long stride = split_rcap(kmax);
+    const long rcap = rlim > 0 && rlim < stride ? rlim : stride;
+    double* rs = rot + pt * 2 * stride;
     const SplitRec S = split_rec(kmax, nullptr);
     double* d = sm + lane;
     double* e = sm + (long)KR * 64 + lane;
@@ -3064,7 +3068,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
 
 // Kernel 3 (one wave per point): the eigenvectors by replaying the rotations
 // on Q, then the W assembly, inflation, outputs and the fused apply of
-// eig_point.  A point whose stream overflowed is solved again in full.
+// eig_point.  A point whose stream overflowed is left to lekf_eig_fix_kernel.
 template <int KR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(eig_waves_per_simd(KR))))
 void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
@@ -3077,6 +3081,26 @@ void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
     const SplitRec S = split_rec(kmax, rot + (gp - gp0) * 2 * split_rcap(kmax));
     eig_point<KR, 0, true>(J, (int)(gp - J.pt_begin + J.pt0), ws + (gp - gp0) * gstride, gstride,
                            nvmax, 0L, smem, threadIdx.x, &S);
+}
+
+// The points of a split launch whose rotation stream overflowed: the fused
+// eigen kernel's work (from the Gram still at the start of the record), the
+// others return at once.
+template <int KR>
+__global__ __launch_bounds__(64) void lekf_eig_fix_kernel(const LekfJob* __restrict__ jobs,
+                                                           int njobs, long gp0,
+                                                           double* __restrict__ ws, long gstride,
+                                                           int nvmax, int kmax)
+{
+    extern __shared__ double smem[];
+    const long gp = gp0 + blockIdx.x;
+    const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
+    double* rec = ws + (gp - gp0) * gstride;
+    const SplitRec S = split_rec(kmax, nullptr);
+    if (p <= 0 || !(((int)rec[S.flagoff]) & kSplitOverflow)) return;
+    eig_point<KR, 0>(J, b, rec, gstride, nvmax, 0L, smem, threadIdx.x);
 }
 
 // LEKF_KSPLIT < k: one persistent wave per workgroup slot strides over the
@@ -3228,11 +3252,13 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
         {                                                                                        \
             KTime kt("eig_ql", st);                                                              \
             hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs,    \
-                               njobs, g0, nb, ws, gs, rot, kmax);                               \
+                               njobs, g0, nb, ws, gs, rot, kmax, g_lekf_tune.split_rcap);       \
         }                                                                                        \
         KTime kt("eig_apply", st);                                                               \
         hipLaunchKernelGGL(lekf_eig3_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,    \
                            djobs, njobs, g0, ws, gs, nvmax, kmax, rot);                         \
+        hipLaunchKernelGGL(lekf_eig_fix_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
+                           djobs, njobs, g0, ws, gs, nvmax, kmax);                              \
         break;                                                                                   \
     }
         LEKF_SPLIT_CASE(36)

@@ -8,6 +8,8 @@ struct LekfTune {
     size_t gram_ws_bytes = 4ull << 30;  // Gram workspace per launch (points per sub-batch)
     int sel_per_cu = 3;                 // selection workgroups (scratch slots) per CU: 1..4
     int eig_split = 1;                  // 32 < k <= 64: three-kernel eigensolver (1) or fused (0)
+    long split_rcap = 0;                // rotations a split stream holds before its point is
+                                        // solved in full by the fallback (0: 2k^2 + 8k)
 };
 
 extern LekfTune g_lekf_tune;

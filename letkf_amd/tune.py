@@ -36,12 +36,22 @@ PARAMS = {
     "eig_split": (_lib.LEKF_TUNE_EIG_SPLIT, (1, 0)),
 }
 
+# knobs that are set by hand, never searched: the rotations a point's QL stream
+# may hold in the three-kernel eigensolver before the point takes the fused
+# kernel's steps instead (0: the default 2k^2 + 8k)
+FIXED = {"split_rcap": _lib.LEKF_TUNE_SPLIT_RCAP}
+
+
+def _pid(name):
+    return PARAMS[name][0] if name in PARAMS else FIXED[name]
+
+
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "letkf_amd", "tune.json")
 
 
 def get(name):
     """Current value of a tuning parameter."""
-    v = _lib.lib().lekf_tune_get(PARAMS[name][0])
+    v = _lib.lib().lekf_tune_get(_pid(name))
     if v < 0:
         raise _lib.LetkfError(-v, "tune_get")
     return int(v)
@@ -49,7 +59,7 @@ def get(name):
 
 def set(name, value):  # noqa: A001 - mirrors lekf_tune_set
     """Set a tuning parameter (raises LetkfError for a value out of range)."""
-    _lib.check(_lib.lib().lekf_tune_set(PARAMS[name][0], int(value)), "tune_set")
+    _lib.check(_lib.lib().lekf_tune_set(_pid(name), int(value)), "tune_set")
 
 
 def current():

@@ -96,6 +96,24 @@ def test_eig_split_bit_identical(gpu, mk):
     _eq(_analysis(job), a1)
 
 
+@pytest.mark.parametrize("mk", [_radar, _global_capped])
+def test_split_overflow_fallback_bit_identical(gpu, mk):
+    """A point whose QL rotation stream outgrows its cap is solved by the fused
+    kernel's steps (lekf_eig_fix_kernel) from the Gram still in its record:
+    with a cap of one rotation every point with obs takes that path, with 900
+    some do; the analysis stays bit-identical to the default cap."""
+    job = mk()
+    tune.set("eig_split", 1)
+    tune.set("split_rcap", 0)
+    a1 = _analysis(job)
+    try:
+        for cap in (1, 900):
+            tune.set("split_rcap", cap)
+            _eq(_analysis(job), a1)
+    finally:
+        tune.set("split_rcap", 0)
+
+
 @pytest.mark.parametrize("waves", [2, 8])
 def test_gram_waves_agree(gpu, waves):
     job = _radar()
