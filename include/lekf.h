@@ -406,18 +406,20 @@ int lekf_event_sync(void *ev);
 /* launch tuning (process-wide; read when a launch is issued).  The values
  * change kernel shapes only: the workspace size and the selection slot count
  * leave every result bit-identical, the Gram wave count changes the order of
- * the partial-sum reduction only.  Handles recapture their graphs after a
+ * the partial-sum reduction only, the eigensolver choice agrees to rounding.  Handles recapture their graphs after a
  * change.  lekf_tune_get returns the value, or a negative error code. */
 #define LEKF_TUNE_GRAM_WAVES 0 /* waves per point of the k > 32 Gram kernel: 2, 4 or 8 */
 #define LEKF_TUNE_GRAM_WS_MB 1 /* Gram workspace per launch, MiB, 16 .. 65536 */
 #define LEKF_TUNE_SEL_PER_CU 2 /* capped-selection workgroups per CU: 1 .. 4 */
 #define LEKF_TUNE_EIG_SPLIT 3  /* 32 < k <= 64: 1 = tridiagonalisation, QL (one lane per point)
-                                  and eigenvector/apply kernels, 0 = one fused eigen kernel;
-                                  both give bit-identical results */
+                                  and apply kernels (the apply pushes the state rows through
+                                  Q and the recorded rotations instead of forming the
+                                  eigenvectors, unless trans or pao is asked for), 0 = one
+                                  fused eigen kernel; the two agree to rounding */
 #define LEKF_TUNE_SPLIT_RCAP 4 /* rotations a point's QL stream may hold in the three-kernel
                                   eigensolver; a point whose stream is longer is solved by
-                                  the fused kernel's steps instead (bit-identical); 0: the
-                                  default 2k^2 + 8k (smaller values test that fallback) */
+                                  the fused kernel's steps instead; 0: the default 2k^2 + 8k
+                                  (smaller values test that fallback) */
 int lekf_tune_set(int param, long value);
 long lekf_tune_get(int param);
 

@@ -931,6 +931,8 @@ struct RowTab {
     int pre[65];
 };
 
+constexpr int kTileX = 16, kTileY = 8;  // columns of a selection tile
+
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long npt)
 {
@@ -947,10 +949,30 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
     RowTab& T = R[wv];
     const double r2cut = kNdMax * A.sigma_h * A.sigma_h * (1.0 + 1e-9);
 
-    for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
-        const long b = A.pt_begin + pi;  // global point = lev * nij + ij
-        const int lev = (int)(b / nij), ij = (int)(b % nij);
-        const double ri = (double)(ij % A.nx + 1), rj = (double)(ij / A.nx + 1), rz = A.zlev[lev];
+    // XCD-aware tile order: workgroup w runs on XCD w % 8 (round-robin
+    // dispatch); the XCD's workgroups take the points of one kTileX x kTileY
+    // block of columns at one level together, so the obs of the block's
+    // rectangles stay in that XCD's L2.  Group x (x < ng = min(8, grid)) takes
+    // the tiles t = x (mod ng), its workgroups the tile's points in turn.
+    const int ng = gridDim.x < 8 ? (int)gridDim.x : 8;
+    const int xcd = (int)blockIdx.x % ng, wj = (int)blockIdx.x / ng;
+    const int nwg = ((int)gridDim.x - xcd + ng - 1) / ng;
+    const int tnx = (A.nx + kTileX - 1) / kTileX, tny = (A.ny + kTileY - 1) / kTileY;
+    const long tpl = (long)tnx * tny;  // tiles per level
+    const long bend = A.pt_begin + npt;
+    const long t_first = (A.pt_begin / nij) * tpl, t_last = ((bend - 1) / nij + 1) * tpl - 1;
+    const long t0 = t_first + ((xcd - t_first % ng) + ng) % ng;
+    for (long q = wj;; q += nwg) {
+        const long t = t0 + (q / (kTileX * kTileY)) * ng;
+        if (t > t_last) break;
+        const int w = (int)(q % (kTileX * kTileY));
+        const int lev = (int)(t / tpl), tl = (int)(t % tpl);
+        const int ix = (tl % tnx) * kTileX + w % kTileX, iy = (tl / tnx) * kTileY + w / kTileX;
+        if (ix >= A.nx || iy >= A.ny) continue;
+        const int ij = iy * A.nx + ix;
+        const long b = (long)lev * nij + ij;  // global point
+        if (b < A.pt_begin || b >= bend) continue;
+        const double ri = (double)(ix + 1), rj = (double)(iy + 1), rz = A.zlev[lev];
         int imin = (int)floor(ri - A.dlon_zero), imax = (int)ceil(ri + A.dlon_zero);
         int jmin = (int)floor(rj - A.dlat_zero), jmax = (int)ceil(rj + A.dlat_zero);
         if (imin < 1) imin = 1;

@@ -2634,7 +2634,7 @@ __device__ void big_dual_point(const LekfJob& J, int b, double* rec, double* rbu
 // rotation stream of a point holds, per QL sweep, a header (m, count) and the
 // sweep's (c, s) pairs for i = m-1 down to m-count.
 struct SplitRec {
-    long qoff, doff, eoff, lamoff, cntoff, flagoff;
+    long qoff, doff, eoff, lamoff, cntoff, flagoff, qroff;
     const double* rot;  // this point's rotation stream
     long rcap;          // its capacity in (c, s) entries
 };
@@ -2643,9 +2643,12 @@ constexpr int kSplitOverflow = 8;  // flag: the stream overflowed (the point is 
 __host__ __device__ inline long split_qoff(int kmax) { return gram_stride(kmax); }
 __host__ __device__ inline long split_stride(int kmax)
 {
-    return (gram_stride(kmax) + (long)kmax * kmax + 3L * kmax + 2 + 1) & ~1L;
+    return (gram_stride(kmax) + 2L * kmax * kmax + 3L * kmax + 2 + 1) & ~1L;
 }
 __host__ __device__ inline long split_rcap(int kmax) { return 2L * kmax * kmax + 8L * kmax; }
+// per point: the column of each rotation-stream entry (-1: a sweep header),
+// after the streams, in doubles
+__host__ __device__ inline long split_ridx_doubles(int kmax) { return (split_rcap(kmax) + 3) / 4; }
 
 __device__ inline SplitRec split_rec(int kmax, const double* rot)
 {
@@ -2656,6 +2659,7 @@ __device__ inline SplitRec split_rec(int kmax, const double* rot)
     S.lamoff = S.eoff + kmax;
     S.cntoff = S.lamoff + kmax;
     S.flagoff = S.cntoff + 1;
+    S.qroff = S.flagoff + 1;  // Q again, row-major (Q[i][j] at qroff + i * kmax + j)
     S.rot = rot;
     S.rcap = split_rcap(kmax);
     return S;
@@ -2961,7 +2965,11 @@ __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict_
     __syncthreads();
     eig_tridiag64(L, k, lda, lane);
     if (lane < k) {
-        for (int t = 0; t < k; ++t) rec[S.qoff + (long)t * k + lane] = L.Z[lane * lda + t];
+        for (int t = 0; t < k; ++t) {
+            const double z = L.Z[lane * lda + t];
+            rec[S.qoff + (long)t * k + lane] = z;
+            rec[S.qroff + (long)lane * k + t] = z;
+        }
         rec[S.doff + lane] = L.d[lane];
         rec[S.eoff + lane] = L.e[lane];
     }
@@ -2977,7 +2985,8 @@ template <int KR>
 __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__ jobs, int njobs,
                                                       long gp0, long nb, double* __restrict__ ws,
                                                       long gstride, double* __restrict__ rot,
-                                                      int kmax, long rlim)
+                                                      short* __restrict__ ridx, int kmax,
+                                                      long rlim)
 {
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
@@ -2994,6 +3003,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
     const long stride = split_rcap(kmax);
     const long rcap = rlim > 0 && rlim < stride ? rlim : stride;
     double* rs = rot + pt * 2 * stride;
+    short* rx = ridx + pt * stride;
     const SplitRec S = split_rec(kmax, nullptr);
     double* d = sm + lane;
     double* e = sm + (long)KR * 64 + lane;
@@ -3046,6 +3056,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
                 if (nr < rcap) {
                     rs[2 * nr] = c;
                     rs[2 * nr + 1] = sn;
+                    rx[nr] = (short)i;
                 }
                 ++nr;
                 ++cnt;
@@ -3058,6 +3069,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
             if (hpos < rcap) {
                 rs[2 * hpos] = (double)m;
                 rs[2 * hpos + 1] = (double)cnt;
+                rx[hpos] = -1;
             }
         }
     }
@@ -3066,9 +3078,9 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
     rec[S.flagoff] = (double)(bad | (nr > rcap ? kSplitOverflow : 0));
 }
 
-// Kernel 3 (one wave per point): the eigenvectors by replaying the rotations
-// on Q, then the W assembly, inflation, outputs and the fused apply of
-// eig_point.  A point whose stream overflowed is left to lekf_eig_fix_kernel.
+// Kernel 3 for jobs that output trans or pao (one wave per point): the
+// eigenvectors by replaying the rotations on Q, then the W assembly,
+// inflation, outputs and the fused apply of eig_point.  A point whose stream overflowed is left to lekf_eig_fix_kernel.
 template <int KR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(eig_waves_per_simd(KR))))
 void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
@@ -3078,9 +3090,244 @@ void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
     extern __shared__ double smem[];
     const long gp = gp0 + blockIdx.x;
     const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
+    if (!J.trans && !J.pao) return;  // lekf_evf_kernel's
     const SplitRec S = split_rec(kmax, rot + (gp - gp0) * 2 * split_rcap(kmax));
     eig_point<KR, 0, true>(J, (int)(gp - J.pt_begin + J.pt0), ws + (gp - gp0) * gstride, gstride,
                            nvmax, 0L, smem, threadIdx.x, &S);
+}
+
+// Kernel 3 for the other jobs (no trans or pao output): the analysis without
+// forming the eigenvectors.  With A = Q T Q^T (kernel 1) and T = R L R^T (the
+// recorded rotations, R = G_1 G_2 ...), V = Q R, and every quantity the apply
+// needs is a row vector pushed through Q, R, a diagonal, R^T and Q^T:
+//   u = L^-1 V^T b             : row b^T -> b^T Q -> (.) R -> / lambda
+//   X'V diag(U) V^T            : row x   -> x Q   -> (.) R -> * U -> (.) R^T -> (.) Q^T
+//   X' w-bar = (X'V) . u       : the row after R, dotted with u
+//   w-bar = V u                : u -> (.) R^T -> (.) Q^T
+// (U = sqrt((k-1)/lambda), common/common_letkf.f90:119-147).  A rotation costs
+// 6 flops per row instead of 6k for the whole of V, so G = nvmax + 1 lanes
+// serve one point (its state rows, then b) and a wave takes 64 / G points;
+// each lane keeps its row in LDS and follows its own point's rotation
+// stream (column of each entry from the QL kernel, -1 for a sweep header),
+// forward for R and backward with the transposed rotations for R^T.
+// Results agree with the explicit form to rounding.
+constexpr int kEvfChunk = 24;  // rotation-stream entries loaded at once per lane
+
+template <int KR>
+__global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict__ jobs, int njobs,
+                                                       long gp0, long nb,
+                                                       const double* __restrict__ ws, long gstride,
+                                                       const double* __restrict__ rot,
+                                                       const short* __restrict__ ridx, int nvmax,
+                                                       int kmax)
+{
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x;
+    const int G = nvmax + 1, P = 64 / G;
+    const int grp = lane / G, v = lane - grp * G;
+    const long pt = (long)blockIdx.x * P + grp;
+    const int ld = kmax | 1;
+    double* row = sm + (size_t)lane * ld;
+    const long rstride = split_rcap(kmax);
+    const SplitRec S = split_rec(kmax, nullptr);
+
+    // this lane's point and row
+    const LekfJob* Jp = nullptr;
+    int b = 0, k = 0, p = 0, nrow = 0, n3 = 0, lev = 0, ij = 0;
+    bool work = false;  // a point this kernel solves
+    if (grp < P && pt < nb) {
+        const long gp = gp0 + pt;
+        Jp = &jobs[lekf_job_of(jobs, njobs, gp)];
+        b = (int)(gp - Jp->pt_begin + Jp->pt0);
+        k = Jp->k;
+        p = Jp->loc_cnt ? Jp->loc_cnt[b] : (int)(Jp->loc_off[b + 1] - Jp->loc_off[b]);
+        work = !Jp->trans && !Jp->pao;
+        if (work && p > 0 && (((int)ws[pt * gstride + S.flagoff]) & kSplitOverflow)) work = false;
+        if ((Jp->flags & LEKF_F_APPLY)) {
+            lev = b / Jp->nij;
+            ij = b - lev * Jp->nij;
+            n3 = Jp->nv3d;
+            nrow = n3 + (lev == 0 ? Jp->nv2d : 0);
+        }
+    }
+    const bool isb = work && v == nvmax;           // the lane of b
+    const bool isx = work && v < nrow;             // a state row
+    const double* rec = ws + pt * gstride;
+    const double* xsrc = nullptr;
+    double* xdst = nullptr;
+    bool upd = true;
+    if (isx) {
+        const LekfJob& J = *Jp;
+        xsrc = (v < n3) ? J.x3b + ((size_t)b * n3 + v) * k : J.x2b + ((size_t)ij * J.nv2d + (v - n3)) * k;
+        xdst = (v < n3) ? J.x3a + ((size_t)b * n3 + v) * k : J.x2a + ((size_t)ij * J.nv2d + (v - n3)) * k;
+        const uint32_t mask = J.upd_mask ? J.upd_mask[lev] : 0xffffffffu;
+        upd = (v >= n3) || ((mask >> v) & 1u);
+    }
+    double alpha = 1.0;
+    if (work) alpha = Jp->infl_in ? Jp->infl_in[b] : Jp->infl_fixed;
+    // the row's mean in member order (speedy/common/common_speedy.f90:335-370)
+    double mean = 0.0;
+    if (isx) {
+        double sx = xsrc[0];
+        for (int m = 1; m < k; ++m) sx += xsrc[m];
+        mean = sx / (double)k;
+    }
+    const bool solve = work && p > 0;
+    if (work && p <= 0) {
+        // no local obs (common_letkf.f90:82-96): T = sqrt(alpha) I
+        const double sa = sqrt(alpha);
+        if (isx)
+            for (int m = 0; m < k; ++m) {
+                const double xm = xsrc[m] - mean;
+                xdst[m] = upd ? mean + sa * xm : mean + xm;
+            }
+        if (isb) {
+            const LekfJob& J = *Jp;
+            if (J.transm)
+                for (int i = 0; i < k; ++i) J.transm[(size_t)b * k + i] = 0.0;
+            if (J.infl_out) J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL) ? J.infl_fixed : alpha;
+            if (J.status) J.status[b] = 0;
+        }
+    }
+    const bool act = solve && (isx || isb);
+    // row <- x Q (Q row-major at qroff)
+    if (act) {
+        if (isx)
+            for (int m = 0; m < k; ++m) row[m] = xsrc[m] - mean;
+        else
+            for (int i = 0; i < k; ++i) row[i] = rec[(size_t)k * k + i];
+        double y[KR];
+#pragma unroll
+        for (int t = 0; t < KR; ++t) y[t] = 0.0;
+        const double* qr = rec + S.qroff;
+        for (int l = 0; l < k; ++l) {
+            const double xl = row[l];
+            const double* ql = qr + (size_t)l * k;
+#pragma unroll
+            for (int t = 0; t < KR; ++t)
+                if (t < k) y[t] = fma(xl, ql[t], y[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < KR; ++t)
+            if (t < k) row[t] = y[t];
+    }
+    // forward through R: entry e rotates columns (i, i+1), (c, s) as the
+    // replay applies it
+    const long ntot = act ? (long)rec[S.cntoff] : 0;
+    const double* rs = rot + pt * 2 * rstride;
+    const short* rx = ridx + pt * rstride;
+    long emax = ntot;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const long t = __shfl_xor(emax, o, 64);
+        emax = t > emax ? t : emax;
+    }
+    // the stream is read kEvfChunk entries at a time, all loads issued before
+    // the chunk's rotations run
+    for (long e0 = 0; e0 < emax; e0 += kEvfChunk) {
+        double cc[kEvfChunk], ss[kEvfChunk];
+        int ii[kEvfChunk];
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const long e = e0 + u;
+            const bool ok = e < ntot;
+            ii[u] = ok ? (int)rx[e] : -1;
+            cc[u] = ok ? rs[2 * e] : 0.0;
+            ss[u] = ok ? rs[2 * e + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const int i = ii[u];
+            if (i >= 0) {
+                const double a0 = row[i], a1 = row[i + 1];
+                row[i] = fma(cc[u], a0, -ss[u] * a1);
+                row[i + 1] = fma(ss[u], a0, cc[u] * a1);
+            }
+        }
+    }
+    __syncthreads();
+    // u = (b^T V) / lambda in b's lane; truncation test on lambda
+    double lmax = 0.0, lmin = 1.0e308;
+    if (act)
+        for (int t = 0; t < k; ++t) {
+            const double lt = rec[S.lamoff + t];
+            lmax = lt > lmax ? lt : lmax;
+            lmin = lt < lmin ? lt : lmin;
+            if (isb) row[t] = row[t] / lt;
+        }
+    __syncthreads();
+    // X' w-bar = (x V) . u, then x V diag(U)
+    double cw = 0.0;
+    if (act && isx) {
+        const double* u = sm + (size_t)(grp * G + nvmax) * ld;
+        for (int t = 0; t < k; ++t) cw = fma(row[t], u[t], cw);
+        for (int t = 0; t < k; ++t) row[t] = row[t] * sqrt((double)(k - 1) / rec[S.lamoff + t]);
+    }
+    __syncthreads();
+    // backward through R^T: the entries in reverse, each rotation transposed
+    for (long e0 = emax > 0 ? ((emax - 1) / kEvfChunk) * kEvfChunk : -1; e0 >= 0;
+         e0 -= kEvfChunk) {
+        double cc[kEvfChunk], ss[kEvfChunk];
+        int ii[kEvfChunk];
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const long e = e0 + u;
+            const bool ok = e < ntot;
+            ii[u] = ok ? (int)rx[e] : -1;
+            cc[u] = ok ? rs[2 * e] : 0.0;
+            ss[u] = ok ? rs[2 * e + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = kEvfChunk - 1; u >= 0; --u) {
+            const int i = ii[u];
+            if (i >= 0) {
+                const double a0 = row[i], a1 = row[i + 1];
+                row[i] = fma(cc[u], a0, ss[u] * a1);
+                row[i + 1] = fma(-ss[u], a0, cc[u] * a1);
+            }
+        }
+    }
+    // out = row Q^T (Q column-major at qoff: Q[m][t] at qoff + t * k + m)
+    if (act) {
+        double out[KR];
+#pragma unroll
+        for (int m = 0; m < KR; ++m) out[m] = 0.0;
+        const double* qc = rec + S.qoff;
+        for (int t = 0; t < k; ++t) {
+            const double zt = row[t];
+            const double* qt = qc + (size_t)t * k;
+#pragma unroll
+            for (int m = 0; m < KR; ++m)
+                if (m < k) out[m] = fma(zt, qt[m], out[m]);
+        }
+        const LekfJob& J = *Jp;
+        if (isx) {
+#pragma unroll
+            for (int m = 0; m < KR; ++m)
+                if (m < k) xdst[m] = upd ? mean + (cw + out[m]) : mean + (xsrc[m] - mean);
+        } else {
+            // w-bar, inflation, status
+            if (J.transm)
+#pragma unroll
+                for (int i = 0; i < KR; ++i)
+                    if (i < k) J.transm[(size_t)b * k + i] = out[i];
+            if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
+            if (J.infl_out) {
+                double tr = 0.0;
+                for (int i = 0; i < k; ++i) tr += rec[(size_t)i * k + i];
+                const double p1 = rec[(size_t)k * k + k], p3 = rec[(size_t)k * k + k + 1];
+                J.infl_out[b] = (J.flags & LEKF_F_FIXED_INFL)
+                                    ? J.infl_fixed
+                                    : adaptive_alpha(alpha, p1, tr / (double)(k - 1), p3);
+            }
+            if (J.status) {
+                int status = (((int)rec[S.flagoff]) & 1) ? LEKF_PT_NOCONV : 0;
+                if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08)
+                    status |= LEKF_PT_RANKDEF;
+                J.status[b] = status;
+            }
+        }
+    }
 }
 
 // The points of a split launch whose rotation stream overflowed: the fused
@@ -3177,7 +3424,8 @@ static size_t ws_rec_bytes(int kmax)
 {
     if (kmax > LEKF_KSPLIT) return (size_t)big_stride(kmax) * sizeof(double);
     if (split_path(kmax))
-        return (size_t)(split_stride(kmax) + 2 * split_rcap(kmax)) * sizeof(double);
+        return (size_t)(split_stride(kmax) + 2 * split_rcap(kmax) + split_ridx_doubles(kmax)) *
+               sizeof(double);
     return (size_t)gram_stride(kmax) * sizeof(double);
 }
 
@@ -3236,9 +3484,12 @@ static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const
 
 static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
                                    const LekfJob* djobs, int njobs, long g0, double* ws, long gs,
-                                   int nvmax, int kmax, double* rot)
+                                   int nvmax, int kmax, double* rot, short* ridx)
 {
     const unsigned nwq = (unsigned)((nb + 63) / 64);
+    const int evp = 64 / (nvmax + 1);  // points per wave of the eigenvector-free kernel
+    const unsigned nwe = (unsigned)((nb + evp - 1) / evp);
+    const size_t flds = 64 * (size_t)(kmax | 1) * sizeof(double);
     const size_t qlds = 2 * (size_t)kr * 64 * sizeof(double);
     const size_t tlds = eig_lds_doubles(kmax, 0) * sizeof(double);
     switch (kr) {
@@ -3252,11 +3503,14 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
         {                                                                                        \
             KTime kt("eig_ql", st);                                                              \
             hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs,    \
-                               njobs, g0, nb, ws, gs, rot, kmax, g_lekf_tune.split_rcap);       \
+                               njobs, g0, nb, ws, gs, rot, ridx, kmax,                           \
+                               g_lekf_tune.split_rcap);                                         \
         }                                                                                        \
         KTime kt("eig_apply", st);                                                               \
         hipLaunchKernelGGL(lekf_eig3_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,    \
                            djobs, njobs, g0, ws, gs, nvmax, kmax, rot);                         \
+        hipLaunchKernelGGL(lekf_evf_kernel<K>, dim3(nwe), dim3(LEKF_WAVE), flds, st, djobs,      \
+                           njobs, g0, nb, ws, gs, rot, ridx, nvmax, kmax);                      \
         hipLaunchKernelGGL(lekf_eig_fix_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
                            djobs, njobs, g0, ws, gs, nvmax, kmax);                              \
         break;                                                                                   \
@@ -3284,6 +3538,7 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
     const long per = (long)(ws_bytes / ws_rec_bytes(kmax));
     if (per < 1 || !ws) return hipErrorInvalidValue;
     double* rot = (double*)ws + (size_t)per * gs;  // the rotation streams after the records
+    short* ridx = (short*)(rot + (size_t)per * 2 * split_rcap(kmax));  // then their columns
     const int W = g_lekf_tune.gram_waves;
     const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 4 + W) * sizeof(double);
     const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
@@ -3305,7 +3560,7 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
         if (e != hipSuccess) return e;
         if (s3)
             e = launch_eig_split(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs,
-                                 nvmax, kmax, rot);
+                                 nvmax, kmax, rot, ridx);
         else
             e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs, nvmax);
         if (e != hipSuccess) return e;
