@@ -90,10 +90,12 @@ def build_fortran(verbose: bool = False) -> str | None:
     _run(cmd, verbose)
     os.replace(tmp, FLIB)
     # the driver programs of the Fortran host side (used by the tests):
-    # the Lorenz-96 loop and the SPEEDY analysis sequence
-    for name in ("l96_driver", "speedy_driver"):
+    # the Lorenz-96 loop, the SPEEDY analysis sequence and letkf_core from
+    # OpenMP threads
+    for name in ("l96_driver", "speedy_driver", "omp_core_driver"):
         exe = os.path.join(FSRC, name)
-        cmd = [fc, "-O2", "-I", mod, os.path.join(FSRC, name + ".f90"), "-L", FSRC,
+        omp = ["-fopenmp"] if name == "omp_core_driver" else []
+        cmd = [fc, "-O2", *omp, "-I", mod, os.path.join(FSRC, name + ".f90"), "-L", FSRC,
                "-llekf_fortran", "-L", HERE, "-llekf_hip", "-Wl,-rpath,$ORIGIN",
                "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
         _run(cmd, verbose)

@@ -126,6 +126,8 @@ def _lib_ref():
         for f in (_ref.ref_spd_read_grd, _ref.ref_spd_write_grd):
             f.restype = None
             f.argtypes = [ctypes.c_char_p, c_int, _dp, _dp]
+        _ref.ref_spd_obsope1.restype = None
+        _ref.ref_spd_obsope1.argtypes = [_dp, _dp] + [c_double] * 5 + [_dp] * 4 + [_ip]
         _ref.ref_spd_get_nobs.restype = c_int
         _ref.ref_spd_get_nobs.argtypes = [ctypes.c_char_p, c_int, c_int]
         _ref.ref_spd_read_obs.restype = None
@@ -484,6 +486,21 @@ def trans_xtoy(elm, rlon, rlat, rlev, odat, lat, sig, v3d, ps):
     st = _lib_c().orc_trans_xtoy(int(elm), rlon, rlat, rlev, odat, nlon, lat.size, sig.size,
                                  _d(lat), _d(sig), _d(v3d), _d(ps), _d(y), _d(ri), _d(rj), _d(rk))
     return st, float(y[0]), float(ri[0]), float(rj[0]), float(rk[0])
+
+
+def ref_obsope1(elm, rlon, rlat, rlev, odat, v3d, v2d):
+    """The reference's phys2ijk + obsope range rules + Trans_XtoY for one obs
+    (speedy/common/common_obs_speedy.f90:30-282, speedy/letkf/obsope.f90:56-89) on
+    the T30 grid.  v3d: (nv3d, nlev, nlat, nlon) C-order, v2d: (nv2d, nlat, nlon).
+    Returns (status, yobs, ri, rj, rk); status 0 = used."""
+    out = np.zeros(4)
+    st = np.zeros(1, dtype=np.int32)
+    v3 = _f64(v3d)
+    v2 = _f64(v2d)
+    _lib_ref().ref_spd_obsope1(_d(v3), _d(v2), float(elm), float(rlon), float(rlat), float(rlev),
+                               float(odat), _d(out[0:]), _d(out[1:]), _d(out[2:]), _d(out[3:]),
+                               _i(st))
+    return int(st[0]), float(out[3]), float(out[0]), float(out[1]), float(out[2])
 
 
 def ref_solve_apply(loc_off, loc_idx, loc_rho, hdxf, rdiag, dep, xb, infl, fixed_infl=0.0):

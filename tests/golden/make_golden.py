@@ -14,7 +14,8 @@ Fixtures (numpy .npz, no pickles):
                    apply (:255-262) driven over the 40 points;
   global_patch.npz a reduced global sigma-level problem: selection by the oracle's
                    obs_local restatement, local solves + apply by the reference
-                   Fortran, for the das-level parity test.
+                   Fortran (patch_by_reference: ref_solve_apply over the lists), for
+                   the das-level parity test.
 """
 import os
 import sys
@@ -93,12 +94,64 @@ def l96_step():
                         loc_rho=np.concatenate(rho))
 
 
+def patch_by_reference(p, r):
+    """The patch's analysis with every local solve and apply done by the reference
+    Fortran letkf_core (oracle/_ref, ref_solve_apply) over the oracle's obs_local
+    lists: 3-D rows at every level, the 2-D rows with the level-1 transform
+    (speedy/letkf/letkf_tools.f90:178-197), adaptive inflation.  Returns (anal3d,
+    anal2d, infl3d) in the layouts of das_global_problem."""
+    s = r["sobs"]
+    nij, nlev, ne = p["nij"], p["nlev"], p["ne"]
+    lon1 = np.tile(p["lon"], p["nlat"])
+    lat1 = np.repeat(p["lat"], p["nlon"])
+    dz = p["sigma_obs"] * np.sqrt(10.0 / 3.0) * 2.0
+    dzv = p["sigma_obsv"] * np.sqrt(10.0 / 3.0) * 2.0
+    dlat = dz / oracle.PI_REF / oracle.RE_REF * 180.0
+    g2 = p["gues2d"][:, :, 0]
+    psm = g2[:, 0].copy()
+    for m in range(1, ne):
+        psm = psm + g2[:, m]
+    psm = psm / ne
+    a3 = np.zeros_like(p["gues3d"])
+    a2 = np.zeros_like(p["gues2d"])
+    i3 = np.zeros((nij, nlev))
+    for lev in range(nlev):
+        idx_all, rho_all, off, xb = [], [], [0], []
+        for ij in range(nij):
+            dlon = dlat / np.cos(oracle.PI_REF * lat1[ij] / 180.0)
+            idx, rho, _ = oracle.obs_local_global(lon1[ij], lat1[ij], dlon, dlat,
+                                                  np.log(psm[ij] * p["sig"][lev]), lev + 1,
+                                                  p["lon"], p["lat"], r["nobsgrd"], s, dz, dzv,
+                                                  p["sigma_obs"], p["sigma_obsv"])
+            idx_all.append(idx)
+            rho_all.append(rho)
+            off.append(off[-1] + idx.size)
+            rows = [p["gues3d"][ij, lev, :, v] for v in range(4)]
+            if lev == 0:
+                rows.append(p["gues2d"][ij, :, 0])
+            xb.append(np.stack(rows + [np.zeros(ne)] * (5 - len(rows))))
+        xa, io = oracle.ref_solve_apply(np.array(off), np.concatenate(idx_all),
+                                        np.concatenate(rho_all), np.ascontiguousarray(s["hdxf"].T),
+                                        s["err"] ** 2, s["dep"], np.stack(xb),
+                                        np.full(nij, p["infl"]))
+        a3[:, lev] = xa[:, :4].transpose(0, 2, 1)
+        if lev == 0:
+            a2[:, :, 0] = xa[:, 4]
+        i3[:, lev] = io
+    return a3, a2, i3
+
+
 def global_patch():
     p = synth.global_problem(nlon=24, lat=synth.SPEEDY_LAT[2::4], sig=synth.SPEEDY_SIG[::2], ne=10,
                              nstation=60, seed=synth.SEED_BASE + 90, nmode=16)
     p.update(sigma_obs=1500.0e3, sigma_obsv=0.4, gross_error=10.0, infl=1.01, ips=0)
     o = p["obs"]
     r = oracle.das_global_problem(p)
+    a3, a2, i3 = patch_by_reference(p, r)
+    # the oracle's das restatement gives the reference's analysis to rounding
+    sc = np.abs(a3 - a3.mean(axis=2, keepdims=True)).max()
+    assert np.abs(r["anal3d"] - a3).max() <= 1e-12 * sc
+    r.update(anal3d=a3, anal2d=a2, infl3d=i3)
     np.savez_compressed(os.path.join(HERE, "global_patch.npz"), lon=p["lon"], lat=p["lat"],
                         sig=p["sig"], gues3d=p["gues3d"], gues2d=p["gues2d"], elm=o["elm"],
                         olon=o["lon"], olat=o["lat"], olev=o["lev"], dat=o["dat"], err=o["err"],

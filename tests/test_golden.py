@@ -73,3 +73,28 @@ def test_global_patch_restatement():
     np.testing.assert_array_equal(r["anal3d"], z["anal3d"])
     np.testing.assert_array_equal(r["anal2d"], z["anal2d"])
     np.testing.assert_array_equal(r["infl3d"], z["infl3d"])
+
+
+def test_global_patch_is_the_reference_fortran():
+    """The fixture's analysis is the reference Fortran letkf_core + apply
+    (oracle/_ref) over the oracle's obs_local lists, bit for bit, as
+    tests/golden/make_golden.py writes it."""
+    import sys
+    if not oracle.ref_available():
+        import pytest
+        pytest.skip("reference build absent")
+    sys.path.insert(0, G)
+    import make_golden
+    z = _load("global_patch.npz")
+    sh, sv, ge, infl = z["sigma"]
+    nlon, nlat, nlev = z["lon"].size, z["lat"].size, z["sig"].size
+    p = dict(lon=z["lon"], lat=z["lat"], sig=z["sig"], nlon=nlon, nlat=nlat, nlev=nlev,
+             nij=nlon * nlat, ne=z["gues3d"].shape[2], gues3d=z["gues3d"], gues2d=z["gues2d"],
+             sigma_obs=sh, sigma_obsv=sv, gross_error=ge, infl=infl,
+             obs=dict(elm=z["elm"], lon=z["olon"], lat=z["olat"], lev=z["olev"], dat=z["dat"],
+                      err=z["err"], hxf=z["hxf"]))
+    r = oracle.das_global_problem(p)
+    a3, a2, i3 = make_golden.patch_by_reference(p, r)
+    np.testing.assert_array_equal(a3, z["anal3d"])
+    np.testing.assert_array_equal(a2, z["anal2d"])
+    np.testing.assert_array_equal(i3, z["infl3d"])

@@ -127,3 +127,24 @@ def test_batch_matches_single_and_reference(gpu, k):
                               R[sel] if p else [1.0], rho[b] if p else [1.0],
                               D[sel] if p else [0.0], 1.03)[0]
         assert np.array_equal(one, tb_b), (b, p)
+
+
+@pytest.mark.parametrize("k", [20, 50, 140])
+def test_rank_deficient_point_flagged(gpu, k):
+    """lambda_min < lambda_max * sqrt(eps) (common/common_mtx.f90:66-74): one obs with
+    rdiag 1e-9 puts the top eigenvalue of (k-1)/rho I + Y^T R^-1 Y about 1e9 times above
+    the others (condition past 1/sqrt(eps) = 6.7e7).  The C ABI returns
+    LEKF_WARN_RANKDEF (the strict wrapper raises it) with finite outputs, where the
+    reference truncates the rank and divides by the zeroed eigenvalues; the same
+    problem with that obs's rdiag restored returns LEKF_OK."""
+    from letkf_amd import _lib
+    h, rd, rl, dp = synth.local_problem(30, k, seed=5 + k)
+    bad = rd.copy()
+    bad[3] = 1.0e-9
+    with pytest.raises(_lib.LetkfError) as e:
+        core.letkf_core(k, 30, 30, h, bad, rl, dp, 1.05, strict=True)
+    assert e.value.code == _lib.LEKF_WARN_RANKDEF
+    tr, parm = core.letkf_core(k, 30, 30, h, bad, rl, dp, 1.05)
+    assert np.isfinite(tr).all() and np.isfinite(parm)
+    tr0, _ = core.letkf_core(k, 30, 30, h, rd, rl, dp, 1.05, strict=True)
+    assert np.isfinite(tr0).all()

@@ -112,3 +112,65 @@ def test_fortran_speedy_das_letkf(gpu, tmp_path, speedy_prob, cov_infl_mul):
         np.testing.assert_array_equal(infl, infl_in)
     else:
         np.testing.assert_allclose(infl, ref["infl3d"], rtol=1e-12)
+
+
+OMP = os.path.join(ROOT, "letkf_amd", "fortran", "omp_core_driver")
+
+
+def _omp_cases(ne, ncase, nobs, seed):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for c in range(ncase):
+        p = int(rng.integers(0, nobs + 1)) if c % 7 else 0
+        h = rng.normal(size=(nobs, ne))
+        h -= h.mean(axis=1, keepdims=True)
+        cases.append(dict(p=p, h=h, rd=rng.uniform(0.5, 2.0, nobs), rl=rng.uniform(0.05, 1.0, nobs),
+                          dep=rng.normal(size=nobs), parm=1.0 + 0.1 * rng.uniform()))
+    return cases
+
+
+def _run_omp(cases, ne, nobs, threads, tmp_path):
+    fin, fout = tmp_path / f"omp_in_{threads}.bin", tmp_path / f"omp_out_{threads}.bin"
+    with open(fin, "wb") as f:
+        np.array([len(cases), ne, nobs], dtype=np.int32).tofile(f)
+        for c in cases:
+            np.array([c["p"]], dtype=np.int32).tofile(f)
+            for a in (c["h"].ravel(order="F"), c["rd"], c["rl"], c["dep"], [c["parm"]]):
+                np.asarray(a, dtype=np.float64).tofile(f)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([OMP, str(fin), str(fout)], capture_output=True, text=True, timeout=300,
+                       env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(fout, dtype=np.float64).reshape(len(cases), 2 * ne * ne + ne + 1)
+    return out
+
+
+@pytest.mark.parametrize("ne", [20, 40])
+def test_fortran_letkf_core_openmp(gpu, tmp_path, ne):
+    """letkf_core through the drop-in common_letkf from 8 OpenMP threads at once
+    (the reference's PARALLEL DO over grid points, scale/letkf/letkf_tools.f90:175):
+    every case equals the single-thread run bit for bit, and the reference
+    Fortran letkf_core (oracle/_ref) to 1e-10; k = 20 (fused kernel) and 40
+    (three-kernel eigensolver)."""
+    if not os.path.exists(OMP):
+        pytest.skip("Fortran host side not built (amdflang missing)")
+    import oracle
+    nobs = 300
+    cases = _omp_cases(ne, 96, nobs, seed=ne)
+    o8 = _run_omp(cases, ne, nobs, 8, tmp_path)
+    o1 = _run_omp(cases, ne, nobs, 1, tmp_path)
+    np.testing.assert_array_equal(o8, o1)
+    if not oracle.ref_available():
+        return
+    for c, row in zip(cases, o8):
+        p = c["p"]
+        ref = oracle.letkf_core(c["h"][:p], c["rd"][:p], c["rl"][:p], c["dep"][:p], c["parm"],
+                                transm=True, pao=True, impl="ref")
+        tr = row[:ne * ne].reshape(ne, ne, order="F")
+        tm = row[ne * ne:ne * ne + ne]
+        pa = row[ne * ne + ne:2 * ne * ne + ne].reshape(ne, ne, order="F")
+        sc = max(np.abs(ref["trans"]).max(), 1.0)
+        assert np.abs(tr - ref["trans"]).max() <= 1e-10 * sc
+        assert np.abs(tm - ref["transm"]).max() <= 1e-10 * max(np.abs(ref["transm"]).max(), 1e-3)
+        assert np.abs(pa - ref["pao"]).max() <= 1e-10 * max(np.abs(ref["pao"]).max(), 1e-3)
+        assert abs(row[-1] - ref["parm_infl"]) <= 1e-10 * abs(ref["parm_infl"])

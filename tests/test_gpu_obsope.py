@@ -1,5 +1,6 @@
 """GPU obs operator (phys2ijk + obsope rules + Trans_XtoY on the resident ensemble)
-against the oracle's restatement of speedy/common/common_obs_speedy.f90:30-282.
+against the oracle's restatement of speedy/common/common_obs_speedy.f90:30-282
+and against the reference's own routines (oracle/_ref).
 
 Bit-exact: the QC flag, ri, rj and the CEILING corners (i, j, k).  rk and H(x)
 pass through log (device vs host libm): rk within 1e-12, H(x) within 1e-12 of
@@ -83,6 +84,43 @@ def test_obs_operator_matches_oracle(gpu, prob):
     np.testing.assert_array_equal(qc, ref_qc)
     assert ref_qc.sum() > 400 and ref_qc[-12:].sum() < 12  # the edge rows include rejects
     h.close()
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_obs_operator_matches_reference(gpu, prob):
+    """The same obs set through the reference's phys2ijk + obsope rules +
+    Trans_XtoY: the QC flag, ri, rj and the corners bit for bit, rk and H(x)
+    within 1e-12."""
+    p = prob
+    h = _handle(p)
+    o = _obs_sample(p)
+    keep = o["elm"] != oracle.ID_RH  # RH: see test_oracle_obsope
+    o = {k: v[keep] for k, v in o.items()}
+    hxf, qc, ri, rj, rk = h.obs_operator(o["elm"], o["lon"], o["lat"], o["lev"], o["dat"])
+    h.close()
+    n, ne = hxf.shape
+    scale = np.abs(p["gues3d"]).max(axis=(0, 1, 2))
+    ref_qc = np.ones(n, dtype=np.int32)  # an obs is used if every member uses it
+    for m in range(ne):
+        v3d, ps = _member_fields(p, m)
+        v2d = np.stack([ps, np.zeros_like(ps)])
+        for i in range(n):
+            st, y, ri0, rj0, rk0 = oracle.ref_obsope1(o["elm"][i], o["lon"][i], o["lat"][i],
+                                                      o["lev"][i], o["dat"][i], v3d, v2d)
+            if st:
+                ref_qc[i] = 0
+                continue
+            if not qc[i]:
+                continue
+            assert ri[i] == ri0 and rj[i] == rj0, i
+            assert np.ceil(ri[i]) == np.ceil(ri0) and np.ceil(rj[i]) == np.ceil(rj0)
+            assert np.ceil(rk[i, m]) == np.ceil(rk0), (i, m)
+            assert abs(rk[i, m] - rk0) <= 1e-12, (i, m)
+            el = int(o["elm"][i])
+            sc = 1.0e5 if el == oracle.ID_PS else scale[[oracle.ID_U, oracle.ID_V, oracle.ID_T,
+                                                         oracle.ID_Q].index(el)]
+            assert abs(hxf[i, m] - y) <= 1e-12 * sc, (i, m, hxf[i, m], y)
+    np.testing.assert_array_equal(qc, ref_qc)
 
 
 def test_set_obs_with_device_hx_equals_host_hx(gpu, prob):

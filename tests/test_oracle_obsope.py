@@ -2,12 +2,18 @@
 
 The restatement (oracle/lekf_oracle.c: orc_phys2ij, orc_itpl_*, orc_trans_xtoy)
 follows speedy/common/common_obs_speedy.f90:30-282 and
-speedy/letkf/obsope.f90:56-89.  Pinned here by what the mathematics and the
-reference's rules fix: bi-/trilinear interpolation reproduces fields linear in
-the grid indices; the longitude wrap uses column 1; latitudes on the table map
-to integer rj; the vertical and type rejection rules.
+speedy/letkf/obsope.f90:56-89.  Pinned here against the reference's own
+phys2ijk and Trans_XtoY (built in oracle/_ref, with obsope's range rules in
+the binding) bit for bit, on the SPEEDY-sized obs set with its edge rows, and
+by what the mathematics and the reference's rules fix: bi-/trilinear
+interpolation reproduces fields linear in the grid indices; the longitude wrap
+uses column 1; latitudes on the table map to integer rj; the vertical and type
+rejection rules.  The oracle covers u, v, T, q and ps; RH and rain obs (which
+Trans_XtoY also maps, via calc_rh and the rain field) are rejected by the
+oracle and the GPU operator alike.
 """
 import numpy as np
+import pytest
 
 import oracle
 from letkf_amd import synth
@@ -89,3 +95,46 @@ def test_vertical_and_type_rules():
     assert st == 0 and rk == 0.0 and y == 1.0e5
     assert oracle.trans_xtoy(oracle.ID_PS, 30.0, 10.0, 0.0, -200.0, LAT, SIG, v3d, ps)[0] == 1
     assert oracle.trans_xtoy(oracle.ID_RH, 30.0, 10.0, 5.0e4, 0.0, LAT, SIG, v3d, ps)[0] == 1
+
+
+def _speedy_obs_with_edges(p, n=600, seed=1):
+    o = p["obs"]
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(o["elm"].size, n, replace=False)
+    rows = [tuple(float(np.asarray(o[k])[i]) for k in ("elm", "lon", "lat", "lev", "dat"))
+            for i in sel]
+    lat0, latn = p["lat"][0], p["lat"][-1]
+    rows += [(oracle.ID_U, 0.0, 10.0, 5.0e4, 1.0), (oracle.ID_V, 360.0, -20.0, 3.0e4, 1.0),
+             (oracle.ID_T, 359.5, 30.0, 7.0e4, 1.0), (oracle.ID_T, 100.0, lat0 - 0.5, 5.0e4, 1.0),
+             (oracle.ID_Q, 100.0, latn + 0.5, 5.0e4, 1.0), (oracle.ID_U, 50.0, 0.0, 1.2e5, 1.0),
+             (oracle.ID_T, 50.0, 0.0, 1.2e5, 1.0), (oracle.ID_T, 50.0, 0.0, 1.0e3, 1.0),
+             (oracle.ID_PS, 50.0, 0.0, 0.0, -500.0), (oracle.ID_PS, 3.0, 40.0, 0.0, 1.0e5),
+             (oracle.ID_U, -2.0, 0.0, 5.0e4, 1.0), (oracle.ID_V, 361.0, 5.0, 5.0e4, 1.0),
+             (oracle.ID_Q, 180.0, LAT[5], 1.0e5 * SIG[3], 1.0), (oracle.ID_PS, 0.0, LAT[0], 0.0, 9.0e4)]
+    return rows
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_oracle_equals_reference_obs_operator():
+    """orc_trans_xtoy against the reference's phys2ijk + obsope rules + Trans_XtoY
+    on three members of the SPEEDY-sized problem: the same obs rejected, and
+    ri, rj, rk and H(x) bit for bit."""
+    p = synth.speedy()
+    rows = _speedy_obs_with_edges(p)
+    nlon, nlat, nlev = p["lon"].size, p["lat"].size, p["sig"].size
+    assert np.array_equal(p["lat"], LAT) and np.array_equal(p["sig"], SIG)
+    used = 0
+    for m in (0, 7, 19):
+        v3d = np.ascontiguousarray(p["gues3d"][:, :, m, :].transpose(2, 1, 0)).reshape(4, nlev, nlat,
+                                                                                      nlon)
+        ps = p["gues2d"][:, m, 0].reshape(nlat, nlon)
+        v2d = np.stack([ps, np.zeros_like(ps)])
+        for r in rows:
+            a = oracle.trans_xtoy(*r, LAT, SIG, v3d, ps)
+            b = oracle.ref_obsope1(*r, v3d, v2d)
+            assert (a[0] == 0) == (b[0] == 0), r
+            if b[0] == 0:
+                used += 1
+                assert a[1:] == b[1:], (r, a, b)
+    assert used > 1500
+
