@@ -3111,7 +3111,94 @@ void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
 // stream (column of each entry from the QL kernel, -1 for a sweep header),
 // forward for R and backward with the transposed rotations for R^T.
 // Results agree with the explicit form to rounding.
-constexpr int kEvfChunk = 24;  // rotation-stream entries loaded at once per lane
+constexpr int kEvfChunk = 16;  // rotation-stream entries loaded at once per lane
+
+// kEvfChunk entries of a lane's rotation stream: column (-1: a sweep header), c, s
+struct EvfChunk {
+    double c[kEvfChunk], s[kEvfChunk];
+    int i[kEvfChunk];
+};
+
+__device__ __forceinline__ void evf_load(EvfChunk& K, const double* rs, const short* rx, long e0,
+                                         long ntot)
+{
+#pragma unroll
+    for (int u = 0; u < kEvfChunk; ++u) {
+        const long e = e0 + u;
+        const bool ok = e >= 0 && e < ntot;
+        K.i[u] = ok ? (int)rx[e] : -1;
+        K.c[u] = ok ? rs[2 * e] : 0.0;
+        K.s[u] = ok ? rs[2 * e + 1] : 0.0;
+    }
+}
+
+// Rotations of a chunk in stream order, (a_i, a_i+1) -> (c a_i - s a_i+1, s a_i + c a_i+1).
+// A sweep runs down the columns: rotation i after i+1 shares column i+1,
+// which is carried in cv (ci = its column) instead of going through LDS.
+__device__ __forceinline__ void evf_fwd(const EvfChunk& K, double* row, int& ci, double& cv)
+{
+    double pre = 0.0;
+    int pi = -2;  // column whose value pre holds
+#pragma unroll
+    for (int u = 0; u < kEvfChunk; ++u) {
+        const int i = K.i[u];
+        if (i >= 0) {
+            double a1;
+            if (ci == i + 1) {
+                a1 = cv;
+            } else {
+                if (ci >= 0) row[ci] = cv;
+                a1 = row[i + 1];
+            }
+            const double a0 = (pi == i) ? pre : row[i];
+            const double n0 = fma(K.c[u], a0, -K.s[u] * a1);
+            const double n1 = fma(K.s[u], a0, K.c[u] * a1);
+            if (u + 1 < kEvfChunk && K.i[u + 1] == i - 1) {
+                pi = i - 1;
+                pre = row[i - 1];
+            } else {
+                pi = -2;
+            }
+            row[i + 1] = n1;
+            ci = i;
+            cv = n0;
+        }
+    }
+}
+
+// The transposed rotations of a chunk in reverse stream order,
+// (a_i, a_i+1) -> (c a_i + s a_i+1, -s a_i + c a_i+1): a reversed sweep runs up
+// the columns and carries the upper one.
+__device__ __forceinline__ void evf_bwd(const EvfChunk& K, double* row, int& ci, double& cv)
+{
+    double pre = 0.0;
+    int pi = -2;
+#pragma unroll
+    for (int u = kEvfChunk - 1; u >= 0; --u) {
+        const int i = K.i[u];
+        if (i >= 0) {
+            double a0;
+            if (ci == i) {
+                a0 = cv;
+            } else {
+                if (ci >= 0) row[ci] = cv;
+                a0 = row[i];
+            }
+            const double a1 = (pi == i + 1) ? pre : row[i + 1];
+            const double n0 = fma(K.c[u], a0, K.s[u] * a1);
+            const double n1 = fma(-K.s[u], a0, K.c[u] * a1);
+            if (u > 0 && K.i[u - 1] == i + 1) {
+                pi = i + 2;
+                pre = row[i + 2];
+            } else {
+                pi = -2;
+            }
+            row[i] = n0;
+            ci = i + 1;
+            cv = n1;
+        }
+    }
+}
 
 template <int KR>
 __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict__ jobs, int njobs,
@@ -3222,29 +3309,22 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
         const long t = __shfl_xor(emax, o, 64);
         emax = t > emax ? t : emax;
     }
-    // the stream is read kEvfChunk entries at a time, all loads issued before
-    // the chunk's rotations run
-    for (long e0 = 0; e0 < emax; e0 += kEvfChunk) {
-        double cc[kEvfChunk], ss[kEvfChunk];
-        int ii[kEvfChunk];
-#pragma unroll
-        for (int u = 0; u < kEvfChunk; ++u) {
-            const long e = e0 + u;
-            const bool ok = e < ntot;
-            ii[u] = ok ? (int)rx[e] : -1;
-            cc[u] = ok ? rs[2 * e] : 0.0;
-            ss[u] = ok ? rs[2 * e + 1] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kEvfChunk; ++u) {
-            const int i = ii[u];
-            if (i >= 0) {
-                const double a0 = row[i], a1 = row[i + 1];
-                row[i] = fma(cc[u], a0, -ss[u] * a1);
-                row[i + 1] = fma(ss[u], a0, cc[u] * a1);
-            }
-        }
+    // the stream is read in chunks of kEvfChunk entries, the next chunk's
+    // loads in flight while one is applied; within a sweep the shared column
+    // of consecutive rotations stays in a register (cv, column ci) and the
+    // next rotation's other column is read before this one writes (it lies
+    // below everything the sweep writes from here on)
+    EvfChunk ka, kb;
+    evf_load(ka, rs, rx, 0, ntot);
+    int ci = -1;
+    double cv = 0.0;
+    for (long e0 = 0; e0 < emax; e0 += 2 * kEvfChunk) {
+        evf_load(kb, rs, rx, e0 + kEvfChunk, ntot);
+        evf_fwd(ka, row, ci, cv);
+        evf_load(ka, rs, rx, e0 + 2 * kEvfChunk, ntot);
+        evf_fwd(kb, row, ci, cv);
     }
+    if (ci >= 0) row[ci] = cv;
     __syncthreads();
     // u = (b^T V) / lambda in b's lane; truncation test on lambda
     double lmax = 0.0, lmin = 1.0e308;
@@ -3265,27 +3345,19 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
     }
     __syncthreads();
     // backward through R^T: the entries in reverse, each rotation transposed
-    for (long e0 = emax > 0 ? ((emax - 1) / kEvfChunk) * kEvfChunk : -1; e0 >= 0;
-         e0 -= kEvfChunk) {
-        double cc[kEvfChunk], ss[kEvfChunk];
-        int ii[kEvfChunk];
-#pragma unroll
-        for (int u = 0; u < kEvfChunk; ++u) {
-            const long e = e0 + u;
-            const bool ok = e < ntot;
-            ii[u] = ok ? (int)rx[e] : -1;
-            cc[u] = ok ? rs[2 * e] : 0.0;
-            ss[u] = ok ? rs[2 * e + 1] : 0.0;
+    {
+        // chunks from the last down, each applied in reverse
+        const long top = emax > 0 ? ((emax - 1) / kEvfChunk) * kEvfChunk : -kEvfChunk;
+        EvfChunk ka2, kb2;
+        evf_load(ka2, rs, rx, top, ntot);
+        ci = -1;
+        for (long e0 = top; e0 >= 0; e0 -= 2 * kEvfChunk) {
+            evf_load(kb2, rs, rx, e0 - kEvfChunk, ntot);
+            evf_bwd(ka2, row, ci, cv);
+            evf_load(ka2, rs, rx, e0 - 2 * kEvfChunk, ntot);
+            evf_bwd(kb2, row, ci, cv);
         }
-#pragma unroll
-        for (int u = kEvfChunk - 1; u >= 0; --u) {
-            const int i = ii[u];
-            if (i >= 0) {
-                const double a0 = row[i], a1 = row[i + 1];
-                row[i] = fma(cc[u], a0, ss[u] * a1);
-                row[i + 1] = fma(-ss[u], a0, cc[u] * a1);
-            }
-        }
+        if (ci >= 0) row[ci] = cv;
     }
     // out = row Q^T (Q column-major at qoff: Q[m][t] at qoff + t * k + m)
     if (act) {
