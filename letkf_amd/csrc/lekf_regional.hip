@@ -490,12 +490,14 @@ __device__ int cap_stream(SelLds& S, const CapGlobal& G, KeyAt keyat, Emit emit,
 //
 // The kept set S (the cap smallest (key, scan position) of the stream so
 // far) is held ascending in key[], with each entry's storage slot in slot[];
-// so[s] is the candidate in storage slot s (an obs or list position, which
-// increases in scan order).  The first cap candidates fill slots 0..cap-1
-// and S is then sorted once.  A later candidate can enter only if its key is
-// below the key tau of S's last entry (an earlier candidate with an equal key
-// precedes it), and tau only falls, so candidates are filtered against tau
-// into a queue and each block of 64 queued candidates is merged at once:
+// so[s] (caller's memory) is the candidate in storage slot s (an obs or list
+// position, which increases in scan order).  The first cap candidates fill
+// slots 0..cap-1; each 64 of them are sorted across the lanes and merged
+// into the sorted prefix as they arrive.  A later candidate can enter only if
+// its key is below the key tau of S's last entry (an earlier candidate with
+// an equal key precedes it), and tau only falls, so candidates are filtered
+// against tau into a queue and each block of 64 queued candidates is merged
+// at once:
 // * a candidate enters iff fewer than cap of S and of the earlier queued
 //   candidates precede it; all of S precedes it in scan order, so its count
 //   in S is the number of keys <= its key, and the earlier queued candidates
@@ -515,12 +517,9 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-constexpr int kWcPer = LEKF_CAP_LDS / 64;  // entries of S per lane in the merge
-
 struct WaveCap {
     unsigned long long key[LEKF_CAP_LDS];
     unsigned short slot[LEKF_CAP_LDS];
-    int so[LEKF_CAP_LDS];
     unsigned long long qkey[128];  // queued candidates, scan order
     int qo[128];
     int ev[64];                    // entries that left a block, ascending: slot, or -(entrant + 1)
@@ -530,6 +529,7 @@ struct WaveCap {
 
 struct WcState {
     int n;                   // stream candidates so far
+    int m;                   // sorted prefix of S while it fills
     int q;                   // queued candidates
     unsigned long long tau;  // key of S's last entry (once n >= cap)
 };
@@ -554,59 +554,84 @@ __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int 
     }
 }
 
-// the first cap stream candidates (slots 0..cap-1 in scan order) sorted by
-// (key, slot); entries up to the next power of two are padding
-__device__ void wc_sort_fill(WaveCap& L, int cap, int lane)
+// number of the first n keys (ascending) that are <= y
+__device__ __forceinline__ int wc_count_le(const WaveCap& L, int n, unsigned long long y)
 {
-    int F = 64, lf = 6;
-    while (F < cap) {
-        F <<= 1;
-        ++lf;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (L.key[mid] <= y)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
-    for (int i = cap + lane; i < F; i += 64) {
-        L.key[i] = ~0ull;
-        L.slot[i] = 0xffff;
-    }
-    wave_sync();
-    for (int ls = 1; ls <= lf; ++ls) {
-        for (int lj = ls - 1; lj >= 0; --lj) {
-            const int j = 1 << lj;
-            for (int q = lane; q < (F >> 1); q += 64) {
-                const int i = ((q >> lj) << (lj + 1)) | (q & (j - 1));
-                const unsigned long long ka = L.key[i], kb = L.key[i + j];
-                const int sa = L.slot[i], sb = L.slot[i + j];
-                const bool up = ((i >> ls) & 1) == 0;
-                const bool bless = kb < ka || (kb == ka && sb < sa);
-                if (bless == up) {
-                    L.key[i] = kb;
-                    L.key[i + j] = ka;
-                    L.slot[i] = (unsigned short)sb;
-                    L.slot[i + j] = (unsigned short)sa;
-                }
+    return lo;
+}
+
+// Entries 0..n-1 of key[]/slot[] move up by the new entries placed before
+// them (lanes with ent, at count c1 of the old entries: before old entry i
+// iff c1 <= i); an entry that lands at lim or above goes to ev[] instead
+// (its slot).  Windows of 64 go top down, so a window's writes only reach
+// entries that already moved, and each lane holds one entry at a time.
+__device__ __forceinline__ void wc_shift(WaveCap& L, int n, int lim, bool ent, int c1, int lane)
+{
+    const unsigned long long em = __ballot(ent);
+    if (!em || n <= 0) return;
+    const int c1min = __builtin_amdgcn_readlane(c1, __builtin_ctzll(em));
+    for (int lo = ((n - 1) >> 6) << 6; lo >= 0 && lo + 64 > c1min; lo -= 64) {
+        const int i = lo + lane;
+        unsigned long long k = 0;
+        int s = 0;
+        if (i < n) {
+            k = L.key[i];
+            s = L.slot[i];
+        }
+        int ptr = __popcll(__ballot(ent && c1 < lo));
+        unsigned long long wm = __ballot(ent && c1 >= lo && c1 < lo + 64);
+        while (wm) {
+            const int bl = __builtin_ctzll(wm);
+            wm &= wm - 1;
+            const int d = __builtin_amdgcn_readlane(c1, bl) - lo;
+            ptr += lane >= d ? 1 : 0;
+        }
+        if (i < n && ptr > 0) {
+            const int ni = i + ptr;
+            if (ni < lim) {
+                L.key[ni] = k;
+                L.slot[ni] = (unsigned short)s;
+            } else {
+                L.ev[ni - lim] = s;
             }
-            wave_sync();
         }
     }
 }
 
+// While S fills: the nt (<= 64) entries at m.. (slots m.., unsorted) join
+// the sorted prefix 0..m-1.
+__device__ void wc_fill_merge(WaveCap& L, int m, int nt, int lane)
+{
+    unsigned long long sk = lane < nt ? L.key[m + lane] : ~0ull;
+    int sid = lane;
+    wave_sort64(sk, sid, lane);  // the nt entries take lanes 0..nt-1 (ties: slot order)
+    const bool v = lane < nt;
+    const int c1 = v ? wc_count_le(L, m, sk) : m;
+    wave_sync();
+    wc_shift(L, m, m + nt, v, c1, lane);
+    wave_sync();
+    if (v) {
+        L.key[c1 + lane] = sk;
+        L.slot[c1 + lane] = (unsigned short)(m + sid);
+    }
+    wave_sync();
+}
+
 // One block: queued candidates 0..np-1 (np <= 64) against the full set S.
-__device__ void wc_block(WaveCap& L, int np, int cap, int lane, unsigned long long& tau)
+__device__ void wc_block(WaveCap& L, int* so, int np, int cap, int lane, unsigned long long& tau)
 {
     const bool valid = lane < np;
     const unsigned long long y = valid ? L.qkey[lane] : ~0ull;
     const bool live = valid && y < tau;  // tau has fallen below it since it was queued: out
-    int c1 = cap;
-    if (live) {
-        int lo = 0, hi = cap;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (L.key[mid] <= y)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        c1 = lo;
-    }
+    const int c1 = live ? wc_count_le(L, cap, y) : cap;
     // sort the live candidates by (key, lane): lane L holds the L-th
     unsigned long long sk = live ? y : ~0ull;
     int sid = lane;
@@ -632,42 +657,9 @@ __device__ void wc_block(WaveCap& L, int np, int cap, int lane, unsigned long lo
     const int ento = __builtin_amdgcn_ds_permute(sid << 2, ents ? 1 : 0);
     const unsigned long long emO = __ballot(ento != 0);
     const int t = __shfl((int)__popcll(emO & lane_lt(lane)), sid, 64);
-    // merge: lane takes the entries u * 64 + lane; every lane reads its
-    // entries before any lane writes
-    unsigned long long vk[kWcPer];
-    int vs[kWcPer];
-#pragma unroll
-    for (int u = 0; u < kWcPer; ++u) {
-        const int i = u * 64 + lane;
-        const int ii = i < cap ? i : 0;
-        vk[u] = L.key[ii];
-        vs[u] = L.slot[ii];
-    }
     wave_sync();
-#pragma unroll
-    for (int u = 0; u < kWcPer; ++u) {
-        const int lo = u * 64;
-        if (lo >= cap) break;
-        // entrants before entry i = lo + lane: those with c1 <= i
-        int ptr = __popcll(__ballot(ents && c1s < lo));
-        unsigned long long wm = __ballot(ents && c1s >= lo && c1s < lo + 64);
-        while (wm) {
-            const int bl = __builtin_ctzll(wm);
-            wm &= wm - 1;
-            const int d = __builtin_amdgcn_readlane(c1s, bl) - lo;
-            ptr += lane >= d ? 1 : 0;
-        }
-        const int i = lo + lane;
-        if (i < cap) {
-            const int ni = i + ptr;
-            if (ni < cap) {
-                L.key[ni] = vk[u];
-                L.slot[ni] = (unsigned short)vs[u];
-            } else {
-                L.ev[ni - cap] = vs[u];
-            }
-        }
-    }
+    wc_shift(L, cap, cap, ents, c1s, lane);
+    wave_sync();
     if (ents) {
         if (U < cap)
             L.key[U] = sk;
@@ -705,7 +697,7 @@ __device__ void wc_block(WaveCap& L, int np, int cap, int lane, unsigned long lo
     }
     if (ents && U < cap && sl >= 0) {
         L.slot[U] = (unsigned short)sl;
-        L.so[sl] = L.qo[sid];
+        so[sl] = L.qo[sid];
     }
     wave_sync();
     tau = L.key[cap - 1];
@@ -713,8 +705,8 @@ __device__ void wc_block(WaveCap& L, int np, int cap, int lane, unsigned long lo
 
 // One chunk of the stream: lane l's position is a stream candidate iff ok
 // (key k, candidate o); positions are in scan order across the lanes.
-__device__ void wc_push(WaveCap& L, WcState& S, bool ok, unsigned long long k, int o, int cap,
-                        int lane)
+__device__ void wc_push(WaveCap& L, int* so, WcState& S, bool ok, unsigned long long k, int o,
+                        int cap, int lane)
 {
     const unsigned long long am = __ballot(ok);
     if (!am) return;
@@ -726,13 +718,17 @@ __device__ void wc_push(WaveCap& L, WcState& S, bool ok, unsigned long long k, i
         if (ok && n0 + rank < cap) {
             const int f = n0 + rank;
             L.key[f] = k;
-            L.slot[f] = (unsigned short)f;
-            L.so[f] = o;
+            so[f] = o;
             pend = false;
         }
-        if (S.n < cap) return;
         wave_sync();
-        wc_sort_fill(L, cap, lane);
+        const int nf = S.n < cap ? S.n : cap;
+        while (nf - S.m >= 64 || (S.n >= cap && S.m < cap)) {
+            const int nt = nf - S.m < 64 ? nf - S.m : 64;
+            wc_fill_merge(L, S.m, nt, lane);
+            S.m += nt;
+        }
+        if (S.n < cap) return;
         S.tau = L.key[cap - 1];
         if (S.n == cap) return;
     }
@@ -747,7 +743,7 @@ __device__ void wc_push(WaveCap& L, WcState& S, bool ok, unsigned long long k, i
     S.q += __popcll(pm);
     wave_sync();
     if (S.q >= 64) {
-        wc_block(L, 64, cap, lane, S.tau);
+        wc_block(L, so, 64, cap, lane, S.tau);
         // the rest of the queue moves to its front
         const bool mv = 64 + lane < S.q;
         unsigned long long k2 = 0;
@@ -767,10 +763,11 @@ __device__ void wc_push(WaveCap& L, WcState& S, bool ok, unsigned long long k, i
 }
 
 // End of the stream: the last queued block.  Returns the kept count; the
-// kept candidates are so[0 .. kept) in slot order.
-__device__ int wc_finish(WaveCap& L, WcState& S, int cap, int lane)
+// kept candidates are so[0 .. kept) in slot order (visible to the wave's
+// loads after a workgroup-scope fence).
+__device__ int wc_finish(WaveCap& L, int* so, WcState& S, int cap, int lane)
 {
-    if (S.q > 0) wc_block(L, S.q, cap, lane, S.tau);
+    if (S.q > 0) wc_block(L, so, S.q, cap, lane, S.tau);
     S.q = 0;
     wave_sync();
     return S.n < cap ? S.n : cap;
@@ -947,6 +944,7 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
     for (int u = 0; u < wv; ++u) bs += A.bcc[(size_t)u * nij + nij - 1];
     WaveCap& W = L[wv];
     RowTab& T = R[wv];
+    int* so = A.wc_so + ((size_t)blockIdx.x * NW + wv) * LEKF_CAP_LDS;  // slot -> obs
     const double r2cut = kNdMax * A.sigma_h * A.sigma_h * (1.0 + 1e-9);
 
     // XCD-aware tile order: workgroup w runs on XCD w % 8 (round-robin
@@ -979,7 +977,7 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
         if (imax > A.nx) imax = A.nx;
         if (jmin < 1) jmin = 1;
         if (jmax > A.ny) jmax = A.ny;
-        WcState S{0, 0, ~0ull};
+        WcState S{0, 0, 0, ~0ull};
         if (imin <= imax) {
             for (int j0 = jmin; j0 <= jmax; j0 += 64) {
                 const int nr = jmax - j0 + 1 < 64 ? jmax - j0 + 1 : 64;
@@ -1053,12 +1051,12 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
                     c.ok = false;
                     c.bits = 0;
                     if (co >= 0) c = eval_rec(A, cur, co, ri, rj, rz);
-                    wc_push(W, S, c.ok, c.bits, co, cap, lane);
+                    wc_push(W, so, S, c.ok, c.bits, co, cap, lane);
                 }
                 wave_sync();
             }
         }
-        const int kept = wc_finish(W, S, cap, lane);
+        const int kept = wc_finish(W, so, S, cap, lane);
         if (lane == 0) kept_s[wv] = kept;
         __syncthreads();
         int off = 0, tot = 0;
@@ -1069,7 +1067,7 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
         }
         const long obase = A.list_off[b] - A.list_base;  // relative to this chunk's list buffer
         for (int sl = lane; sl < kept; sl += 64) {
-            const int o = W.so[sl];
+            const int o = so[sl];
             const Cand c = eval_rec(A, load_obs(A, o), o, ri, rj, rz);
             A.loc_idx[obase + off + sl] = o;
             A.loc_rho[obase + off + sl] = c.rho;
@@ -1140,6 +1138,7 @@ __device__ __forceinline__ unsigned long long rho_key(double rho)
 __global__ __launch_bounds__(64) void list_cap_wave(ListCapArgs A)
 {
     __shared__ WaveCap L;
+    __shared__ int so[LEKF_CAP_LDS];
     const int lane = threadIdx.x;
     const long npt = (long)A.ncols * A.nlev;
     for (long pi = blockIdx.x; pi < npt; pi += gridDim.x) {
@@ -1156,17 +1155,17 @@ __global__ __launch_bounds__(64) void list_cap_wave(ListCapArgs A)
                 A.out_rho[ob + i] = ir[i];
             }
         } else {
-            WcState S{0, 0, ~0ull};
+            WcState S{0, 0, 0, ~0ull};
             double rn = lane < n ? ir[lane] : 0.0;
             for (int i0 = 0; i0 < n; i0 += 64) {
                 const double rc = rn;
                 const int i = i0 + lane;
                 rn = i + 64 < n ? ir[i + 64] : 0.0;
-                wc_push(L, S, i < n, rho_key(rc), i, A.cap, lane);
+                wc_push(L, so, S, i < n, rho_key(rc), i, A.cap, lane);
             }
-            kept = wc_finish(L, S, A.cap, lane);
+            kept = wc_finish(L, so, S, A.cap, lane);
             for (int sl = lane; sl < kept; sl += 64) {
-                const int i = L.so[sl];
+                const int i = so[sl];
                 A.out_idx[ob + sl] = ii[i];
                 A.out_rho[ob + sl] = ir[i];
             }
@@ -1375,6 +1374,15 @@ int lekf_reg_select_slots_max()
     return 4 * (ncu > 0 ? ncu : 256);
 }
 
+// a CU holds at most 16 waves of the fused selection (its LDS and registers)
+long lekf_reg_wave_slots_max()
+{
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    return 16L * (ncu > 0 ? ncu : 256);
+}
+
 hipError_t lekf_list_cap_launch(const ListCapArgs& A, int nslots, hipStream_t st)
 {
     const long npt = (long)A.ncols * A.nlev;
@@ -1405,7 +1413,9 @@ static hipError_t launch_select_wave(const RegionalArgs& A, long npt, hipStream_
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reg_select_wave<NW>, 64 * NW, 0);
     long g = (long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 256);
+    if (g * NW > A.wc_so_waves) g = A.wc_so_waves / NW;  // one slot map per wave
     if (g > npt) g = npt;
+    if (g < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reg_select_wave<NW>, dim3((unsigned)g), dim3(64 * NW), 0, st, A, npt);
     return hipGetLastError();
 }

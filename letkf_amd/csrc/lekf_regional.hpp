@@ -39,6 +39,10 @@ struct RegionalArgs {
     const int* bcc;
     const double *bri, *brj, *bz, *berr;
     const int* bo;
+    // the fused selection's slot -> obs map, LEKF_CAP_LDS ints per wave, for
+    // wc_so_waves waves (the launch holds no more waves than that)
+    int* wc_so;
+    long wc_so_waves;
     long pt_begin;                          // first global point of this launch
     const long* list_off;                   // per point list offsets (global numbering)
     long list_base;                         // list_off of the launch's first point
@@ -103,3 +107,5 @@ hipError_t lekf_reg_capacity_launch(const RegionalArgs& A, int* capn, long* cap_
 hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, hipStream_t st);
 int lekf_reg_select_slots();      // tuned count
 int lekf_reg_select_slots_max();  // scratch is sized for this many
+// waves the fused selection's slot maps are sized for
+long lekf_reg_wave_slots_max();
