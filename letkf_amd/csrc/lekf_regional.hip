@@ -534,6 +534,34 @@ struct WcState {
     unsigned long long tau;  // key of S's last entry (once n >= cap)
 };
 
+// x of lane (lane ^ j), for the constant j of an unrolled loop: quad
+// permutes for 1 and 2, row shifts (lane n reads n + 4 or n - 4) for 4, a
+// rotation by half a row for 8; for 16 and 32 the permlane swaps, which
+// return x of the row (half) above in the even rows (lower half) and x of
+// the row (half) below in the odd rows (upper half)
+__device__ __forceinline__ int lane_xor(int x, int j, int lane)
+{
+    switch (j) {
+    case 16: {
+        const auto p = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
+        return (int)((lane & 16) ? p[0] : p[1]);
+    }
+    case 32: {
+        const auto p = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false);
+        return (int)((lane & 32) ? p[0] : p[1]);
+    }
+    case 1: return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    case 4: {
+        const int up = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const int dn = __builtin_amdgcn_mov_dpp(x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4) ? dn : up;
+    }
+    case 8: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    default: return __shfl_xor(x, j, 64);
+    }
+}
+
 // ascending bitonic sort of the 64 lanes' (key, id) pairs (ids distinct)
 __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int lane)
 {
@@ -541,8 +569,10 @@ __device__ __forceinline__ void wave_sort64(unsigned long long& k, int& id, int 
     for (int sz = 2; sz <= 64; sz <<= 1) {
 #pragma unroll
         for (int j = sz >> 1; j > 0; j >>= 1) {
-            const unsigned long long ok = __shfl_xor(k, j, 64);
-            const int oid = __shfl_xor(id, j, 64);
+            const unsigned lo = (unsigned)lane_xor((int)(unsigned)k, j, lane);
+            const unsigned hi = (unsigned)lane_xor((int)(unsigned)(k >> 32), j, lane);
+            const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
+            const int oid = lane_xor(id, j, lane);
             const bool up = (lane & sz) == 0;
             const bool lower = (lane & j) == 0;
             const bool oless = ok < k || (ok == k && oid < id);
@@ -1031,10 +1061,12 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
                         int r = r0;
                         while (T.pre[r + 1] <= q) ++r;
                         const int p = T.beg[r] + (q - T.pre[r]);
-                        nrec.ri = A.bri[p];
-                        nrec.rj = A.brj[p];
-                        nrec.z = A.bz[p];
-                        nrec.err = A.berr[p];
+                        const double2 rij = *reinterpret_cast<const double2*>(A.brec + 4 * (size_t)p);
+                        const double2 zer = *reinterpret_cast<const double2*>(A.brec + 4 * (size_t)p + 2);
+                        nrec.ri = rij.x;
+                        nrec.rj = rij.y;
+                        nrec.z = zer.x;
+                        nrec.err = zer.y;
                         nrec.b = wv;
                         no = A.bo[p];
                     }
@@ -1099,8 +1131,7 @@ __global__ void reg_bucket_count(const int* cellcum, const int* obkt, int ncell,
 
 __global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const double* orj,
                                    const double* oz, const double* oerr, const int* obkt,
-                                   int ncell, int nbkt, const int* bcc, double* bri, double* brj,
-                                   double* bz, double* berr, int* bo)
+                                   int ncell, int nbkt, const int* bcc, double* brec, int* bo)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -1116,10 +1147,11 @@ __global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const 
         const int t = obkt[i];
         if (t < 0 || t >= nbkt) continue;
         const int p = pos[t]++;
-        bri[p] = ori[i];
-        brj[p] = orj[i];
-        bz[p] = oz[i];
-        berr[p] = oerr[i];
+        double* r = brec + 4 * (size_t)p;
+        r[0] = ori[i];
+        r[1] = orj[i];
+        r[2] = oz[i];
+        r[3] = oerr[i];
         bo[p] = i;
     }
 }
@@ -1439,8 +1471,7 @@ hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, h
 }
 
 hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
-                                   int* bcnt, int* bcc, double* bri, double* brj, double* bz,
-                                   double* berr, int* bo, hipStream_t st)
+                                   int* bcnt, int* bcc, double* brec, int* bo, hipStream_t st)
 {
     (void)nbinned;
     const int ncell = A.nx * A.ny;
@@ -1452,6 +1483,6 @@ hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* 
         hipLaunchKernelGGL(scan_inclusive_int, dim3(1), dim3(1024), 0, st, bcnt + (size_t)t * ncell,
                            bcc + (size_t)t * ncell, ncell);
     hipLaunchKernelGGL(reg_bucket_scatter, dim3(nb), dim3(256), 0, st, cellcum, A.o_ri, A.o_rj,
-                       A.o_z, A.o_err, A.o_bkt, ncell, nbkt, bcc, bri, brj, bz, berr, bo);
+                       A.o_z, A.o_err, A.o_bkt, ncell, nbkt, bcc, brec, bo);
     return hipGetLastError();
 }

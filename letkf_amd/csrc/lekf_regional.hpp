@@ -34,10 +34,11 @@ struct RegionalArgs {
     const int* obkt;
     // the sorted obs again, bucket by bucket (each bucket's obs in the sorted
     // order): bcc[t * nx * ny + c] = inclusive count of bucket t's obs through
-    // cell c; bucket t starts at the sum of the earlier buckets' totals; bo =
-    // the obs's position in the sorted order
+    // cell c; bucket t starts at the sum of the earlier buckets' totals;
+    // brec[4 p ..] = (ri, rj, z, err) of the obs at position p (32-byte
+    // aligned); bo = the obs's position in the sorted order
     const int* bcc;
-    const double *bri, *brj, *bz, *berr;
+    const double* brec;
     const int* bo;
     // the fused selection's slot -> obs map, LEKF_CAP_LDS ints per wave, for
     // wc_so_waves waves (the launch holds no more waves than that)
@@ -93,8 +94,7 @@ hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
 // the bucket-by-bucket copy of the sorted obs (RegionalArgs bcc, bri .. bo):
 // bcnt (nbkt * ncell ints) is work space
 hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
-                                   int* bcnt, int* bcc, double* bri, double* brj, double* bz,
-                                   double* berr, int* bo, hipStream_t st);
+                                   int* bcnt, int* bcc, double* brec, int* bo, hipStream_t st);
 // the capped selection keeps its kept sets in LDS, one wave per bucket
 inline bool lekf_reg_select_fused(int nobs_max) { return nobs_max > 0 && nobs_max <= LEKF_CAP_LDS; }
 hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
