@@ -1296,6 +1296,26 @@ __host__ __device__ inline size_t eig_lds_doubles_big(int k, int nvmax)
     return 10 * (size_t)k + 8 + 2 * (size_t)nvmax * k + 3 * (size_t)nvmax + 128;
 }
 
+// the split path's tridiagonalisation uses only Z, d, e, tau, vb and wb
+__host__ __device__ inline long tri_lds_doubles(int k) { return (long)k * eig_lda(k) + 3L * k + 128; }
+
+__device__ inline EigLds eig_carve_tri(double* s, int k)
+{
+    EigLds L = {};
+    L.Z = s;
+    s += (size_t)k * eig_lda(k);
+    L.d = s;
+    s += k;
+    L.e = s;
+    s += k;
+    L.tau = s;
+    s += k;
+    L.vb = s;
+    s += 64;
+    L.wb = s;
+    return L;
+}
+
 __device__ inline EigLds eig_carve(double* s, int k, int nvmax, double* zg = nullptr)
 {
     EigLds L;
@@ -2952,7 +2972,7 @@ __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict_
     if (p <= 0) return;
     double* rec = ws + (gp - gp0) * gstride;
     const SplitRec S = split_rec(kmax, nullptr);
-    EigLds L = eig_carve(smem, k, 0);
+    EigLds L = eig_carve_tri(smem, k);
     double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
     if ((J.flags & LEKF_F_MINFL) && J.minfl > 0.0 && alpha < J.minfl) alpha = J.minfl;
     const double shift = (double)(k - 1) * (1.0 / alpha);
@@ -3111,93 +3131,49 @@ void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
 // stream (column of each entry from the QL kernel, -1 for a sweep header),
 // forward for R and backward with the transposed rotations for R^T.
 // Results agree with the explicit form to rounding.
-constexpr int kEvfChunk = 16;  // rotation-stream entries loaded at once per lane
+constexpr int kEvfChunk = 24;  // rotation-stream entries loaded at once per lane
 
-// kEvfChunk entries of a lane's rotation stream: column (-1: a sweep header), c, s
-struct EvfChunk {
-    double c[kEvfChunk], s[kEvfChunk];
-    int i[kEvfChunk];
-};
-
-__device__ __forceinline__ void evf_load(EvfChunk& K, const double* rs, const short* rx, long e0,
-                                         long ntot)
+// The rows of a point group times a k x k matrix M (row-major, global
+// memory), in place in the rows: row_r <- row_r M for the group's G <= RM rows
+// (LDS, stride ld).  Lane v of the group forms columns 8v .. 8v + 7 of every
+// row (G * 8 >= k), so each element of M is loaded by one lane of the group
+// and the loads are contiguous; each sum runs over l in order, as the
+// one-row-per-lane form does.  Uniform call (barriers inside).
+template <int RM>
+__device__ __forceinline__ void evf_coop_mv(double* rows, int ld, int G, int k, const double* M,
+                                            int v, bool solve)
 {
+    double acc[RM][8];
 #pragma unroll
-    for (int u = 0; u < kEvfChunk; ++u) {
-        const long e = e0 + u;
-        const bool ok = e >= 0 && e < ntot;
-        K.i[u] = ok ? (int)rx[e] : -1;
-        K.c[u] = ok ? rs[2 * e] : 0.0;
-        K.s[u] = ok ? rs[2 * e + 1] : 0.0;
-    }
-}
-
-// Rotations of a chunk in stream order, (a_i, a_i+1) -> (c a_i - s a_i+1, s a_i + c a_i+1).
-// A sweep runs down the columns: rotation i after i+1 shares column i+1,
-// which is carried in cv (ci = its column) instead of going through LDS.
-__device__ __forceinline__ void evf_fwd(const EvfChunk& K, double* row, int& ci, double& cv)
-{
-    double pre = 0.0;
-    int pi = -2;  // column whose value pre holds
+    for (int r = 0; r < RM; ++r)
 #pragma unroll
-    for (int u = 0; u < kEvfChunk; ++u) {
-        const int i = K.i[u];
-        if (i >= 0) {
-            double a1;
-            if (ci == i + 1) {
-                a1 = cv;
-            } else {
-                if (ci >= 0) row[ci] = cv;
-                a1 = row[i + 1];
-            }
-            const double a0 = (pi == i) ? pre : row[i];
-            const double n0 = fma(K.c[u], a0, -K.s[u] * a1);
-            const double n1 = fma(K.s[u], a0, K.c[u] * a1);
-            if (u + 1 < kEvfChunk && K.i[u + 1] == i - 1) {
-                pi = i - 1;
-                pre = row[i - 1];
-            } else {
-                pi = -2;
-            }
-            row[i + 1] = n1;
-            ci = i;
-            cv = n0;
+        for (int j = 0; j < 8; ++j) acc[r][j] = 0.0;
+    const int c0 = v * 8;
+    const bool mine = solve && c0 < k;
+    if (mine) {
+        for (int l = 0; l < k; ++l) {
+            const double* ml = M + (size_t)l * k + c0;
+            double q[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) q[j] = (c0 + j < k) ? ml[j] : 0.0;
+#pragma unroll
+            for (int r = 0; r < RM; ++r)
+                if (r < G) {
+                    const double xr = rows[(size_t)r * ld + l];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc[r][j] = fma(xr, q[j], acc[r][j]);
+                }
         }
     }
-}
-
-// The transposed rotations of a chunk in reverse stream order,
-// (a_i, a_i+1) -> (c a_i + s a_i+1, -s a_i + c a_i+1): a reversed sweep runs up
-// the columns and carries the upper one.
-__device__ __forceinline__ void evf_bwd(const EvfChunk& K, double* row, int& ci, double& cv)
-{
-    double pre = 0.0;
-    int pi = -2;
+    __syncthreads();
+    if (mine)
 #pragma unroll
-    for (int u = kEvfChunk - 1; u >= 0; --u) {
-        const int i = K.i[u];
-        if (i >= 0) {
-            double a0;
-            if (ci == i) {
-                a0 = cv;
-            } else {
-                if (ci >= 0) row[ci] = cv;
-                a0 = row[i];
-            }
-            const double a1 = (pi == i + 1) ? pre : row[i + 1];
-            const double n0 = fma(K.c[u], a0, K.s[u] * a1);
-            const double n1 = fma(-K.s[u], a0, K.c[u] * a1);
-            if (u > 0 && K.i[u - 1] == i + 1) {
-                pi = i + 2;
-                pre = row[i + 2];
-            } else {
-                pi = -2;
-            }
-            row[i] = n0;
-            ci = i + 1;
-            cv = n1;
-        }
-    }
+        for (int r = 0; r < RM; ++r)
+            if (r < G)
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (c0 + j < k) rows[(size_t)r * ld + c0 + j] = acc[r][j];
+    __syncthreads();
 }
 
 template <int KR>
@@ -3277,12 +3253,20 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
         }
     }
     const bool act = solve && (isx || isb);
+    // the group's rows go through the matrix products together when they fit
+    // (G <= 8 rows, 8 columns per lane); else each lane its own row
+    const bool coop = G <= 8 && G * 8 >= kmax;
     // row <- x Q (Q row-major at qroff)
     if (act) {
         if (isx)
             for (int m = 0; m < k; ++m) row[m] = xsrc[m] - mean;
         else
             for (int i = 0; i < k; ++i) row[i] = rec[(size_t)k * k + i];
+    }
+    if (coop) {
+        __syncthreads();
+        evf_coop_mv<8>(sm + (size_t)grp * G * ld, ld, G, k, rec + S.qroff, v, solve);
+    } else if (act) {
         double y[KR];
 #pragma unroll
         for (int t = 0; t < KR; ++t) y[t] = 0.0;
@@ -3309,22 +3293,29 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
         const long t = __shfl_xor(emax, o, 64);
         emax = t > emax ? t : emax;
     }
-    // the stream is read in chunks of kEvfChunk entries, the next chunk's
-    // loads in flight while one is applied; within a sweep the shared column
-    // of consecutive rotations stays in a register (cv, column ci) and the
-    // next rotation's other column is read before this one writes (it lies
-    // below everything the sweep writes from here on)
-    EvfChunk ka, kb;
-    evf_load(ka, rs, rx, 0, ntot);
-    int ci = -1;
-    double cv = 0.0;
-    for (long e0 = 0; e0 < emax; e0 += 2 * kEvfChunk) {
-        evf_load(kb, rs, rx, e0 + kEvfChunk, ntot);
-        evf_fwd(ka, row, ci, cv);
-        evf_load(ka, rs, rx, e0 + 2 * kEvfChunk, ntot);
-        evf_fwd(kb, row, ci, cv);
+    // the stream is read kEvfChunk entries at a time, all loads issued before
+    // the chunk's rotations run
+    for (long e0 = 0; e0 < emax; e0 += kEvfChunk) {
+        double cc[kEvfChunk], ss[kEvfChunk];
+        int ii[kEvfChunk];
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const long e = e0 + u;
+            const bool ok = e < ntot;
+            ii[u] = ok ? (int)rx[e] : -1;
+            cc[u] = ok ? rs[2 * e] : 0.0;
+            ss[u] = ok ? rs[2 * e + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const int i = ii[u];
+            if (i >= 0) {
+                const double a0 = row[i], a1 = row[i + 1];
+                row[i] = fma(cc[u], a0, -ss[u] * a1);
+                row[i + 1] = fma(ss[u], a0, cc[u] * a1);
+            }
+        }
     }
-    if (ci >= 0) row[ci] = cv;
     __syncthreads();
     // u = (b^T V) / lambda in b's lane; truncation test on lambda
     double lmax = 0.0, lmin = 1.0e308;
@@ -3345,32 +3336,51 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
     }
     __syncthreads();
     // backward through R^T: the entries in reverse, each rotation transposed
-    {
-        // chunks from the last down, each applied in reverse
-        const long top = emax > 0 ? ((emax - 1) / kEvfChunk) * kEvfChunk : -kEvfChunk;
-        EvfChunk ka2, kb2;
-        evf_load(ka2, rs, rx, top, ntot);
-        ci = -1;
-        for (long e0 = top; e0 >= 0; e0 -= 2 * kEvfChunk) {
-            evf_load(kb2, rs, rx, e0 - kEvfChunk, ntot);
-            evf_bwd(ka2, row, ci, cv);
-            evf_load(ka2, rs, rx, e0 - 2 * kEvfChunk, ntot);
-            evf_bwd(kb2, row, ci, cv);
+    for (long e0 = emax > 0 ? ((emax - 1) / kEvfChunk) * kEvfChunk : -1; e0 >= 0;
+         e0 -= kEvfChunk) {
+        double cc[kEvfChunk], ss[kEvfChunk];
+        int ii[kEvfChunk];
+#pragma unroll
+        for (int u = 0; u < kEvfChunk; ++u) {
+            const long e = e0 + u;
+            const bool ok = e < ntot;
+            ii[u] = ok ? (int)rx[e] : -1;
+            cc[u] = ok ? rs[2 * e] : 0.0;
+            ss[u] = ok ? rs[2 * e + 1] : 0.0;
         }
-        if (ci >= 0) row[ci] = cv;
+#pragma unroll
+        for (int u = kEvfChunk - 1; u >= 0; --u) {
+            const int i = ii[u];
+            if (i >= 0) {
+                const double a0 = row[i], a1 = row[i + 1];
+                row[i] = fma(cc[u], a0, ss[u] * a1);
+                row[i + 1] = fma(-ss[u], a0, cc[u] * a1);
+            }
+        }
     }
-    // out = row Q^T (Q column-major at qoff: Q[m][t] at qoff + t * k + m)
+    // out = row Q^T (Q column-major at qoff: Q[m][t] at qoff + t * k + m, i.e.
+    // Q^T row-major)
+    if (coop) {
+        __syncthreads();
+        evf_coop_mv<8>(sm + (size_t)grp * G * ld, ld, G, k, rec + S.qoff, v, solve);
+    }
     if (act) {
         double out[KR];
 #pragma unroll
         for (int m = 0; m < KR; ++m) out[m] = 0.0;
-        const double* qc = rec + S.qoff;
-        for (int t = 0; t < k; ++t) {
-            const double zt = row[t];
-            const double* qt = qc + (size_t)t * k;
+        if (coop) {
 #pragma unroll
             for (int m = 0; m < KR; ++m)
-                if (m < k) out[m] = fma(zt, qt[m], out[m]);
+                if (m < k) out[m] = row[m];
+        } else {
+            const double* qc = rec + S.qoff;
+            for (int t = 0; t < k; ++t) {
+                const double zt = row[t];
+                const double* qt = qc + (size_t)t * k;
+#pragma unroll
+                for (int m = 0; m < KR; ++m)
+                    if (m < k) out[m] = fma(zt, qt[m], out[m]);
+            }
         }
         const LekfJob& J = *Jp;
         if (isx) {
@@ -3563,7 +3573,7 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
     const unsigned nwe = (unsigned)((nb + evp - 1) / evp);
     const size_t flds = 64 * (size_t)(kmax | 1) * sizeof(double);
     const size_t qlds = 2 * (size_t)kr * 64 * sizeof(double);
-    const size_t tlds = eig_lds_doubles(kmax, 0) * sizeof(double);
+    const size_t tlds = tri_lds_doubles(kmax) * sizeof(double);
     switch (kr) {
 #define LEKF_SPLIT_CASE(K)                                                                       \
     case K: {                                                                                    \
