@@ -82,7 +82,45 @@ def measured_traffic(name):
     return None
 
 
-def roofline(kt, nobsl, k, nv, steps, profile_name):
+def radar_scan_records(p):
+    """Obs records the regional selection scans per cycle: at every point, the obs
+    of the cells of its search rectangle's rows that can hold an obs within the
+    cutoff (rows trimmed as the selection trims them; the same horizontal set at
+    each of the nlev levels of a column).  Each record is 36 B (ri, rj, z, err and
+    the obs index)."""
+    nx, ny, nlev = p["nx"], p["ny"], p["nlev"]
+    dx, dy, sh = p["dx"], p["dy"], p["sigma_h"]
+    ci = np.clip(np.ceil(np.asarray(p["ri"]) - 0.5).astype(np.int64), 1, nx)
+    cj = np.clip(np.ceil(np.asarray(p["rj"]) - 0.5).astype(np.int64), 1, ny)
+    cnt = np.bincount((cj - 1) * nx + (ci - 1), minlength=nx * ny).reshape(ny, nx)
+    cum = np.zeros((ny, nx + 1), np.int64)
+    cum[:, 1:] = np.cumsum(cnt, axis=1)
+    dlon, dlat = sh * 3.651483717 / dx, sh * 3.651483717 / dy
+    r2cut = 13.33333333 * sh * sh * (1.0 + 1e-9)
+    iy, ix = np.meshgrid(np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    ri, rj = ix.astype(float), iy.astype(float)
+    imin = np.maximum(1, np.floor(ri - dlon).astype(np.int64))
+    imax = np.minimum(nx, np.ceil(ri + dlon).astype(np.int64))
+    jmin = np.maximum(1, np.floor(rj - dlat).astype(np.int64))
+    jmax = np.minimum(ny, np.ceil(rj + dlat).astype(np.int64))
+    tot = np.zeros((ny, nx), np.int64)
+    R = int(np.ceil(dlat)) + 1
+    for dj in range(-R, R + 1):
+        j = iy + dj
+        ok = (j >= jmin) & (j <= jmax)
+        dyg = np.maximum(0.0, np.abs(j - rj) - 0.5) * dy
+        rem = r2cut - dyg * dyg
+        ok &= rem >= 0.0
+        rx = np.sqrt(np.maximum(rem, 0.0)) / dx + 0.5
+        ilo = np.maximum(np.ceil(ri - rx).astype(np.int64), imin)
+        ihi = np.minimum(np.floor(ri + rx).astype(np.int64), imax)
+        ok &= ilo <= ihi
+        jj = np.clip(j, 1, ny) - 1
+        tot += np.where(ok, cum[jj, np.clip(ihi, 0, nx)] - cum[jj, np.clip(ilo - 1, 0, nx)], 0)
+    return int(tot.sum()) * nlev
+
+
+def roofline(kt, nobsl, k, nv, steps, profile_name, scan_records=None):
     """Roofline of the dominant kernel of the timed cycles, from the per-kernel
     HIP-event times kt = {name: (total ms, launches)}."""
     name, (ms, nl) = max(kt.items(), key=lambda kv: kv[1][0])
@@ -103,11 +141,15 @@ def roofline(kt, nobsl, k, nv, steps, profile_name):
         achieved = fl / tot_s / 1e12
         work = {"flops_per_cycle": float(fl), "kernels": names}
     else:
-        # selection: lists written (index + rho, 12 B per kept entry)
-        by = 12.0 * float(nobsl.sum())
+        # selection, a scan: the obs records read (36 B per candidate and point)
+        # and the lists written (index + rho, 12 B per kept entry)
+        lb = 12.0 * float(nobsl.sum())
+        sb = 36.0 * float(scan_records) if scan_records else 0.0
+        by = sb + lb
         bound, unit, peak = "hbm", "GB/s", HBM_PEAK_GBS
         achieved = by / per_cycle_s / 1e9
-        work = {"bytes_per_cycle": by}
+        work = {"bytes_per_cycle": by, "bytes_per_launch": by / launches,
+                "scan_bytes_per_cycle": sb, "list_bytes_per_cycle": lb}
     return {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": measured_traffic(profile_name),
             "ms_per_cycle": ms / steps, "launches_per_cycle": launches, **work}
@@ -263,7 +305,8 @@ def radar_headline(args):
         f"({oracle.ref_threads()} threads, nproc {cores['nproc']})")
 
     f64 = forms["f64"]
-    rl = roofline(res64["kt"], nobsl, k, nv, args.steps, "radar_256x256x60_f64.md")
+    rl = roofline(res64["kt"], nobsl, k, nv, args.steps, "radar_256x256x60_f64.md",
+                  scan_records=radar_scan_records(p))
     out = {
         "metric": METRIC, "value": f64["local_analyses_per_s"], "unit": "local analyses/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -64,6 +64,34 @@ __device__ __forceinline__ double rn_div(double a, double b)
 
 // job of flat point gp: the last job whose pt_begin <= gp (binary search;
 // pt_begin increases with the job index)
+// x of lane (lane ^ j), for the constant j of an unrolled loop: quad
+// permutes for 1 and 2, row shifts (lane n reads n + 4 or n - 4) for 4, a
+// rotation by half a row for 8; for 16 and 32 the permlane swaps, which
+// return x of the row (half) above in the even rows (lower half) and x of
+// the row (half) below in the odd rows (upper half)
+__device__ __forceinline__ int lane_xor(int x, int j, int lane)
+{
+    switch (j) {
+    case 16: {
+        const auto p = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
+        return (int)((lane & 16) ? p[0] : p[1]);
+    }
+    case 32: {
+        const auto p = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false);
+        return (int)((lane & 32) ? p[0] : p[1]);
+    }
+    case 1: return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    case 4: {
+        const int up = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const int dn = __builtin_amdgcn_mov_dpp(x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4) ? dn : up;
+    }
+    case 8: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    default: return __shfl_xor(x, j, 64);
+    }
+}
+
 __device__ __forceinline__ int lekf_job_of(const LekfJob* jobs, int njobs, long gp)
 {
     int lo = 0, hi = njobs - 1;
