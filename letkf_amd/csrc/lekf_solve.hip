@@ -2668,7 +2668,9 @@ __host__ __device__ inline long split_stride(int kmax)
 __host__ __device__ inline long split_rcap(int kmax) { return 2L * kmax * kmax + 8L * kmax; }
 // per point: the column of each rotation-stream entry (-1: a sweep header),
 // after the streams, in doubles
-__host__ __device__ inline long split_ridx_doubles(int kmax) { return (split_rcap(kmax) + 3) / 4; }
+// shorts per point of the column stream, a multiple of 4 (8-byte rows)
+__host__ __device__ inline long split_ridx_stride(int kmax) { return (split_rcap(kmax) + 3) & ~3L; }
+__host__ __device__ inline long split_ridx_doubles(int kmax) { return split_ridx_stride(kmax) / 4; }
 
 __device__ inline SplitRec split_rec(int kmax, const double* rot)
 {
@@ -3023,7 +3025,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
     const long stride = split_rcap(kmax);
     const long rcap = rlim > 0 && rlim < stride ? rlim : stride;
     double* rs = rot + pt * 2 * stride;
-    short* rx = ridx + pt * stride;
+    short* rx = ridx + pt * split_ridx_stride(kmax);
     const SplitRec S = split_rec(kmax, nullptr);
     double* d = sm + lane;
     double* e = sm + (long)KR * 64 + lane;
@@ -3132,6 +3134,33 @@ void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
 // forward for R and backward with the transposed rotations for R^T.
 // Results agree with the explicit form to rounding.
 constexpr int kEvfChunk = 24;  // rotation-stream entries loaded at once per lane
+
+// Entries e0 .. e0 + kEvfChunk - 1 of a lane's rotation stream (e0 a multiple
+// of 4): (c, s) pairs as 16-byte loads, columns four at a time (the column
+// rows are 8-byte aligned and padded to a multiple of 4); entries from ntot on
+// have column -1.
+__device__ __forceinline__ void evf_chunk(double* cc, double* ss, int* ii, const double* rs,
+                                          const short* rx, long e0, long ntot)
+{
+#pragma unroll
+    for (int u = 0; u < kEvfChunk; u += 4) {
+        const long e = e0 + u;
+        short4 q = make_short4(-1, -1, -1, -1);
+        if (e < ntot) q = *reinterpret_cast<const short4*>(rx + e);
+        ii[u] = e < ntot ? q.x : -1;
+        ii[u + 1] = e + 1 < ntot ? q.y : -1;
+        ii[u + 2] = e + 2 < ntot ? q.z : -1;
+        ii[u + 3] = e + 3 < ntot ? q.w : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kEvfChunk; ++u) {
+        const long e = e0 + u;
+        double2 cs = make_double2(0.0, 0.0);
+        if (e < ntot) cs = *reinterpret_cast<const double2*>(rs + 2 * e);
+        cc[u] = cs.x;
+        ss[u] = cs.y;
+    }
+}
 
 // The rows of a point group times a k x k matrix M (row-major, global
 // memory), in place in the rows: row_r <- row_r M for the group's G <= RM rows
@@ -3286,7 +3315,7 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
     // replay applies it
     const long ntot = act ? (long)rec[S.cntoff] : 0;
     const double* rs = rot + pt * 2 * rstride;
-    const short* rx = ridx + pt * rstride;
+    const short* rx = ridx + pt * split_ridx_stride(kmax);
     long emax = ntot;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -3298,14 +3327,7 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
     for (long e0 = 0; e0 < emax; e0 += kEvfChunk) {
         double cc[kEvfChunk], ss[kEvfChunk];
         int ii[kEvfChunk];
-#pragma unroll
-        for (int u = 0; u < kEvfChunk; ++u) {
-            const long e = e0 + u;
-            const bool ok = e < ntot;
-            ii[u] = ok ? (int)rx[e] : -1;
-            cc[u] = ok ? rs[2 * e] : 0.0;
-            ss[u] = ok ? rs[2 * e + 1] : 0.0;
-        }
+        evf_chunk(cc, ss, ii, rs, rx, e0, ntot);
 #pragma unroll
         for (int u = 0; u < kEvfChunk; ++u) {
             const int i = ii[u];
@@ -3340,14 +3362,7 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
          e0 -= kEvfChunk) {
         double cc[kEvfChunk], ss[kEvfChunk];
         int ii[kEvfChunk];
-#pragma unroll
-        for (int u = 0; u < kEvfChunk; ++u) {
-            const long e = e0 + u;
-            const bool ok = e < ntot;
-            ii[u] = ok ? (int)rx[e] : -1;
-            cc[u] = ok ? rs[2 * e] : 0.0;
-            ss[u] = ok ? rs[2 * e + 1] : 0.0;
-        }
+        evf_chunk(cc, ss, ii, rs, rx, e0, ntot);
 #pragma unroll
         for (int u = kEvfChunk - 1; u >= 0; --u) {
             const int i = ii[u];
