@@ -2987,10 +2987,10 @@ __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict_
     __syncthreads();
     eig_tridiag64(L, k, lda, lane);
     if (lane < k) {
+        // Z[r * lda + c] = Q[r][c]; both copies are written a row at a time
         for (int t = 0; t < k; ++t) {
-            const double z = L.Z[lane * lda + t];
-            rec[S.qoff + (long)t * k + lane] = z;
-            rec[S.qroff + (long)lane * k + t] = z;
+            rec[S.qoff + (long)t * k + lane] = L.Z[lane * lda + t];   // Q[lane][t], column-major
+            rec[S.qroff + (long)t * k + lane] = L.Z[t * lda + lane];  // Q[t][lane], row-major
         }
         rec[S.doff + lane] = L.d[lane];
         rec[S.eoff + lane] = L.e[lane];
@@ -3076,8 +3076,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
                 g = fma(c, r, -bb);
                 di1 = di;
                 if (nr < rcap) {
-                    rs[2 * nr] = c;
-                    rs[2 * nr + 1] = sn;
+                    *reinterpret_cast<double2*>(rs + 2 * nr) = make_double2(c, sn);
                     rx[nr] = (short)i;
                 }
                 ++nr;
@@ -3089,8 +3088,7 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
                 e[m * 64] = 0.0;
             }
             if (hpos < rcap) {
-                rs[2 * hpos] = (double)m;
-                rs[2 * hpos + 1] = (double)cnt;
+                *reinterpret_cast<double2*>(rs + 2 * hpos) = make_double2((double)m, (double)cnt);
                 rx[hpos] = -1;
             }
         }
