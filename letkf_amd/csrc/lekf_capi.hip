@@ -743,7 +743,7 @@ struct lekf_handle {
     DevBuf sel_key, sel_ob, sel_rho;  // per-slot bucket streams of the capped regional selection
     DevBuf sel_gkey, sel_gpos, sel_gslot;  // per-slot kept sets of caps above LEKF_CAP_LDS
     // the sorted obs bucket by bucket, for the LDS-resident capped selection
-    DevBuf b_cnt, b_cc, b_rec, b_o;
+    DevBuf b_cnt, b_cc, b_rec, b_o, b_err2min;
     DevBuf wc_so;                    // its slot -> obs maps, one per resident wave
     DevBuf maxcand;
     long slot_cap = 0;
@@ -778,7 +778,7 @@ long handle_bytes(const lekf_handle* h)
                           &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos, &h->sel_gslot,
                           &h->maxcand, &h->capl_off, &h->capl_cnt,
                           &h->capl_idx, &h->capl_rho, &h->d_hgt, &h->b_cnt, &h->b_cc,
-                          &h->b_rec, &h->b_o, &h->wc_so};
+                          &h->b_rec, &h->b_o, &h->b_err2min, &h->wc_so};
     long s = 0;
     for (auto* b : bs) s += (long)b->n;
     return s;
@@ -799,7 +799,7 @@ void free_all(lekf_handle* h)
                     &h->gws, &h->sel_key, &h->sel_ob, &h->sel_rho, &h->sel_gkey, &h->sel_gpos,
                     &h->sel_gslot, &h->maxcand, &h->capl_off,
                     &h->capl_cnt, &h->capl_idx, &h->capl_rho, &h->d_hgt, &h->b_cnt, &h->b_cc,
-                    &h->b_rec, &h->b_o, &h->wc_so};
+                    &h->b_rec, &h->b_o, &h->b_err2min, &h->wc_so};
     for (auto* b : bs) b->release();
 }
 
@@ -939,6 +939,7 @@ RegionalArgs make_reg_args(const lekf_handle* h)
     R.bcc = h->b_cc.as<int>();
     R.brec = h->b_rec.as<double>();
     R.bo = h->b_o.as<int>();
+    R.err2_min = h->b_err2min.as<unsigned long long>();
     R.wc_so = h->wc_so.as<int>();
     R.wc_so_waves = h->wc_so.p ? lekf_reg_wave_slots_max() : 0;
     R.list_off = h->cap_off.as<long>();
@@ -1958,9 +1959,11 @@ static int regional_obs_index(lekf_handle* h, int nobs, bool has_qc, int qc_keep
         HIPCHK(h->b_rec.ensure(4 * sizeof(double) * nb1));
         HIPCHK(h->b_o.ensure(sizeof(int) * nb1));
         HIPCHK(h->wc_so.ensure(sizeof(int) * LEKF_CAP_LDS * (size_t)lekf_reg_wave_slots_max()));
+        HIPCHK(h->b_err2min.ensure(sizeof(unsigned long long) * LEKF_REG_NB));
+        HIPCHK(hipMemsetAsync(h->b_err2min.p, 0xff, sizeof(unsigned long long) * LEKF_REG_NB, st));
         HIPCHK(lekf_reg_buckets_launch(A, h->nbinned, h->cellcum.as<int>(), h->nbkt,
                                        h->b_cnt.as<int>(), h->b_cc.as<int>(), h->b_rec.as<double>(),
-                                       h->b_o.as<int>(), st));
+                                       h->b_o.as<int>(), h->b_err2min.as<unsigned long long>(), st));
     }
     // list capacity per point -> chunks of points whose lists fit the budget
     const long npt = (long)h->nij * h->nlev;

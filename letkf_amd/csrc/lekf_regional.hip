@@ -948,6 +948,8 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
     RowTab& T = R[wv];
     int* so = A.wc_so + ((size_t)blockIdx.x * NW + wv) * LEKF_CAP_LDS;  // slot -> obs
     const double r2cut = kNdMax * A.sigma_h * A.sigma_h * (1.0 + 1e-9);
+    // the bucket's smallest err^2 (NaN for an empty bucket: never used)
+    const double e2min = A.err2_min ? __longlong_as_double((long long)A.err2_min[wv]) : 0.0;
 
     // XCD-aware tile order: workgroup w runs on XCD w % 8 (round-robin
     // dispatch); the XCD's workgroups take the points of one kTileX x kTileY
@@ -1045,6 +1047,17 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
                 };
                 fetch(0);
                 for (int q0 = 0; q0 < tot; q0 += 64) {
+                    if (S.n >= cap && (double)(j0 + r0) - rj > 0.5) {
+                        // rows from this chunk's first one on lie beyond the
+                        // point: an obs of row j has |rj_o - rj| > j - rj - 0.5,
+                        // so nd > ((j - rj - 0.5) dy / sigma_h)^2 and its key
+                        // err^2 / rho exceeds err2_min exp(nd / 2); once that
+                        // bound (less a 1e-9 margin for rounding) reaches tau,
+                        // which only falls, no later candidate can enter
+                        const double g = ((double)(j0 + r0) - rj - 0.5) * A.dy / A.sigma_h;
+                        const double lb = e2min * exp(0.5 * g * g) * (1.0 - 1e-9);
+                        if (lb >= __longlong_as_double((long long)S.tau)) break;
+                    }
                     const ObsRec cur = nrec;
                     const int co = no;
                     if (q0 + 64 < tot)
@@ -1103,7 +1116,8 @@ __global__ void reg_bucket_count(const int* cellcum, const int* obkt, int ncell,
 
 __global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const double* orj,
                                    const double* oz, const double* oerr, const int* obkt,
-                                   int ncell, int nbkt, const int* bcc, double* brec, int* bo)
+                                   int ncell, int nbkt, const int* bcc, double* brec, int* bo,
+                                   unsigned long long* err2_min)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -1115,6 +1129,7 @@ __global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const 
         bs += cc[ncell - 1];
     }
     const int beg = c > 0 ? cellcum[c - 1] : 0, end = cellcum[c];
+    unsigned long long e2[LEKF_REG_NB] = {~0ull, ~0ull, ~0ull, ~0ull};
     for (int i = beg; i < end; ++i) {
         const int t = obkt[i];
         if (t < 0 || t >= nbkt) continue;
@@ -1125,7 +1140,13 @@ __global__ void reg_bucket_scatter(const int* cellcum, const double* ori, const 
         r[2] = oz[i];
         r[3] = oerr[i];
         bo[p] = i;
+        // err^2 >= 0: its bits order as the values do
+        const unsigned long long b2 = (unsigned long long)__double_as_longlong(rn_mul(oerr[i], oerr[i]));
+        for (int u = 0; u < LEKF_REG_NB; ++u)
+            if (u == t && b2 < e2[u]) e2[u] = b2;
     }
+    for (int u = 0; u < nbkt && u < LEKF_REG_NB; ++u)
+        if (e2[u] != ~0ull) atomicMin(&err2_min[u], e2[u]);
 }
 
 // Nearest-first cap of the global grid's per-point lists: a list longer than
@@ -1443,7 +1464,8 @@ hipError_t lekf_reg_select_launch(const RegionalArgs& A, long npt, int nslots, h
 }
 
 hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
-                                   int* bcnt, int* bcc, double* brec, int* bo, hipStream_t st)
+                                   int* bcnt, int* bcc, double* brec, int* bo,
+                                   unsigned long long* err2_min, hipStream_t st)
 {
     (void)nbinned;
     const int ncell = A.nx * A.ny;
@@ -1455,6 +1477,6 @@ hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* 
         hipLaunchKernelGGL(scan_inclusive_int, dim3(1), dim3(1024), 0, st, bcnt + (size_t)t * ncell,
                            bcc + (size_t)t * ncell, ncell);
     hipLaunchKernelGGL(reg_bucket_scatter, dim3(nb), dim3(256), 0, st, cellcum, A.o_ri, A.o_rj,
-                       A.o_z, A.o_err, A.o_bkt, ncell, nbkt, bcc, brec, bo);
+                       A.o_z, A.o_err, A.o_bkt, ncell, nbkt, bcc, brec, bo, err2_min);
     return hipGetLastError();
 }

@@ -40,6 +40,8 @@ struct RegionalArgs {
     const int* bcc;
     const double* brec;
     const int* bo;
+    // per bucket, the smallest err^2 of its obs (double bits)
+    const unsigned long long* err2_min;
     // the fused selection's slot -> obs map, LEKF_CAP_LDS ints per wave, for
     // wc_so_waves waves (the launch holds no more waves than that)
     int* wc_so;
@@ -94,7 +96,8 @@ hipError_t lekf_reg_keys_launch(const RegObsArgs& A, hipStream_t st);
 // the bucket-by-bucket copy of the sorted obs (RegionalArgs bcc, bri .. bo):
 // bcnt (nbkt * ncell ints) is work space
 hipError_t lekf_reg_buckets_launch(const RegObsArgs& A, int nbinned, const int* cellcum, int nbkt,
-                                   int* bcnt, int* bcc, double* brec, int* bo, hipStream_t st);
+                                   int* bcnt, int* bcc, double* brec, int* bo,
+                                   unsigned long long* err2_min, hipStream_t st);
 // the capped selection keeps its kept sets in LDS, one wave per bucket
 inline bool lekf_reg_select_fused(int nobs_max) { return nobs_max > 0 && nobs_max <= LEKF_CAP_LDS; }
 hipError_t lekf_reg_scatter_launch(const RegObsArgs& A, int nbinned, int ncell, int* cellcum,
