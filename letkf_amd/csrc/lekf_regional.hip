@@ -950,6 +950,8 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
     const double r2cut = kNdMax * A.sigma_h * A.sigma_h * (1.0 + 1e-9);
     // the bucket's smallest err^2 (NaN for an empty bucket: never used)
     const double e2min = A.err2_min ? __longlong_as_double((long long)A.err2_min[wv]) : 0.0;
+    const float ish2 = (float)(1.0 / (A.sigma_h * A.sigma_h));
+    const float isv2 = (float)(1.0 / (A.sigma_v * A.sigma_v));
 
     // XCD-aware tile order: workgroup w runs on XCD w % 8 (round-robin
     // dispatch); the XCD's workgroups take the points of one kTileX x kTileY
@@ -1067,7 +1069,21 @@ __global__ __launch_bounds__(64 * NW) void reg_select_wave(RegionalArgs A, long 
                     Cand c;
                     c.ok = false;
                     c.bits = 0;
-                    if (co >= 0) c = eval_rec(A, cur, co, ri, rj, rz);
+                    // once the kept set is full, a float bound in the log2 domain
+                    // (errors near 1e-5 for candidates within the cutoff, against
+                    // a 1e-3 margin) skips the exact evaluation of a candidate whose
+                    // key certainly exceeds tau: it could not enter, and past the
+                    // fill the stream count no longer matters
+                    bool need = co >= 0;
+                    if (need && S.n >= cap) {
+                        const float fx = (float)((ri - cur.ri) * A.dx);
+                        const float fy = (float)((rj - cur.rj) * A.dy);
+                        const float fz = (float)(cur.z - rz);
+                        const float nd = (fx * fx + fy * fy) * ish2 + fz * fz * isv2;
+                        const float l2 = __log2f((float)(cur.err * cur.err)) + 0.72134752f * nd;
+                        need = !(l2 > __log2f((float)__longlong_as_double((long long)S.tau)) + 1e-3f);
+                    }
+                    if (need) c = eval_rec(A, cur, co, ri, rj, rz);
                     wc_push(W, so, S, c.ok, c.bits, co, cap, lane);
                 }
                 wave_sync();
