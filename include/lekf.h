@@ -409,7 +409,7 @@ int lekf_event_sync(void *ev);
  * the partial-sum reduction only, the eigensolver choice agrees to rounding.  Handles recapture their graphs after a
  * change.  lekf_tune_get returns the value, or a negative error code. */
 #define LEKF_TUNE_GRAM_WAVES 0 /* waves per point of the k > 32 Gram kernel: 2, 4 or 8 */
-#define LEKF_TUNE_GRAM_WS_MB 1 /* Gram workspace per launch, MiB, 16 .. 65536 */
+#define LEKF_TUNE_GRAM_WS_MB 1 /* Gram workspace per launch, MiB, 16 .. 65536 (16384) */
 #define LEKF_TUNE_SEL_PER_CU 2 /* capped-selection workgroups per CU: 1 .. 4 */
 #define LEKF_TUNE_EIG_SPLIT 3  /* 32 < k <= 64: 1 = tridiagonalisation, QL (one lane per point)
                                   and apply kernels (the apply pushes the state rows through
@@ -439,6 +439,11 @@ int lekf_ktime_read(const char *name, double *total_ms, long *launches);
 int lekf_device_count(void);
 /* free and total device memory of the current GPU, bytes */
 int lekf_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* device bytes of the local-solve workspace a grid of npt points at ensemble
+ * size k takes under the current tuning (0 for k <= 32), for callers that
+ * budget device memory (no counterpart in the reference, which has no device
+ * workspace) */
+long lekf_solve_workspace_bytes(long npt, int k);
 const char *lekf_version(void);
 const char *lekf_last_error(void);
 

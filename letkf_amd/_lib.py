@@ -241,6 +241,8 @@ def lib():
     L.lekf_event_sync.argtypes = [c_void_p]
     L.lekf_mem_info.restype = c_int
     L.lekf_mem_info.argtypes = [POINTER(ctypes.c_size_t), POINTER(ctypes.c_size_t)]
+    L.lekf_solve_workspace_bytes.restype = c_long
+    L.lekf_solve_workspace_bytes.argtypes = [c_long, c_int]
     L.lekf_das_run_batch.restype = c_int
     L.lekf_das_run_batch.argtypes = [POINTER(c_void_p), c_int, c_void_p]
     L.lekf_l96_forecast.restype = c_int
@@ -279,6 +281,7 @@ EXPORTED = [
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",
     "lekf_event_create", "lekf_event_destroy", "lekf_event_record", "lekf_event_elapsed_ms",
     "lekf_event_query", "lekf_event_sync", "lekf_das_run_batch", "lekf_mem_info",
+    "lekf_solve_workspace_bytes",
     "lekf_set_obs_global_hx", "lekf_obsope_global", "lekf_set_graph", "lekf_tune_set",
     "lekf_tune_get", "lekf_ktime_enable", "lekf_ktime_read", "lekf_set_relax", "lekf_l96_forecast", "lekf_set_heights",
     "lekf_obsope_radar", "lekf_obs_departure_scale", "lekf_set_obs_radar",

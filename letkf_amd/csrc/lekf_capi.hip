@@ -544,6 +544,12 @@ int lekf_mem_info(size_t* free_bytes, size_t* total_bytes)
     return LEKF_OK;
 }
 
+long lekf_solve_workspace_bytes(long npt, int k)
+{
+    if (npt <= 0 || k < 2 || k > LEKF_KMAX) return 0;
+    return (long)lekf_solve_ws_bytes(npt, k);
+}
+
 int lekf_core_f64(int ne, int nobs, int nobsl, const double* hdxb, const double* rdiag,
                   const double* rloc, const double* dep, double* parm_infl, double* trans,
                   double* transm, double* pao, int rdiag_wloc, const double* minfl)

@@ -85,12 +85,8 @@ def estimate_bytes(job: Job) -> int:
         nobs = len(o["ri"])
         lists = npt * min(nobs, 2 * max(c.get("nobs_max", 1000), 1))
     extra = 64 << 20
-    if ne > _lib.LEKF_KSPLIT:
-        # per-slot records of the large-k solve: Gram/working matrix + rotation buffer
-        rec = ((ne * ne + ne + 3) & ~1) + 2 * (2 * ne * ne + 4 * ne)
-        extra += min(npt, 1024) * rec * 8
-    elif ne > 32:
-        extra += 1 << 30  # Gram workspace of the k > 32 solve
+    # the local solve's workspace (k > 32) under the current tuning
+    extra += int(_lib.lib().lekf_solve_workspace_bytes(npt, ne))
     if job.kind == "regional":
         extra += 1024 * min(nobs, 1 << 16) * 20  # selection scratch slots
     return int(2 * npt * nv * ne * 8 + nobs * (ne * 12 + 160) + lists * 12 + extra)

@@ -31,7 +31,7 @@ from .device import Event
 
 PARAMS = {
     "gram_waves": (_lib.LEKF_TUNE_GRAM_WAVES, (2, 4, 8)),
-    "gram_ws_mb": (_lib.LEKF_TUNE_GRAM_WS_MB, (256, 1024, 4096)),
+    "gram_ws_mb": (_lib.LEKF_TUNE_GRAM_WS_MB, (1024, 4096, 16384)),
     "sel_per_cu": (_lib.LEKF_TUNE_SEL_PER_CU, (1, 2, 3, 4)),
     "eig_split": (_lib.LEKF_TUNE_EIG_SPLIT, (1, 0)),
 }

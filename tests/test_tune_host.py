@@ -52,3 +52,21 @@ def test_cache_load(tmp_path):
     cur = tune.current()
     assert {n: cur[n] for n in ("gram_waves", "gram_ws_mb", "sel_per_cu")} == \
         {"gram_waves": 2, "gram_ws_mb": 256, "sel_per_cu": 1}
+
+
+def test_solve_workspace_bytes_follows_the_tuning():
+    """lekf_solve_workspace_bytes: none for k <= 32, one record per point up to
+    the tuned workspace (at least 1024 points on the split path), then flat."""
+    L = _lib.lib()
+    assert L.lekf_solve_workspace_bytes(1000, 20) == 0
+    assert L.lekf_solve_workspace_bytes(0, 50) == 0
+    assert L.lekf_solve_workspace_bytes(1000, 300) == 0        # above LEKF_KMAX
+    rec = L.lekf_solve_workspace_bytes(1, 50)
+    assert rec > 50 * 50 * 8                                    # a record holds C at least
+    assert L.lekf_solve_workspace_bytes(10, 50) == 10 * rec
+    tune.set("gram_ws_mb", 1024)
+    small = L.lekf_solve_workspace_bytes(10 ** 7, 50)
+    assert small // rec == max(1024, (1024 << 20) // rec)
+    tune.set("gram_ws_mb", 4096)
+    assert L.lekf_solve_workspace_bytes(10 ** 7, 50) == (4096 << 20) // rec * rec
+    assert L.lekf_solve_workspace_bytes(10 ** 7, 50) == L.lekf_solve_workspace_bytes(10 ** 8, 50)
