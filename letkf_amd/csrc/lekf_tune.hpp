@@ -4,7 +4,7 @@
 #include <cstddef>
 
 struct LekfTune {
-    int gram_waves = 4;                 // waves per point of the k > 32 Gram kernel: 2, 4, 8
+    int gram_waves = 2;                 // waves per point of the k > 32 Gram kernel: 2, 4, 8
     size_t gram_ws_bytes = 16ull << 30; // Gram workspace per launch (points per sub-batch)
     int sel_per_cu = 3;                 // selection workgroups (scratch slots) per CU: 1..4
     int eig_split = 1;                  // 32 < k <= 64: three-kernel eigensolver (1) or fused (0)
