@@ -2,9 +2,10 @@
 
 The GPU form (lekf_solve.hip gram_bf16x3) rounds each row x of [Y d] and its
 weighted copy w x to FP32, splits both into BF16 high and low parts, sums
-AhBh + AhBl + AlBh in FP32 per 32-obs MFMA step and per wave (a quarter of the
-list), and adds the waves' partial Grams in FP64; the eigensolver, W and the
-apply are FP64 (common/common_letkf.f90:101-118 for the product it replaces).
+AhBh + AhBl + AlBh in FP32 per 32-obs MFMA step and per wave (half the list at
+the default two waves per point), and adds the waves' partial Grams in FP64;
+the eigensolver, W and the apply are FP64 (common/common_letkf.f90:101-118 for
+the product it replaces).
 This emulation reproduces those roundings with numpy and shows that the
 relative error of the Gram is about 1e-6 whatever the weights, and that the
 analysis error grows with the condition number of C = Y^T R^-1 Y + (k-1)/a I:
@@ -31,10 +32,10 @@ def _gram_bf16x3(Y, w, d):
     ah = _bf16(xa)
     al = _bf16(xa - ah)
     G = np.zeros((K + 1, K + 1))
-    for w0 in range(0, P, P // 4):
+    for w0 in range(0, P, P // 2):
         acc = np.zeros((K + 1, K + 1), np.float32)
-        for q0 in range(w0, w0 + P // 4, 32):
-            s = slice(q0, min(q0 + 32, w0 + P // 4))
+        for q0 in range(w0, w0 + P // 2, 32):
+            s = slice(q0, min(q0 + 32, w0 + P // 2))
             for A, B in ((al, bh), (ah, bl), (ah, bh)):
                 acc = (acc + (A[s].astype(np.float64).T @ B[s].astype(np.float64)).astype(
                     np.float32)).astype(np.float32)
