@@ -253,6 +253,7 @@ def radar_headline(args):
         a3, _ = h.analysis()
         xa_s[prec] = np.ascontiguousarray(a3[s_ij, s_lev].transpose(0, 2, 1))
         del a3
+        nfp64 = h.fp64_count()
         h.close()
         ms = r["wall_ms"] / steps
         f = {"local_analyses_per_s": npt / (ms / 1e3), "ms_per_step": ms, "steps": steps,
@@ -265,6 +266,8 @@ def radar_headline(args):
              "latency_with_set_obs_ms": {"p50": percentile(r["lat"], 50) + r["obs_ms"],
                                          "p99": percentile(r["lat"], 99) + r["obs_ms"]},
              "kernels": kernel_table(r["kt"], steps)}
+        if prec != "f64":
+            f["fp64_redo_points"] = nfp64  # condition number above the mixed_kappa threshold
         forms[prec] = f
         if prec == "f64":
             res64 = r
@@ -467,10 +470,13 @@ def gfs_bench(steps=3, precs=("f64", "f32", "bf16x3")):
         ktime_enable(False)
         a3, _ = h.analysis()
         nl = h.nobsl()
+        nfp64 = h.fp64_count()
         h.close()
         r = {"local_analyses_per_s": npt / (np.mean(lat) / 1e3), "cycle_ms": float(np.mean(lat)),
              "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
              "mean_nobsl": float(nl.mean()), "kernels": kernel_table(kt, steps)}
+        if prec != "f64":
+            r["fp64_redo_points"] = nfp64
         if ref is None:
             ref = a3
         else:

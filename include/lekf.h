@@ -64,6 +64,8 @@ extern "C" {
 /* per-point status bits (lekf_job.status) */
 #define LEKF_PT_RANKDEF 1
 #define LEKF_PT_NOCONV 2
+#define LEKF_PT_FP64 4       /* mixed forms: C was worse conditioned than LEKF_TUNE_MIXED_KAPPA
+                                and the point was solved again from an FP64 Gram */
 
 #define LEKF_NVMAX 16        /* state rows per point in the fused apply */
 #define LEKF_KMAX 256        /* largest ensemble size */
@@ -374,6 +376,8 @@ int lekf_get_local(lekf_handle *h, long b, int cap, int *idx, double *rho);
 int lekf_get_obs_index(lekf_handle *h, int *perm, int *nobsgrd, int *nbinned);
 /* number of points flagged rank-deficient / not converged in the last run */
 int lekf_get_flags(lekf_handle *h, long *n_rankdef, long *n_noconv);
+/* number of points of the last run solved again in FP64 (LEKF_PT_FP64) */
+int lekf_get_fp64_count(lekf_handle *h, long *n_fp64);
 /* use the analysis as the next background (cycling without host traffic) */
 int lekf_cycle_swap(lekf_handle *h);
 /* device pointers of the resident ensembles (for device-side pipelines) */
@@ -420,6 +424,11 @@ int lekf_event_sync(void *ev);
                                   eigensolver; a point whose stream is longer is solved by
                                   the fused kernel's steps instead; 0: the default 2k^2 + 8k
                                   (smaller values test that fallback) */
+#define LEKF_TUNE_MIXED_KAPPA 5 /* split-BF16x3 and FP32 forms on the three-kernel eigensolver
+                                   (32 < k <= 64): a point whose C + (k-1)/alpha I has condition
+                                   number above this value (16 times it for FP32) is solved
+                                   again from an FP64 Gram (status bit LEKF_PT_FP64);
+                                   0 = never (0 .. 1e12; 500) */
 int lekf_tune_set(int param, long value);
 long lekf_tune_get(int param);
 

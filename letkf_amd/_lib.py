@@ -49,9 +49,11 @@ LEKF_TUNE_GRAM_WS_MB = 1
 LEKF_TUNE_SEL_PER_CU = 2
 LEKF_TUNE_EIG_SPLIT = 3
 LEKF_TUNE_SPLIT_RCAP = 4
+LEKF_TUNE_MIXED_KAPPA = 5
 LEKF_NOALLOC_BYTES = 256 << 20
 LEKF_NOALLOC_MAX_JOBS = 1024
 LEKF_PT_NOCONV = 2
+LEKF_PT_FP64 = 4
 LEKF_NVMAX = 16
 LEKF_KMAX = 256
 LEKF_KSPLIT = 95
@@ -205,6 +207,8 @@ def lib():
     L.lekf_get_obs_index.argtypes = [c_void_p, _ip, _ip, _ip]
     L.lekf_get_flags.restype = c_int
     L.lekf_get_flags.argtypes = [c_void_p, _lp, _lp]
+    L.lekf_get_fp64_count.restype = c_int
+    L.lekf_get_fp64_count.argtypes = [c_void_p, _lp]
     L.lekf_cycle_swap.restype = c_int
     L.lekf_cycle_swap.argtypes = [c_void_p]
     L.lekf_device_state.restype = c_int
@@ -275,7 +279,7 @@ EXPORTED = [
     "lekf_l96_create", "lekf_regional_create", "lekf_destroy", "lekf_set_ensemble",
     "lekf_set_infl", "lekf_set_obs_global", "lekf_set_obs_l96", "lekf_set_obs_regional",
     "lekf_das_run", "lekf_das_select", "lekf_das_solve", "lekf_set_stream", "lekf_das_run_many", "lekf_sync", "lekf_get_analysis", "lekf_get_infl",
-    "lekf_get_nobsl", "lekf_get_local", "lekf_get_obs_index", "lekf_get_flags",
+    "lekf_get_nobsl", "lekf_get_local", "lekf_get_obs_index", "lekf_get_flags", "lekf_get_fp64_count",
     "lekf_cycle_swap", "lekf_device_state", "lekf_device_bytes", "lekf_stream",
     "lekf_device_count", "lekf_version", "lekf_last_error", "lekf_dev_malloc", "lekf_dev_free",
     "lekf_memcpy", "lekf_memset", "lekf_stream_create", "lekf_stream_destroy", "lekf_stream_sync",

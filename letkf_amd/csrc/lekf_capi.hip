@@ -433,6 +433,10 @@ int lekf_tune_set(int param, long value)
         if (value < 0) return LEKF_ERR_ARG;
         g_lekf_tune.split_rcap = value;
         break;
+    case LEKF_TUNE_MIXED_KAPPA:
+        if (value < 0 || value > 1000000000000L) return LEKF_ERR_ARG;
+        g_lekf_tune.mixed_kappa = value;
+        break;
     default: return LEKF_ERR_ARG;
     }
     g_tune_epoch.fetch_add(1);
@@ -504,6 +508,7 @@ long lekf_tune_get(int param)
     case LEKF_TUNE_SEL_PER_CU: return g_lekf_tune.sel_per_cu;
     case LEKF_TUNE_EIG_SPLIT: return g_lekf_tune.eig_split;
     case LEKF_TUNE_SPLIT_RCAP: return g_lekf_tune.split_rcap;
+    case LEKF_TUNE_MIXED_KAPPA: return g_lekf_tune.mixed_kappa;
     default: return -LEKF_ERR_ARG;
     }
 }
@@ -2376,6 +2381,19 @@ int lekf_get_flags(lekf_handle* h, long* n_rankdef, long* n_noconv)
     }
     if (n_rankdef) *n_rankdef = a;
     if (n_noconv) *n_noconv = b;
+    return LEKF_OK;
+}
+
+int lekf_get_fp64_count(lekf_handle* h, long* n_fp64)
+{
+    if (!h || !n_fp64) return LEKF_ERR_ARG;
+    const size_t npt = (size_t)h->nij * h->nlev;
+    std::vector<int> s(npt);
+    HIPCHK(hipMemcpyAsync(s.data(), h->status.p, sizeof(int) * npt, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    long a = 0;
+    for (int v : s) a += (v & LEKF_PT_FP64) ? 1 : 0;
+    *n_fp64 = a;
     return LEKF_OK;
 }
 

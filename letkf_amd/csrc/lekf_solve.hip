@@ -799,14 +799,25 @@ __host__ __device__ inline long gram_stride(int kmax)
     return ((long)kmax * kmax + kmax + 2 + 1) & ~1L;
 }
 
+// split-record flag: the mixed-precision pass found C ill-conditioned and put
+// the point on the redo list, whose points a second pass solves again from an
+// FP64 Gram after the launch's last sub-batch
+constexpr int kSplitRedo = 16;
+
 // W waves per point; wave w takes every W-th chunk of 64 local obs, and the
 // partial Grams are summed in LDS in wave order (deterministic for a given W).
 template <int NT, int MODE, int W>
 __global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
-    const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride)
+    const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride,
+    const int* __restrict__ rlist)
 {
     extern __shared__ double smem[];
-    const long gp = gp0 + blockIdx.x;
+    const long slot = blockIdx.x;
+    long gp = gp0 + slot;
+    if (rlist) {  // FP64 pass of a split launch: slot i holds the redo list's i-th point
+        if (slot >= rlist[0]) return;
+        gp = rlist[1 + slot];
+    }
     const int tid = threadIdx.x;
     const int lane = tid & (LEKF_WAVE - 1), wv = tid / LEKF_WAVE;
     const int ji = lekf_job_of(jobs, njobs, gp);
@@ -853,7 +864,7 @@ __global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
     p3 = wave_sum(p3);
     if (lane == 0) sc[4 + wv] = p3;
     __syncthreads();
-    double* out = ws + (gp - gp0) * gstride;
+    double* out = ws + slot * gstride;
     for (int t = tid; t < k * k; t += 64 * W) {
         const int i = t / k, j = t - i * k;
         out[t] = A[i * lda + j];
@@ -2660,6 +2671,29 @@ struct SplitRec {
 };
 constexpr int kSplitOverflow = 8;  // flag: the stream overflowed (the point is solved again in full)
 
+// Flat point of slot `slot` of a split kernel: pass 0 the sub-batch's own
+// (gp0 + slot, nb slots), pass 1 the redo list's (rlist[0] entries from
+// rlist[1], of which the first nb were written: the count goes on past the
+// list's capacity); false past the end.
+__device__ inline bool split_slot_point(long slot, long gp0, long nb, int pass,
+                                        const int* rlist, long& gp)
+{
+    if (pass) {
+        if (slot >= nb || slot >= rlist[0]) return false;
+        gp = rlist[1 + slot];
+        return true;
+    }
+    gp = gp0 + slot;
+    return slot < nb;
+}
+
+// Whether a split kernel of pass `pass` (0: the launch's own precision, 1: the
+// FP64 redo) leaves a point with local obs to the other pass.
+__device__ inline bool split_other_pass(const double* rec, long flagoff, int pass)
+{
+    return ((((int)rec[flagoff]) & kSplitRedo) != 0) != (pass != 0);
+}
+
 __host__ __device__ inline long split_qoff(int kmax) { return gram_stride(kmax); }
 __host__ __device__ inline long split_stride(int kmax)
 {
@@ -2672,7 +2706,7 @@ __host__ __device__ inline long split_rcap(int kmax) { return 2L * kmax * kmax +
 __host__ __device__ inline long split_ridx_stride(int kmax) { return (split_rcap(kmax) + 3) & ~3L; }
 __host__ __device__ inline long split_ridx_doubles(int kmax) { return split_ridx_stride(kmax) / 4; }
 
-__device__ inline SplitRec split_rec(int kmax, const double* rot)
+__host__ __device__ inline SplitRec split_rec(int kmax, const double* rot)
 {
     SplitRec S;
     S.qoff = split_qoff(kmax);
@@ -2963,16 +2997,20 @@ void lekf_eig_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
 // writes Q (transposed), d and e to the point's record.
 __global__ __launch_bounds__(64) void lekf_tri_kernel(const LekfJob* __restrict__ jobs, int njobs,
                                                        long gp0, double* __restrict__ ws,
-                                                       long gstride, int kmax)
+                                                       long gstride, int kmax, int pass,
+                                                       int* __restrict__ rlist, int zero)
 {
     extern __shared__ double smem[];
-    const long gp = gp0 + blockIdx.x;
+    // the launch's first sub-batch empties the redo list (before its QL kernel)
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) rlist[0] = 0;
+    long gp;
+    if (!split_slot_point(blockIdx.x, gp0, (long)gridDim.x, pass, rlist, gp)) return;
     const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
     const int b = (int)(gp - J.pt_begin + J.pt0);
     const int k = J.k, lane = threadIdx.x, lda = eig_lda(k);
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
     if (p <= 0) return;
-    double* rec = ws + (gp - gp0) * gstride;
+    double* rec = ws + (long)blockIdx.x * gstride;
     const SplitRec S = split_rec(kmax, nullptr);
     EigLds L = eig_carve_tri(smem, k);
     double alpha = J.infl_in ? J.infl_in[b] : J.infl_fixed;
@@ -3008,13 +3046,14 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
                                                       long gp0, long nb, double* __restrict__ ws,
                                                       long gstride, double* __restrict__ rot,
                                                       short* __restrict__ ridx, int kmax,
-                                                      long rlim)
+                                                      long rlim, int pass, double kthr,
+                                                      int* __restrict__ rlist, long lcap)
 {
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
-    const long pt = (long)blockIdx.x * 64 + lane;
-    if (pt >= nb) return;  // no barrier below
-    const long gp = gp0 + pt;
+    const long pt = (long)blockIdx.x * 64 + lane;  // the slot
+    long gp;
+    if (!split_slot_point(pt, gp0, nb, pass, rlist, gp)) return;  // no barrier below
     const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
     const int b = (int)(gp - J.pt_begin + J.pt0);
     const int n = J.k;
@@ -3093,9 +3132,27 @@ __global__ __launch_bounds__(64) void lekf_ql_kernel(const LekfJob* __restrict__
             }
         }
     }
-    for (int i = 0; i < n; ++i) rec[S.lamoff + i] = d[i * 64];
+    double lmin = d[0], lmax = d[0];
+    for (int i = 0; i < n; ++i) {
+        rec[S.lamoff + i] = d[i * 64];
+        lmin = fmin(lmin, d[i * 64]);
+        lmax = fmax(lmax, d[i * 64]);
+    }
+    // pass 0 of a mixed launch (kthr > 0): a point whose C + (k-1)/alpha I has
+    // condition number above kthr goes on the redo list (while it has room: a
+    // point past lcap keeps this pass's solution and no FP64 status), and its
+    // flag tells this pass's later kernels to leave it; the FP64 pass's flag
+    // keeps the bit so that its own later kernels find the point
+    int redo = pass ? kSplitRedo : 0;
+    if (!pass && kthr > 0.0 && !bad && J.hdxf && lmin > 0.0 && lmax > kthr * lmin) {
+        const int pos = atomicAdd(rlist, 1);
+        if (pos < lcap) {
+            rlist[1 + pos] = (int)gp;
+            redo = kSplitRedo;
+        }
+    }
     rec[S.cntoff] = (double)(nr < rcap ? nr : rcap);
-    rec[S.flagoff] = (double)(bad | (nr > rcap ? kSplitOverflow : 0));
+    rec[S.flagoff] = (double)(bad | (nr > rcap ? kSplitOverflow : 0) | redo);
 }
 
 // Kernel 3 for jobs that output trans or pao (one wave per point): the
@@ -3105,15 +3162,22 @@ template <int KR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(eig_waves_per_simd(KR))))
 void lekf_eig3_kernel(const LekfJob* __restrict__ jobs, int njobs, long gp0,
                       double* __restrict__ ws, long gstride, int nvmax, int kmax,
-                      const double* __restrict__ rot)
+                      const double* __restrict__ rot, int pass, const int* __restrict__ rlist)
 {
     extern __shared__ double smem[];
-    const long gp = gp0 + blockIdx.x;
+    const long slot = blockIdx.x;
+    long gp;
+    if (!split_slot_point(slot, gp0, (long)gridDim.x, pass, rlist, gp)) return;
     const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
     if (!J.trans && !J.pao) return;  // lekf_evf_kernel's
-    const SplitRec S = split_rec(kmax, rot + (gp - gp0) * 2 * split_rcap(kmax));
-    eig_point<KR, 0, true>(J, (int)(gp - J.pt_begin + J.pt0), ws + (gp - gp0) * gstride, gstride,
-                           nvmax, 0L, smem, threadIdx.x, &S);
+    const int b = (int)(gp - J.pt_begin + J.pt0);
+    const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
+    double* rec = ws + slot * gstride;
+    const SplitRec S = split_rec(kmax, rot + slot * 2 * split_rcap(kmax));
+    // no local obs: pass 0 writes the outputs; else the pass the flag names
+    if (p <= 0 ? pass != 0 : split_other_pass(rec, S.flagoff, pass)) return;
+    eig_point<KR, 0, true>(J, b, rec, gstride, nvmax, 0L, smem, threadIdx.x, &S);
+    if (pass && threadIdx.x == 0 && J.status) J.status[b] |= LEKF_PT_FP64;
 }
 
 // Kernel 3 for the other jobs (no trans or pao output): the analysis without
@@ -3209,13 +3273,14 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
                                                        const double* __restrict__ ws, long gstride,
                                                        const double* __restrict__ rot,
                                                        const short* __restrict__ ridx, int nvmax,
-                                                       int kmax)
+                                                       int kmax, int pass,
+                                                       const int* __restrict__ rlist)
 {
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
     const int G = nvmax + 1, P = 64 / G;
     const int grp = lane / G, v = lane - grp * G;
-    const long pt = (long)blockIdx.x * P + grp;
+    const long pt = (long)blockIdx.x * P + grp;  // the slot
     const int ld = kmax | 1;
     double* row = sm + (size_t)lane * ld;
     const long rstride = split_rcap(kmax);
@@ -3225,13 +3290,16 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
     const LekfJob* Jp = nullptr;
     int b = 0, k = 0, p = 0, nrow = 0, n3 = 0, lev = 0, ij = 0;
     bool work = false;  // a point this kernel solves
-    if (grp < P && pt < nb) {
-        const long gp = gp0 + pt;
+    long gp = 0;
+    if (grp < P && split_slot_point(pt, gp0, nb, pass, rlist, gp)) {
         Jp = &jobs[lekf_job_of(jobs, njobs, gp)];
         b = (int)(gp - Jp->pt_begin + Jp->pt0);
         k = Jp->k;
         p = Jp->loc_cnt ? Jp->loc_cnt[b] : (int)(Jp->loc_off[b + 1] - Jp->loc_off[b]);
         work = !Jp->trans && !Jp->pao;
+        // no local obs: pass 0 writes the outputs; else the pass the flag names
+        if (work && (p <= 0 ? pass != 0 : split_other_pass(ws + pt * gstride, S.flagoff, pass)))
+            work = false;
         if (work && p > 0 && (((int)ws[pt * gstride + S.flagoff]) & kSplitOverflow)) work = false;
         if ((Jp->flags & LEKF_F_APPLY)) {
             lev = b / Jp->nij;
@@ -3419,7 +3487,7 @@ __global__ __launch_bounds__(64) void lekf_evf_kernel(const LekfJob* __restrict_
                 int status = (((int)rec[S.flagoff]) & 1) ? LEKF_PT_NOCONV : 0;
                 if (!(lmax > 0.0) || lmin < fabs(lmax) * 1.4901161193847656e-08)
                     status |= LEKF_PT_RANKDEF;
-                J.status[b] = status;
+                J.status[b] = status | (pass ? LEKF_PT_FP64 : 0);
             }
         }
     }
@@ -3432,17 +3500,22 @@ template <int KR>
 __global__ __launch_bounds__(64) void lekf_eig_fix_kernel(const LekfJob* __restrict__ jobs,
                                                            int njobs, long gp0,
                                                            double* __restrict__ ws, long gstride,
-                                                           int nvmax, int kmax)
+                                                           int nvmax, int kmax, int pass,
+                                                           const int* __restrict__ rlist)
 {
     extern __shared__ double smem[];
-    const long gp = gp0 + blockIdx.x;
+    long gp;
+    if (!split_slot_point(blockIdx.x, gp0, (long)gridDim.x, pass, rlist, gp)) return;
     const LekfJob& J = jobs[lekf_job_of(jobs, njobs, gp)];
     const int b = (int)(gp - J.pt_begin + J.pt0);
     const int p = J.loc_cnt ? J.loc_cnt[b] : (int)(J.loc_off[b + 1] - J.loc_off[b]);
-    double* rec = ws + (gp - gp0) * gstride;
+    double* rec = ws + (long)blockIdx.x * gstride;
     const SplitRec S = split_rec(kmax, nullptr);
-    if (p <= 0 || !(((int)rec[S.flagoff]) & kSplitOverflow)) return;
+    if (p <= 0 || split_other_pass(rec, S.flagoff, pass) ||
+        !(((int)rec[S.flagoff]) & kSplitOverflow))
+        return;
     eig_point<KR, 0>(J, b, rec, gstride, nvmax, 0L, smem, threadIdx.x);
+    if (pass && threadIdx.x == 0 && J.status) J.status[b] |= LEKF_PT_FP64;
 }
 
 // LEKF_KSPLIT < k: one persistent wave per workgroup slot strides over the
@@ -3518,8 +3591,8 @@ static bool split_path(int kmax)
 static size_t ws_rec_bytes(int kmax)
 {
     if (kmax > LEKF_KSPLIT) return (size_t)big_stride(kmax) * sizeof(double);
-    if (split_path(kmax))
-        return (size_t)(split_stride(kmax) + 2 * split_rcap(kmax) + split_ridx_doubles(kmax)) *
+    if (split_path(kmax))  // record, rotation stream, its columns, a redo-list entry
+        return (size_t)(split_stride(kmax) + 2 * split_rcap(kmax) + split_ridx_doubles(kmax) + 1) *
                sizeof(double);
     return (size_t)gram_stride(kmax) * sizeof(double);
 }
@@ -3579,7 +3652,8 @@ static hipError_t launch_eig(int kr, long nb, size_t elds, hipStream_t st, const
 
 static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
                                    const LekfJob* djobs, int njobs, long g0, double* ws, long gs,
-                                   int nvmax, int kmax, double* rot, short* ridx)
+                                   int nvmax, int kmax, double* rot, short* ridx, int pass,
+                                   double kthr, int* rlist, int zero, long lcap)
 {
     const unsigned nwq = (unsigned)((nb + 63) / 64);
     const int evp = 64 / (nvmax + 1);  // points per wave of the eigenvector-free kernel
@@ -3591,23 +3665,23 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
 #define LEKF_SPLIT_CASE(K)                                                                       \
     case K: {                                                                                    \
         {                                                                                        \
-            KTime kt("eig_tri", st);                                                             \
+            KTime kt(pass ? "redo_tri" : "eig_tri", st);                                         \
             hipLaunchKernelGGL(lekf_tri_kernel, dim3((unsigned)nb), dim3(LEKF_WAVE), tlds, st,    \
-                               djobs, njobs, g0, ws, gs, kmax);                                 \
+                               djobs, njobs, g0, ws, gs, kmax, pass, rlist, zero);              \
         }                                                                                        \
         {                                                                                        \
-            KTime kt("eig_ql", st);                                                              \
+            KTime kt(pass ? "redo_ql" : "eig_ql", st);                                           \
             hipLaunchKernelGGL(lekf_ql_kernel<K>, dim3(nwq), dim3(LEKF_WAVE), qlds, st, djobs,    \
                                njobs, g0, nb, ws, gs, rot, ridx, kmax,                           \
-                               g_lekf_tune.split_rcap);                                         \
+                               g_lekf_tune.split_rcap, pass, kthr, rlist, lcap);                \
         }                                                                                        \
-        KTime kt("eig_apply", st);                                                               \
+        KTime kt(pass ? "redo_apply" : "eig_apply", st);                                         \
         hipLaunchKernelGGL(lekf_eig3_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st,    \
-                           djobs, njobs, g0, ws, gs, nvmax, kmax, rot);                         \
+                           djobs, njobs, g0, ws, gs, nvmax, kmax, rot, pass, rlist);            \
         hipLaunchKernelGGL(lekf_evf_kernel<K>, dim3(nwe), dim3(LEKF_WAVE), flds, st, djobs,      \
-                           njobs, g0, nb, ws, gs, rot, ridx, nvmax, kmax);                      \
+                           njobs, g0, nb, ws, gs, rot, ridx, nvmax, kmax, pass, rlist);         \
         hipLaunchKernelGGL(lekf_eig_fix_kernel<K>, dim3((unsigned)nb), dim3(LEKF_WAVE), elds, st, \
-                           djobs, njobs, g0, ws, gs, nvmax, kmax);                              \
+                           djobs, njobs, g0, ws, gs, nvmax, kmax, pass, rlist);                 \
         break;                                                                                   \
     }
         LEKF_SPLIT_CASE(36)
@@ -3625,6 +3699,23 @@ static hipError_t launch_eig_split(int kr, long nb, size_t elds, hipStream_t st,
 }
 
 template <int NT, int MODE>
+static hipError_t launch_gram(int W, long nb, size_t glds, hipStream_t st, const LekfJob* djobs,
+                              int njobs, long g0, double* ws, long gs, const int* rlist)
+{
+    KTime kt(rlist ? "redo_gram" : "gram", st);
+    if (W == 2)
+        hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 2>), dim3((unsigned)nb), dim3(128), glds, st,
+                           djobs, njobs, g0, ws, gs, rlist);
+    else if (W == 8)
+        hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 8>), dim3((unsigned)nb), dim3(512), glds, st,
+                           djobs, njobs, g0, ws, gs, rlist);
+    else
+        hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 4>), dim3((unsigned)nb), dim3(256), glds, st,
+                           djobs, njobs, g0, ws, gs, rlist);
+    return hipGetLastError();
+}
+
+template <int NT, int MODE>
 static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int kmax, int nvmax,
                                void* ws, size_t ws_bytes, hipStream_t st)
 {
@@ -3634,30 +3725,38 @@ static hipError_t launch_split(const LekfJob* djobs, int njobs, long npt, int km
     if (per < 1 || !ws) return hipErrorInvalidValue;
     double* rot = (double*)ws + (size_t)per * gs;  // the rotation streams after the records
     short* ridx = (short*)(rot + (size_t)per * 2 * split_rcap(kmax));  // then their columns
+    int* rlist = (int*)(ridx + (size_t)per * split_ridx_stride(kmax));  // then the redo list
     const int W = g_lekf_tune.gram_waves;
     const size_t glds = ((size_t)kmax * (kmax + 1) + kmax + 4 + W) * sizeof(double);
     const size_t elds = eig_lds_doubles(kmax, nvmax) * sizeof(double);
+    // mixed forms on the split eigensolver: the QL kernel lists the points whose
+    // C is worse conditioned than kthr (up to one sub-batch of them per launch),
+    // and after the last sub-batch one FP64 pass solves the listed points again
+    // (one set of kernel latencies per launch, whatever the sub-batch count).
+    // The FP32 Gram rounds about 16 times finer than split-BF16x3, so its
+    // threshold is 16 times the value set.
+    const double kthr = (MODE != 0 && s3 && npt <= 0x7fffffffL)
+                            ? (MODE == 1 ? 16.0 : 1.0) * (double)g_lekf_tune.mixed_kappa
+                            : 0.0;
     for (long g0 = 0; g0 < npt; g0 += per) {
         const long nb = per < npt - g0 ? per : npt - g0;
-        {
-            KTime kt("gram", st);
-            if (W == 2)
-                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 2>), dim3((unsigned)nb), dim3(128),
-                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
-            else if (W == 8)
-                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 8>), dim3((unsigned)nb), dim3(512),
-                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
-            else
-                hipLaunchKernelGGL((lekf_gram_kernel<NT, MODE, 4>), dim3((unsigned)nb), dim3(256),
-                                   glds, st, djobs, njobs, g0, (double*)ws, gs);
-        }
-        hipError_t e = hipGetLastError();
+        hipError_t e =
+            launch_gram<NT, MODE>(W, nb, glds, st, djobs, njobs, g0, (double*)ws, gs, nullptr);
         if (e != hipSuccess) return e;
         if (s3)
             e = launch_eig_split(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs,
-                                 nvmax, kmax, rot, ridx);
+                                 nvmax, kmax, rot, ridx, 0, kthr, rlist, g0 == 0, per);
         else
             e = launch_eig(eig_kr(kmax), nb, elds, st, djobs, njobs, g0, (double*)ws, gs, nvmax);
+        if (e != hipSuccess) return e;
+    }
+    if (kthr > 0.0) {
+        // grids of one sub-batch; the slots past the list's length return at once
+        const long nl = per < npt ? per : npt;
+        hipError_t e = launch_gram<NT, 0>(W, nl, glds, st, djobs, njobs, 0, (double*)ws, gs, rlist);
+        if (e == hipSuccess)
+            e = launch_eig_split(eig_kr(kmax), nl, elds, st, djobs, njobs, 0, (double*)ws, gs, nvmax,
+                                 kmax, rot, ridx, 1, 0.0, rlist, 0, per);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

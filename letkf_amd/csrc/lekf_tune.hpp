@@ -10,6 +10,8 @@ struct LekfTune {
     int eig_split = 1;                  // 32 < k <= 64: three-kernel eigensolver (1) or fused (0)
     long split_rcap = 0;                // rotations a split stream holds before its point is
                                         // solved in full by the fallback (0: 2k^2 + 8k)
+    long mixed_kappa = 500;             // mixed forms, split eigensolver: condition number of
+                                        // C above which a point is redone in FP64 (0: never)
 };
 
 extern LekfTune g_lekf_tune;

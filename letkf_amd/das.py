@@ -114,6 +114,13 @@ class _Handle:
         check(_lib.lib().lekf_get_flags(self.h, ctypes.byref(a), ctypes.byref(b)), "get_flags")
         return a.value, b.value
 
+    def fp64_count(self):
+        """Points of the last run that a mixed form solved again in FP64
+        (their C was worse conditioned than the mixed_kappa tuning value)."""
+        n = ctypes.c_long()
+        check(_lib.lib().lekf_get_fp64_count(self.h, ctypes.byref(n)), "get_fp64_count")
+        return n.value
+
     def cycle_swap(self):
         check(_lib.lib().lekf_cycle_swap(self.h), "cycle_swap")
 

@@ -137,7 +137,7 @@ def mem_info():
 
 
 KTIME_NAMES = ("select", "list_cap", "cull", "solve", "gram", "eig", "eig_tri", "eig_ql",
-               "eig_apply", "big")
+               "eig_apply", "big", "redo_gram", "redo_tri", "redo_ql", "redo_apply")
 
 
 def ktime_enable(on=True):

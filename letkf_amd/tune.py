@@ -38,8 +38,10 @@ PARAMS = {
 
 # knobs that are set by hand, never searched: the rotations a point's QL stream
 # may hold in the three-kernel eigensolver before the point takes the fused
-# kernel's steps instead (0: the default 2k^2 + 8k)
-FIXED = {"split_rcap": _lib.LEKF_TUNE_SPLIT_RCAP}
+# kernel's steps instead (0: the default 2k^2 + 8k), and the condition number
+# above which a mixed-precision point is solved again in FP64 (an accuracy
+# choice, not a speed one)
+FIXED = {"split_rcap": _lib.LEKF_TUNE_SPLIT_RCAP, "mixed_kappa": _lib.LEKF_TUNE_MIXED_KAPPA}
 
 
 def _pid(name):

@@ -95,3 +95,19 @@ def test_analysis_error_follows_the_condition_number():
     assert c1 > 10 * c0                        # collinear rows: C ill-conditioned
     assert e1 > 30 * e0                        # and the analysis error follows
     assert max(x[2] for x in coll) < 3e-6      # while the Gram's own error does not
+
+
+def test_fp64_redo_threshold():
+    """The default mixed_kappa threshold (500): collinear lists scaled so that C's
+    condition number runs from about 15 to 15000; every case at or below the
+    threshold stays within the 1e-4 budget, and the cases well above it (which
+    the FP64 redo takes) do not."""
+    rng = np.random.default_rng(7)
+    scale, err = _kinds(P, True)
+    cases = [_case(rng, scale * f, err, modes=6) for f in (0.03, 0.1, 0.2, 0.5, 1.0)
+             for _ in range(2)]
+    below = [e for e, c, _ in cases if c <= 500]
+    above = [e for e, c, _ in cases if c > 2000]
+    assert len(below) >= 4 and len(above) >= 3
+    assert max(below) < 1e-4
+    assert min(above) > 1e-4
