@@ -650,7 +650,8 @@ def main():
                     help="skip the SPEEDY, GFS, k=256 and served-stream measurements")
     ap.add_argument("--serve-s", type=float, default=60.0, help="served-stream duration")
     ap.add_argument("--only", choices=["speedy", "gfs", "k256"],
-                    help="run one of the other configurations alone and print its JSON")
+                    help="run one of the other configurations alone (with --steps, --precs "
+                         "and, for speedy, --warmup) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the LETKF local analyses of a job run on a single "
@@ -670,7 +671,10 @@ def main():
         copy_calibration()
         return
     if args.only:
-        r = {"speedy": speedy_bench, "gfs": gfs_bench, "k256": speedy_k256_bench}[args.only]()
+        kw = {"steps": args.steps, "precs": args.precs}
+        if args.only == "speedy":
+            kw["warmup"] = args.warmup
+        r = {"speedy": speedy_bench, "gfs": gfs_bench, "k256": speedy_k256_bench}[args.only](**kw)
         print(json.dumps(r), flush=True)
         return
     if args.profile:
