@@ -14,6 +14,8 @@
 typedef double lekf_d4 __attribute__((ext_vector_type(4)));
 typedef float lekf_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 lekf_bf8 __attribute__((ext_vector_type(8)));
+typedef float lekf_f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 lekf_b2v __attribute__((ext_vector_type(2)));
 
 // speedy/common/common_obs_speedy.f90:17-24
 #define LEKF_ID_U 2819

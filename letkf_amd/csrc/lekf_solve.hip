@@ -233,6 +233,9 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
     for (int q0 = 64 * wv; q0 < p; q0 += 64 * nw) {
         const ObsChunk ch = stage_chunk(J, off, p, q0, lane, wloc, p3);
+        // the weight and departure in the precision the products use, so each
+        // crosses lanes as one dword
+        const float wf = (float)ch.w, df = (float)ch.d;
         for (int h = 0; h < 64 && q0 + h < p; h += 32) {
             lekf_bf8 ah[NT], al[NT], bh[NT], bl[NT];
             float wv8[8], dv8[8];
@@ -241,28 +244,41 @@ __device__ __forceinline__ void gram_bf16x3(const LekfJob& J, long off, int p, i
             for (int e = 0; e < 8; ++e) {
                 const int src = h + 8 * kq + e;
                 ov[e] = __shfl(ch.o, src, 64);
-                wv8[e] = (float)__shfl(ch.w, src, 64);
-                dv8[e] = (float)__shfl(ch.d, src, 64);
+                wv8[e] = __shfl(wf, src, 64);
+                dv8[e] = __shfl(df, src, 64);
             }
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const int c = 16 * t + lr;
+                float x[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    float x = 0.0f;
+                    x[e] = 0.0f;
                     if (c < k)
-                        x = J.hdxf32[(long)ov[e] * k + c];
+                        x[e] = J.hdxf32[(long)ov[e] * k + c];
                     else if (c == k)
-                        x = dv8[e];
-                    const float xa = wv8[e] * x;
-                    const __bf16 bh1 = (__bf16)x;
-                    const __bf16 bl1 = (__bf16)(x - (float)bh1);
-                    const __bf16 ah1 = (__bf16)xa;
-                    const __bf16 al1 = (__bf16)(xa - (float)ah1);
-                    bh[t][e] = bh1;
-                    bl[t][e] = bl1;
-                    ah[t][e] = ah1;
-                    al[t][e] = al1;
+                        x[e] = dv8[e];
+                }
+                // two elements per rounding instruction (round to nearest even,
+                // as the scalar conversion does)
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    const lekf_f2v xv = {x[e], x[e + 1]};
+                    const lekf_f2v xa = {wv8[e] * x[e], wv8[e + 1] * x[e + 1]};
+                    const lekf_b2v hb = __builtin_convertvector(xv, lekf_b2v);
+                    const lekf_b2v ha = __builtin_convertvector(xa, lekf_b2v);
+                    const lekf_f2v rb = {x[e] - (float)hb.x, x[e + 1] - (float)hb.y};
+                    const lekf_f2v ra = {xa.x - (float)ha.x, xa.y - (float)ha.y};
+                    const lekf_b2v lb = __builtin_convertvector(rb, lekf_b2v);
+                    const lekf_b2v la = __builtin_convertvector(ra, lekf_b2v);
+                    bh[t][e] = hb.x;
+                    bh[t][e + 1] = hb.y;
+                    bl[t][e] = lb.x;
+                    bl[t][e + 1] = lb.y;
+                    ah[t][e] = ha.x;
+                    ah[t][e + 1] = ha.y;
+                    al[t][e] = la.x;
+                    al[t][e + 1] = la.y;
                 }
             }
             int a = 0;
