@@ -184,6 +184,7 @@ __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int 
     const bool wloc = (J.flags & LEKF_F_WLOC) != 0;
     for (int q0 = 64 * wv; q0 < p; q0 += 64 * nw) {
         const ObsChunk ch = stage_chunk(J, off, p, q0, lane, wloc, p3);
+        const float wf = (float)ch.w, df = (float)ch.d;  // one dword each across lanes
 #pragma unroll 1
         for (int h = 0; h < 16 && q0 + 4 * h < p; h += SUB) {
             float v[SUB][NT], wt[SUB];
@@ -191,8 +192,8 @@ __device__ __forceinline__ void gram_f32(const LekfJob& J, long off, int p, int 
             for (int st = 0; st < SUB; ++st) {
                 const int src = 4 * (h + st) + kq;
                 const int o = __shfl(ch.o, src, 64);
-                const float w = (float)__shfl(ch.w, src, 64);
-                const float dd = (float)__shfl(ch.d, src, 64);
+                const float w = __shfl(wf, src, 64);
+                const float dd = __shfl(df, src, 64);
                 wt[st] = w;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
