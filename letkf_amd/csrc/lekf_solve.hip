@@ -1562,15 +1562,36 @@ __device__ void eig_tridiag64(EigLds& L, int n, int lda, int lane)
         if (lane < m) wb[lane] = w0;
         __syncthreads();
         // A22 -= v w^T + w v^T (lane r updates column r; both products are
-        // rounded, then added, so (c, r) and (r, c) stay equal)
+        // rounded, then added, so (c, r) and (r, c) stay equal).  A group's
+        // loads are all issued before its stores: v, w and the column share
+        // LDS, so element by element each load would wait for the store before
         if (lane < m) {
             double* col = A + o * lda + o + lane;
             int c = 0;
-            for (; c + 4 <= m; c += 4) {
+            for (; c + 8 <= m; c += 8) {
+                double cv[8], vv[8], ww[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cv[u] = col[(c + u) * lda];
+                    vv[u] = vb[c + u];
+                    ww[u] = wb[c + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    col[(c + u) * lda] = rn_sub(cv[u], rn_add(rn_mul(vv[u], w0), rn_mul(ww[u], v0)));
+            }
+            if (c + 4 <= m) {
+                double cv[4], vv[4], ww[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cv[u] = col[(c + u) * lda];
+                    vv[u] = vb[c + u];
+                    ww[u] = wb[c + u];
+                }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    col[(c + u) * lda] = rn_sub(col[(c + u) * lda],
-                                                rn_add(rn_mul(vb[c + u], w0), rn_mul(wb[c + u], v0)));
+                    col[(c + u) * lda] = rn_sub(cv[u], rn_add(rn_mul(vv[u], w0), rn_mul(ww[u], v0)));
+                c += 4;
             }
             for (; c < m; ++c)
                 col[c * lda] = rn_sub(col[c * lda], rn_add(rn_mul(vb[c], w0), rn_mul(wb[c], v0)));
@@ -1616,10 +1637,27 @@ __device__ void eig_tridiag64(EigLds& L, int n, int lda, int lane)
             for (; c < m; ++c) y0 = fma(vb[c], col[c * lda], y0);
             const double y = tau * ((y0 + y1) + (y2 + y3));
             c = 0;
-            for (; c + 4 <= m; c += 4) {
+            // loads of a group before its stores (v and Q share LDS)
+            for (; c + 8 <= m; c += 8) {
+                double cv[8], vv[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    col[(c + u) * lda] = fma(-vb[c + u], y, col[(c + u) * lda]);
+                for (int u = 0; u < 8; ++u) {
+                    cv[u] = col[(c + u) * lda];
+                    vv[u] = vb[c + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) col[(c + u) * lda] = fma(-vv[u], y, cv[u]);
+            }
+            if (c + 4 <= m) {
+                double cv[4], vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cv[u] = col[(c + u) * lda];
+                    vv[u] = vb[c + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) col[(c + u) * lda] = fma(-vv[u], y, cv[u]);
+                c += 4;
             }
             for (; c < m; ++c) col[c * lda] = fma(-vb[c], y, col[c * lda]);
         }
