@@ -821,10 +821,16 @@ __host__ __device__ inline long gram_stride(int kmax)
 // FP64 Gram after the launch's last sub-batch
 constexpr int kSplitRedo = 16;
 
+// Waves per SIMD asked of the Gram kernel: the FP64 form with 4 x 4 tiles
+// (49 <= k <= 64) takes 214 registers and so two waves per SIMD unless held
+// to three's budget
+constexpr int gram_waves_per_simd(int nt, int mode) { return (nt == 4 && mode == 0) ? 3 : 1; }
+
 // W waves per point; wave w takes every W-th chunk of 64 local obs, and the
 // partial Grams are summed in LDS in wave order (deterministic for a given W).
 template <int NT, int MODE, int W>
-__global__ __launch_bounds__(64 * W) void lekf_gram_kernel(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(gram_waves_per_simd(NT, MODE))))
+void lekf_gram_kernel(
     const LekfJob* __restrict__ jobs, int njobs, long gp0, double* __restrict__ ws, long gstride,
     const int* __restrict__ rlist)
 {
