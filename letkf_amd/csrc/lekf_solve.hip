@@ -824,7 +824,10 @@ constexpr int kSplitRedo = 16;
 // Waves per SIMD asked of the Gram kernel: the FP64 and FP32 forms with
 // 4 x 4 tiles (49 <= k <= 64) take 214 and 188 registers, two waves per
 // SIMD, unless held to three's budget (which they fit without spilling)
-constexpr int gram_waves_per_simd(int nt, int mode) { return (nt == 4 && mode != 2) ? 3 : 1; }
+constexpr int gram_waves_per_simd(int nt, int mode)
+{
+    return (nt == 4 && mode != 2) ? 3 : (nt == 3 && mode != 2) ? 4 : 1;
+}
 
 // W waves per point; wave w takes every W-th chunk of 64 local obs, and the
 // partial Grams are summed in LDS in wave order (deterministic for a given W).
