@@ -175,3 +175,43 @@ def test_radar_redo_list_overflow(gpu, knobs, radar40):
     eqm = (a == amix).all(axis=1)
     assert (eq64 | eqm).all()
     assert int((eq64 & ~eqm).sum()) == n
+
+
+def test_redo_in_graph_replay(gpu, knobs, radar40):
+    """The FP64 pass captured in the cycle's HIP graph (the list emptied by a
+    kernel, not a memset): every replay equals the eager cycle bit for bit,
+    with the same points redone (about 40% of them at threshold 30)."""
+    p = radar40
+
+    def make():
+        h = das.RegionalLetkf(nx=p["nx"], ny=p["ny"], zlev=p["zlev"], dx=p["dx"], dy=p["dy"],
+                              ne=p["ne"], nv3d=p["nv3d"], sigma_h=p["sigma_h"],
+                              sigma_v=p["sigma_v"], nobs_max=1000, infl_mode="fixed",
+                              infl_value=1.0, prec="bf16x3")
+        h.set_ensemble(p["gues3d"])
+        h.set_obs(p["ri"], p["rj"], p["z"], p["err"], p["dep"], p["bucket"], p["hdxf"])
+        return h
+
+    n0 = 0
+    for thr in (30, 10, 1):
+        tune.set("mixed_kappa", thr)
+        ref = make()
+        ref.run()
+        ref.sync()
+        a0, n0 = ref.analysis()[0], ref.fp64_count()
+        ref.close()
+        if n0 > 0:
+            break
+    print(f"threshold {thr}: {n0} points redone", flush=True)
+    assert n0 > 0
+    h = make()
+    h.set_graph(True)
+    h.run()                      # eager + capture
+    h.sync()
+    assert np.array_equal(h.analysis()[0], a0)
+    for _ in range(3):
+        h.run()                  # replay
+        h.sync()
+        assert np.array_equal(h.analysis()[0], a0)
+        assert h.fp64_count() == n0
+    h.close()
